@@ -1,0 +1,300 @@
+"""Pins the CPU oracle against known answers (CPU only).
+
+Known answers used, per oracle function:
+  make_mrg_seed           closed form of generator/graph_generator.hpp:327-333
+  transition table        entries of the reference's generated table (golden)
+  mrg_skip                composition and brute-force stepping (math)
+  generate_edges          reference tuple-stream SHA-256 (golden), pf_nedge
+  Csr                     reference counts (golden) + brute force on small graphs
+  find_roots              reference roots (golden)
+  bfs                     reference level-array SHA-256, directions, frontier
+                          sizes (golden) + brute-force BFS on small graphs
+  validate                pf_nedge (mpi/utils.hpp:3681) + one planted error per rule
+  statistics              direct formulas of mpi/logfile.h:19-44
+"""
+import collections
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+M = 2**31 - 1
+
+
+def gold(name):
+    return json.load(open(os.path.join(GOLD, name)))
+
+
+@pytest.fixture(scope="module")
+def s16(oracle_mod):
+    O = oracle_mod
+    e = O.generate_edges(16)
+    return e, O.Csr(e)
+
+
+def test_seed_expansion(oracle_mod):
+    assert list(oracle_mod.make_mrg_seed(2, 3)) == [3, 1, 4, 1, 1]
+    s1, s2 = 0x123456789ABCDEF0, 0x0FEDCBA987654321
+    z = oracle_mod.make_mrg_seed(s1, s2)
+    assert list(z) == [(s1 & 0x3FFFFFFF) + 1, ((s1 >> 30) & 0x3FFFFFFF) + 1, (s2 & 0x3FFFFFFF) + 1,
+                       ((s2 >> 30) & 0x3FFFFFFF) + 1, ((s2 >> 60) << 4) + (s1 >> 60) + 1]
+
+
+def test_transition_table_matches_reference_entries(oracle_mod):
+    g = gold("mrg_table.json")
+    for v in g["vectors"]:
+        got = oracle_mod.transition_entry(v["byte"], v["value"]).tolist()
+        assert got == v["entry"], (v["byte"], v["value"])
+
+
+def _py_step(z):
+    nz = (107374182 * z[0] + 104480 * z[4]) % M
+    return [nz, z[0], z[1], z[2], z[3]]
+
+
+def test_mrg_step_recurrence(oracle_mod):
+    z = [3, 1, 4, 1, 1]
+    cur = np.array(z, np.uint32)
+    for _ in range(50):
+        v, cur = oracle_mod.mrg_get_uint(cur)
+        z = _py_step(z)
+        assert v == z[0] and cur.tolist() == z
+
+
+def test_mrg_skip_brute_force_and_composition(oracle_mod):
+    O = oracle_mod
+    z0 = np.array([3, 1, 4, 1, 1], np.uint32)
+    # brute force: skip(k) == k plain steps
+    zz = [int(x) for x in z0]
+    for k in range(1, 700):
+        zz = _py_step(zz)
+        if k in (1, 2, 5, 255, 256, 257, 300, 511, 699):
+            assert O.mrg_skip(z0, 0, 0, k).tolist() == zz, k
+    # composition across all three 64-bit exponent words
+    a = (0x1, 0xDEADBEEF12345678, 0x0123456789ABCDEF)
+    b = (0x7, 0x00000000FFFFFFFF, 0xFEDCBA9876543210)
+    lo = a[2] + b[2]
+    mid = a[1] + b[1] + (lo >> 64)
+    hi = a[0] + b[0] + (mid >> 64)
+    both = O.mrg_skip(O.mrg_skip(z0, *a), *b)
+    assert both.tolist() == O.mrg_skip(z0, hi, mid & (2**64 - 1), lo & (2**64 - 1)).tolist()
+    # the period of the generator divides m^5 - 1: A^(m^5-1) = I
+    e = M**5 - 1
+    assert O.mrg_skip(z0, e >> 128, (e >> 64) & (2**64 - 1), e & (2**64 - 1)).tolist() == z0.tolist()
+
+
+def test_random_doubles_in_unit_interval(oracle_mod):
+    d = oracle_mod.make_random_numbers(1000, 2, 3, 0)
+    assert ((d >= 0) & (d < 1)).all()
+    # position p starts where the stream skipped by 2p draws starts
+    d2 = oracle_mod.make_random_numbers(10, 2, 3, 37)
+    assert np.array_equal(d[37:47], d2)
+
+
+def test_scramble_is_a_permutation(oracle_mod):
+    O = oracle_mod
+    v0, v1 = O.scramble_values(2, 3)
+    for scale in (4, 10):
+        img = [O.lib().or_scramble(i, scale, int(v0), int(v1)) for i in range(1 << scale)]
+        assert sorted(img) == list(range(1 << scale))
+
+
+def test_generator_matches_reference_stream_s16(oracle_mod, s16):
+    e, _ = s16
+    g = gold("ref_s16_16roots.json")
+    t = e.view(np.uint32).reshape(-1, 3)
+    assert len(t) == g["num_tuples"] == 16 << 16
+    assert t[:8].tolist() == g["tuples_head"] and t[-8:].tolist() == g["tuples_tail"]
+    assert hashlib.sha256(t.tobytes()).hexdigest() == g["tuples_sha256"]
+
+
+def test_generator_range_is_position_independent(oracle_mod):
+    O = oracle_mod
+    full = O.generate_edges(10)
+    part = O.generate_edges(10, start=1000, end=3000)
+    assert np.array_equal(full[1000:3000], part)
+
+
+def test_csr_counts_match_reference_s16(s16):
+    _, c = s16
+    g = gold("ref_s16_16roots.json")
+    assert c.scale_estimate == g["scale_estimate"] == 16
+    assert c.num_active == g["num_active"]
+    assert c.nnz == g["nnz"] == 1819202
+    assert c.directed_before == 2096170 and c.num_self_loops == 491
+
+
+def _brute_csr(v0, v1):
+    n = 1 << (int(np.bitwise_or.reduce(np.concatenate([v0, v1]))).bit_length())
+    deg = collections.Counter()
+    for a, b in zip(v0, v1):
+        if a != b:
+            deg[a] += 1
+            deg[b] += 1
+    order = sorted(range(n), key=lambda v: (-deg[v], v))
+    new = {o: i for i, o in enumerate(order)}
+    adj = collections.defaultdict(set)
+    for a, b in zip(v0, v1):
+        if a != b:
+            adj[new[a]].add(new[b])
+            adj[new[b]].add(new[a])
+    na = sum(1 for v in range(n) if deg[v] > 0)
+    offs, cols = [0], []
+    for v in range(na):
+        cols += sorted(adj[v])
+        offs.append(len(cols))
+    return order, na, offs, cols
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_csr_brute_force_small(oracle_mod, seed):
+    O = oracle_mod
+    rng = np.random.default_rng(seed)
+    m = 300
+    v0 = rng.integers(0, 200, m)
+    v1 = rng.integers(0, 200, m)
+    v1[:20] = v0[:20]  # self loops
+    v0[20:40] = v0[40:60]
+    v1[20:40] = v1[40:60]  # duplicates
+    c = O.Csr(O.pack_edges(v0, v1))
+    order, na, offs, cols = _brute_csr(v0, v1)
+    a = c.arrays()
+    assert c.num_active == na
+    assert a["new2old"].tolist() == order
+    assert a["offsets"].tolist() == offs
+    assert a["cols"].tolist() == cols
+
+
+def test_find_roots_match_reference(s16):
+    _, c = s16
+    assert c.find_roots(16).tolist() == gold("ref_s16_16roots.json")["roots"]
+    assert c.find_roots(64).tolist() == gold("ref_s16_64roots.json")["roots"]
+
+
+def test_bfs_matches_reference_levels_schedule_s16(oracle_mod, s16):
+    e, c = s16
+    g = gold("ref_s16_64roots.json")
+    for pr in g["per_root"]:
+        lv, pred, tr = c.bfs(pr["root"])
+        assert hashlib.sha256(lv.tobytes()).hexdigest() == pr["levels_sha256"], pr["root"]
+        assert "".join("T" if d else "B" for d in tr["direction"]) == pr["directions"]
+        assert tr["nq"] == pr["nq"]
+        assert tr["teps_edges"] == pr["nedge"] == 1048570  # pf_nedge[16]
+
+
+def _brute_bfs(n, v0, v1, root):
+    adj = collections.defaultdict(set)
+    for a, b in zip(v0, v1):
+        adj[a].add(b)
+        adj[b].add(a)
+    lv = {root: 0}
+    q = collections.deque([root])
+    while q:
+        u = q.popleft()
+        for w in adj[u]:
+            if w not in lv:
+                lv[w] = lv[u] + 1
+                q.append(w)
+    out = np.full(n, 0xFF, np.uint8)
+    for k, v in lv.items():
+        out[k] = v
+    return out
+
+
+@pytest.mark.parametrize("scale", [6, 8, 10])
+def test_bfs_brute_force_small_kronecker(oracle_mod, scale):
+    O = oracle_mod
+    e = O.generate_edges(scale)
+    c = O.Csr(e)
+    v0, v1 = O.edge_endpoints(e)
+    for r in c.find_roots(8):
+        lv, pred, tr = c.bfs(int(r))
+        assert np.array_equal(lv, _brute_bfs(c.num_verts, v0, v1, int(r)))
+        ok, cnt, msg = O.validate(e, c.max_used_vertex() + 1, int(r), pred)
+        assert ok, msg
+        assert cnt == tr["teps_edges"]
+
+
+def test_bfs_path_graph_switches_and_long_diameter(oracle_mod):
+    O = oracle_mod
+    n = 200
+    v0 = np.arange(n - 1)
+    v1 = np.arange(1, n)
+    c = O.Csr(O.pack_edges(v0, v1))
+    lv, pred, tr = c.bfs(0)
+    assert lv[: n].tolist() == list(range(n))
+
+
+def test_validator_accepts_and_each_rule_rejects(oracle_mod, s16):
+    O = oracle_mod
+    e, c = s16
+    root = int(c.find_roots(1)[0])
+    lv, pred, tr = c.bfs(root)
+    ng = c.max_used_vertex() + 1
+    ok, cnt, msg = O.validate(e, ng, root, pred)
+    assert ok and cnt == 1048570
+    reached = np.nonzero(lv != 0xFF)[0]
+    unreached = np.nonzero((lv == 0xFF) & (np.arange(len(lv)) < ng))[0]
+    deep = [v for v in reached if lv[v] >= 2][0]
+    cases = {}
+    p = pred.copy(); p[deep] = ng + 5 | (int(lv[deep]) << 48); cases["range"] = p
+    p = pred.copy(); p[root] = int(deep); cases["root"] = p
+    p = pred.copy(); p[deep] = int(deep) | (int(lv[deep]) << 48); cases["self"] = p
+    p = pred.copy(); p[deep] = (pred[deep] & ((1 << 48) - 1)) | ((int(lv[deep]) + 1) << 48); cases["depth"] = p
+    p = pred.copy(); p[unreached[0]] = root | (1 << 48); cases["unreached_gets_parent"] = p
+    p = pred.copy(); p[deep] = -1; cases["reached_marked_unreached"] = p
+    # parent with correct depth but no edge: pick a vertex at the parent's level that is not adjacent
+    v0, v1 = O.edge_endpoints(e)
+    nbrs = set(v1[v0 == deep].tolist()) | set(v0[v1 == deep].tolist())
+    cand = [v for v in np.nonzero(lv == lv[deep] - 1)[0] if v not in nbrs]
+    p = pred.copy(); p[deep] = int(cand[0]) | (int(lv[deep]) << 48); cases["no_edge"] = p
+    for name, p in cases.items():
+        ok, _, msg = O.validate(e, ng, root, p)
+        assert not ok, name
+        assert msg, name
+
+
+def test_statistics_formulas(oracle_mod):
+    x = np.array([3.0, 1.0, 2.0, 9.0, 4.0])
+    s = oracle_mod.statistics(x)
+    assert s["min"] == 1 and s["max"] == 9 and s["median"] == 3
+    assert s["mean"] == pytest.approx(x.mean())
+    assert s["std"] == pytest.approx(x.std(ddof=1))
+    assert s["q1"] == 2 and s["q3"] == 4
+
+
+def test_msbfs_bytes_brute_force(oracle_mod):
+    O = oracle_mod
+    e = O.generate_edges(8)
+    c = O.Csr(e)
+    a = c.arrays()
+    roots = c.find_roots(5)
+    n = c.num_verts
+    levels_new = np.full((len(roots), n), 0xFF, np.uint8)
+    for i, r in enumerate(roots):
+        lv, _, _ = c.bfs(int(r), want_pred=False)
+        levels_new[i] = lv[a["new2old"]]
+    tot, per = c.msbfs_bytes(levels_new)
+    # brute force in python
+    na = c.num_active
+    L = int(levels_new[levels_new != 0xFF].max()) + 1
+    exp = float((levels_new != 0xFF).sum())
+    for l in range(1, L + 1):
+        b = 24.0 * n
+        for v in range(na):
+            need = [r for r in range(len(roots)) if levels_new[r, v] == 0xFF or levels_new[r, v] >= l]
+            if not need:
+                continue
+            got, k = set(), 0
+            for j in range(a["offsets"][v], a["offsets"][v + 1]):
+                k += 1
+                w = a["cols"][j]
+                got |= {r for r in range(len(roots)) if levels_new[r, w] == l - 1}
+                if set(need) <= got:
+                    break
+            b += 8 + 12.0 * k
+        exp += b
+    assert tot == pytest.approx(exp)
