@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Graph500 BFS hot path on one MI355X: harmonic-mean GTEPS at Kronecker
+SCALE 28, edgefactor 16 (seeds 2,3; initiator 0.57/0.19/0.19/0.05).
+
+    python bench.py [--gpus 1] [--steps 64] [--warmup 2] [--scale 28]
+
+A step is one single-root direction-optimizing BFS (g5_run_bfs) over the
+graph resident in HBM; steps cycle over the 64 roots chosen by find_roots.
+Generation and CSR construction run on the GPU before the timed region, as in
+the reference driver (mpi/main.cc:50-117, only run_bfs is timed).  Each root
+is timed with the host clock around the synchronous call, like MPI_Wtime
+around run_bfs; GTEPS = 1 / mean(t_i / e_i) with e_i the validator's
+traversed-edge count (mpi/logfile.h:74-82).
+
+Prints ONE JSON line.  `roofline` is for the dominant kernel (the bottom-up
+sweep k_bu): algorithmic bytes of its launches (byte model evaluated on the
+GPU from the final levels, outside timing; it agrees with the CPU oracle's
+count, see tests) over their HIP-event durations on the library stream.
+`cpu_baseline` times the reference CPU runnable (oracle/_ref/runnable) on a
+bounded sample on this host, or the oracle port when that cannot run.
+
+Other modes:  --copy-calibration  runs a plain 1 GiB device copy (for scaling
+rocprof FETCH_SIZE/WRITE_SIZE counters).
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "GTEPS hmean SCALE 28 ef16: 64-root batch vs single-root; serve p50/p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(threads, scale=20, timeout=150):
+    """Reference runnable at a bounded SCALE; falls back to the oracle port."""
+    exe = os.path.join(HERE, "oracle", "_ref", "runnable")
+    mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
+    if os.path.exists(exe) and os.path.exists(mpirun):
+        try:
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            t0 = time.time()
+            r = subprocess.run([mpirun, "-np", "1", "-genv", "OMP_NUM_THREADS", str(threads), exe,
+                                str(scale), "16"], capture_output=True, text=True, timeout=timeout,
+                               env=env)
+            m = re.search(r"harmonic_mean_TEPS:\s+([0-9.eE+-]+)", r.stdout)
+            if r.returncode == 0 and m:
+                mt = re.search(r"mean_time:\s+([0-9.eE+-]+)", r.stdout)
+                return {
+                    "value": float(m.group(1)) / 1e9,
+                    "unit": "GTEPS",
+                    "cores": threads,
+                    "kind": "reference",
+                    "sample": ("reference runnable (VERTEX_REORDERING=2, 64 roots), mpirun -np 1, "
+                               "OMP_NUM_THREADS=%d, Kronecker SCALE %d ef16; harmonic-mean TEPS of "
+                               "its own timed run_bfs calls; mean_time %.4g s; wall %.1f s"
+                               % (threads, scale, float(mt.group(1)) if mt else float("nan"),
+                                  time.time() - t0)),
+                }
+            log("reference runnable failed rc=%s: %s" % (r.returncode, r.stderr[-400:]))
+        except Exception as ex:  # noqa: BLE001
+            log("reference runnable unavailable: %r" % (ex,))
+    # oracle port, single thread
+    from oracle import oracle as O
+    O.build()
+    e = O.generate_edges(scale)
+    c = O.Csr(e)
+    roots = c.find_roots(16)
+    spe = []
+    for r in roots:
+        t0 = time.perf_counter()
+        _, _, tr = c.bfs(int(r), want_pred=True)
+        spe.append((time.perf_counter() - t0) / tr["teps_edges"])
+    return {"value": len(spe) / sum(spe) / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": "oracle BFS (single thread) over 16 roots at SCALE %d ef16" % scale}
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 28  # 2^28 int32 = 1 GiB
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=28)
+    ap.add_argument("--edgefactor", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-scale", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
+    a = ap.parse_args()
+
+    if a.copy_calibration:
+        copy_calibration()
+        return 0
+    if a.gpus != 1:
+        print("bench.py measures one MI355X only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        return 2
+
+    import graph500_amd
+    ctx = graph500_amd.G5Context(0)
+    log("generate SCALE %d ef %d" % (a.scale, a.edgefactor))
+    ctx.generate_kronecker(a.scale, a.edgefactor, 2, 3)
+    log("construct")
+    ctx.construct(2, release_edges=True)
+    info = ctx.graph_info()
+    log("graph: n_active %d nnz %d gen %.1f ms construct %.1f ms" %
+        (info["num_active"], info["nnz"], info["generate_ms"], info["construct_ms"]))
+    roots = ctx.find_roots(64)
+    ctx.prepare_bfs()
+    for i in range(a.warmup):
+        ctx.run_bfs(int(roots[i % len(roots)]))
+    ctx.synchronize()
+
+    wall, dev, edges, bu_bytes, bu_ms, tot_bytes = [], [], [], 0.0, 0.0, 0.0
+    per_root = []
+    t_all = time.perf_counter()
+    for i in range(a.steps):
+        r = int(roots[i % len(roots)])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.run_bfs(r)
+        t1 = time.perf_counter()
+        st = ctx.bfs_stats()
+        e = ctx.teps_edges()
+        per, tot = ctx.model_bytes()
+        wall.append(t1 - t0)
+        dev.append(st["device_ms"] / 1e3)
+        edges.append(e)
+        tot_bytes += tot
+        for L, d in enumerate(st["direction"]):
+            if d == 0:
+                bu_bytes += per[L]
+                bu_ms += st["level_ms"][L]
+        per_root.append({"root": r, "ms": 1e3 * (t1 - t0), "edges": e,
+                         "dirs": "".join("T" if d else "B" for d in st["direction"])})
+        if i % 8 == 0:
+            log("root %d: %.3f ms %.1f GTEPS %s" % (r, 1e3 * (t1 - t0), e / (t1 - t0) / 1e9,
+                                                    per_root[-1]["dirs"]))
+    t_all = time.perf_counter() - t_all
+    hmean = len(wall) / sum(w / e for w, e in zip(wall, edges)) / 1e9
+    hmean_dev = len(dev) / sum(w / e for w, e in zip(dev, edges)) / 1e9
+    bu_gbs = bu_bytes / (bu_ms / 1e3) / 1e9 if bu_ms > 0 else 0.0
+    bu_launches = sum(p["dirs"].count("B") for p in per_root)
+    out = {
+        "metric": METRIC,
+        "value": round(hmean, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * sum(wall) / len(wall), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic: Kronecker graph from the Graph500 splittable-MRG generator "
+                "(seeds 2,3), generated and built on the GPU",
+        "config": {
+            "workload": "single-root direction-optimizing BFS, Kronecker SCALE %d edgefactor %d, "
+                        "64 find_roots roots cycled, VERTEX_REORDERING=2" % (a.scale, a.edgefactor),
+            "scale": a.scale,
+            "edgefactor": a.edgefactor,
+            "num_active": info["num_active"],
+            "directed_entries": info["nnz"],
+            "roots": len(roots),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_bu (bottom-up sweep)",
+            "achieved": round(bu_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(bu_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "launches": bu_launches,
+            "algorithmic_bytes_per_launch": round(bu_bytes / max(bu_launches, 1)),
+            "avg_launch_ms": round(bu_ms / max(bu_launches, 1), 4),
+        },
+        "single_root": {
+            "gteps_hmean_wall": round(hmean, 3),
+            "gteps_hmean_device": round(hmean_dev, 3),
+            "bfs_bytes_per_edge": round(tot_bytes / sum(edges), 4),
+            "bfs_roofline_frac": round((tot_bytes / sum(dev)) / (HBM_PEAK_GBS * 1e9), 4),
+            "traversed_edges_per_root": edges[0],
+            "mean_wall_ms": round(1e3 * sum(wall) / len(wall), 4),
+            "mean_device_ms": round(1e3 * sum(dev) / len(dev), 4),
+            "timed_region_s": round(t_all, 3),
+        },
+        "graph": {"generate_ms": round(info["generate_ms"], 2),
+                  "construct_ms": round(info["construct_ms"], 2)},
+        "batch": None,
+        "serve": None,
+    }
+    ctx.close()
+    if not a.no_cpu_baseline:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(a.cpu_threads, a.cpu_scale)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,539 @@
+// g5_bfs.hip -- single-root direction-optimizing BFS on gfx950.
+//
+// Reference behaviour restated: BfsBase::run_bfs (mpi/bfs.hpp:2651-2940):
+// level-synchronous search that starts top-down, switches to bottom-up when
+// the frontier grows beyond n_active/2000 and back to top-down (for good) when
+// fewer than n_active/8 vertices remain unvisited (mpi/bfs.hpp:2799-2829,
+// constants mpi/parameters.h:87-89).
+//
+// Kernels (vertex ids are degree-ordered "new" ids, rows [0, n_rows)):
+//   top-down   k_td_expand: the frontier's rows are concatenated into one
+//              virtual edge list (exclusive scan of frontier degrees); each
+//              workgroup takes 1024-edge chunks, stages the prefix/row-base/
+//              source of the frontier vertices overlapping its chunk in LDS and
+//              binary-searches there, so hub rows are split across the whole
+//              grid.  Claims are atomicOr test-and-set on the visited bitmap;
+//              winners are appended to the next queue with one atomic per wave.
+//   bottom-up  k_bu: one wave per 64-vertex bitmap word, one lane per vertex.
+//              Unvisited lanes first probe their first (highest-degree)
+//              neighbour from the coalesced first_nbr array; misses with a
+//              short row scan it lane-sequentially, long rows are scanned by
+//              the whole wave 64 entries at a time with a ballot for the first
+//              hit.  The visited snapshot of the previous level is read-only;
+//              the next bitmap word is written once per wave from a ballot.
+#include "g5_ctx.hpp"
+#include "g5_prims.hpp"
+
+namespace g5 {
+
+constexpr int kEpb = 1024;       // top-down edges per workgroup chunk
+constexpr int kShortRow = 32;    // bottom-up: lane-sequential scan up to this length
+constexpr uint32_t kNoNbr = 0xFFFFFFFFu;
+
+enum { C_FOUND = 0, C_AUX = 1, C_NUM = 4 };
+
+static unsigned grid_of(int64_t n, int64_t per, int64_t cap) {
+  int64_t b = (n + per - 1) / per;
+  if (b > cap) b = cap;
+  return (unsigned)(b > 0 ? b : 1);
+}
+
+
+__global__ void __launch_bounds__(kBlock) k_bfs_init(uint8_t* __restrict__ level, uint64_t* __restrict__ vis,
+                                                     int64_t rows, int64_t words) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < rows; v += stride) level[v] = 0xFF;
+  for (int64_t w = (int64_t)blockIdx.x * kBlock + threadIdx.x; w < words; w += stride) {
+    // bits past the last row read as visited so no sweep ever picks them
+    const int64_t first = w * 64;
+    uint64_t x = 0;
+    if (first + 64 > rows) x = (rows <= first) ? ~UINT64_C(0) : (~UINT64_C(0) << (rows - first));
+    vis[w] = x;
+  }
+}
+
+__global__ void k_bfs_root(const uint32_t* __restrict__ old2new, int64_t root_orig, int64_t rows,
+                           uint8_t* __restrict__ level, uint32_t* __restrict__ parent,
+                           uint64_t* __restrict__ vis, uint32_t* __restrict__ queue,
+                           unsigned long long* __restrict__ counters) {
+  const uint32_t r = old2new[root_orig];
+  counters[C_AUX] = r;
+  if ((int64_t)r < rows) {
+    level[r] = 0;
+    parent[r] = r;
+    vis[r >> 6] |= UINT64_C(1) << (r & 63);
+    queue[0] = r;
+  }
+}
+
+// prefix input: degree of frontier vertex i (0 past the end).
+struct FrontierDegree {
+  const uint32_t* queue;
+  const uint64_t* offsets;
+  int64_t nf;
+  __device__ uint64_t operator()(size_t i) const {
+    if ((int64_t)i >= nf) return 0;
+    const uint32_t u = queue[i];
+    return offsets[u + 1] - offsets[u];
+  }
+};
+
+__device__ __forceinline__ void wave_append(bool take, uint32_t value, uint32_t* __restrict__ out,
+                                            unsigned long long* __restrict__ counter) {
+  const uint64_t m = __ballot(take);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint64_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (take) out[base + __popcll(m & lanemask_lt())] = value;
+}
+
+__global__ void __launch_bounds__(kBlock) k_td_expand(
+    const uint32_t* __restrict__ queue, int64_t nf, const uint64_t* __restrict__ prefix,
+    const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols, uint64_t* vis,
+    uint8_t* __restrict__ level, uint32_t* __restrict__ parent, uint32_t* __restrict__ next,
+    unsigned long long* __restrict__ counters, uint32_t lvl) {
+  __shared__ uint64_t s_pre[kEpb + 1];
+  __shared__ uint64_t s_base[kEpb + 1];
+  __shared__ uint32_t s_src[kEpb + 1];
+  __shared__ int64_t s_range[2];
+  const uint64_t E = prefix[nf];
+  for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
+    const uint64_t e0 = chunk * kEpb;
+    const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
+    if (threadIdx.x < 2) {
+      // last i with prefix[i] <= key, over i in [0, nf)
+      const uint64_t key = threadIdx.x == 0 ? e0 : e1 - 1;
+      int64_t lo = 0, hi = nf - 1;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= key) lo = mid; else hi = mid - 1;
+      }
+      s_range[threadIdx.x] = lo;
+    }
+    __syncthreads();
+    const int64_t i0 = s_range[0];
+    // Every frontier vertex has a non-empty row, so at most kEpb rows overlap
+    // a chunk; the clamp only keeps LDS indexing in bounds.
+    int cnt = (int)(s_range[1] - i0 + 1);
+    cnt = cnt > kEpb + 1 ? kEpb + 1 : cnt;
+    for (int j = threadIdx.x; j < cnt; j += kBlock) {
+      const uint32_t u = queue[i0 + j];
+      const uint64_t p = prefix[i0 + j];
+      s_pre[j] = p;
+      s_base[j] = offsets[u] - p;
+      s_src[j] = u;
+    }
+    __syncthreads();
+    for (uint64_t eb = e0; eb < e1; eb += kBlock) {
+      const uint64_t e = eb + threadIdx.x;
+      bool claimed = false;
+      uint32_t w = 0;
+      if (e < e1) {
+        int lo = 0, hi = cnt - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_pre[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        w = cols[s_base[lo] + e];
+        const uint64_t bit = UINT64_C(1) << (w & 63);
+        uint64_t* wp = vis + (w >> 6);
+        if (!(__atomic_load_n(wp, __ATOMIC_RELAXED) & bit)) {
+          const uint64_t old = atomicOr((unsigned long long*)wp, (unsigned long long)bit);
+          claimed = !(old & bit);
+        }
+        if (claimed) {
+          level[w] = (uint8_t)lvl;
+          parent[w] = s_src[lo];
+        }
+      }
+      wave_append(claimed, w, next, counters + C_FOUND);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ bool bit_set(const uint64_t* __restrict__ bm, uint32_t v) {
+  return (bm[v >> 6] >> (v & 63)) & 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bu(
+    const uint64_t* __restrict__ vis, uint64_t* __restrict__ vis_next,
+    const uint32_t* __restrict__ first_nbr, const uint64_t* __restrict__ offsets,
+    const uint32_t* __restrict__ cols, uint8_t* __restrict__ level, uint32_t* __restrict__ parent,
+    int64_t words, uint32_t lvl, unsigned long long* __restrict__ counters) {
+  __shared__ uint64_t s_cnt[kBlock / 64];
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  uint64_t found_total = 0;
+  for (int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < words; w += nwaves) {
+    const uint64_t cur = vis[w];
+    if (cur == ~UINT64_C(0)) {
+      if (lane == 0) vis_next[w] = cur;
+      continue;
+    }
+    const uint32_t v = (uint32_t)(w * 64 + lane);
+    const bool unvisited = !((cur >> lane) & 1);
+    bool found = false;
+    uint32_t p = 0;
+    bool need = false;
+    uint64_t rb = 0, re = 0;
+    if (unvisited) {
+      const uint32_t f = first_nbr[v];
+      if (f != kNoNbr) {
+        if (bit_set(vis, f)) {
+          found = true;
+          p = f;
+        } else {
+          rb = offsets[v] + 1;
+          re = offsets[v + 1];
+          need = rb < re;
+        }
+      }
+    }
+    if (need && re - rb <= (uint64_t)kShortRow) {
+      for (uint64_t k = rb; k < re; ++k) {
+        const uint32_t c = cols[k];
+        if (bit_set(vis, c)) {
+          found = true;
+          p = c;
+          break;
+        }
+      }
+      need = false;
+    }
+    uint64_t lm = __ballot(need);
+    while (lm) {
+      const int src = __ffsll((unsigned long long)lm) - 1;
+      lm &= lm - 1;
+      const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
+      for (uint64_t base = b; base < e; base += 64) {
+        const uint64_t k = base + lane;
+        uint32_t c = 0;
+        bool hit = false;
+        if (k < e) {
+          c = cols[k];
+          hit = bit_set(vis, c);
+        }
+        const uint64_t hm = __ballot(hit);
+        if (hm) {
+          const int first = __ffsll((unsigned long long)hm) - 1;
+          const uint32_t hp = __shfl(c, first, 64);
+          if (lane == src) {
+            found = true;
+            p = hp;
+          }
+          break;
+        }
+      }
+    }
+    const uint64_t fm = __ballot(found);
+    if (lane == 0) vis_next[w] = cur | fm;
+    if (found) {
+      level[v] = (uint8_t)lvl;
+      parent[v] = p;
+    }
+    found_total += __popcll(fm);  // identical in every lane
+  }
+  if (lane == 0) s_cnt[threadIdx.x >> 6] = found_total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int i = 0; i < kBlock / 64; ++i) t += s_cnt[i];
+    if (t) atomicAdd(counters + C_FOUND, (unsigned long long)t);
+  }
+}
+
+// Vertices set in 'now' but not in 'before' -> queue (bottom-up -> top-down).
+__global__ void __launch_bounds__(kBlock) k_bitmap_diff_queue(const uint64_t* __restrict__ now,
+                                                              const uint64_t* __restrict__ before,
+                                                              int64_t words, uint32_t* __restrict__ queue,
+                                                              unsigned long long* __restrict__ counters) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < words; w += nwaves) {
+    const uint64_t d = now[w] & ~before[w];
+    const bool take = (d >> lane) & 1;
+    wave_append(take, (uint32_t)(w * 64 + lane), queue, counters + C_FOUND);
+  }
+}
+
+// --- algorithmic byte model, evaluated from the final levels -------------
+__device__ __forceinline__ void block_add(uint64_t v, unsigned long long* acc) {
+  __shared__ uint64_t s_part[kBlock / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int i = 0; i < kBlock / 64; ++i) t += s_part[i];
+    if (t) atomicAdd(acc, (unsigned long long)t);
+  }
+}
+
+// top-down level L reads the rows of the vertices at level L-1
+__global__ void __launch_bounds__(kBlock) k_model_td(const uint8_t* __restrict__ level,
+                                                     const uint64_t* __restrict__ offsets, int64_t rows,
+                                                     uint32_t L, unsigned long long* acc) {
+  uint64_t b = 0;
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < rows; v += (int64_t)gridDim.x * kBlock)
+    if (level[v] == L - 1) b += 12 + 4 * (offsets[v + 1] - offsets[v]);
+  block_add(b, acc);
+}
+
+// bottom-up level L probes, for every vertex with edges still unvisited, its
+// neighbours in ascending order up to the first one visited before L
+__global__ void __launch_bounds__(kBlock) k_model_bu(const uint8_t* __restrict__ level,
+                                                     const uint64_t* __restrict__ offsets,
+                                                     const uint32_t* __restrict__ cols, int64_t rows,
+                                                     uint32_t L, unsigned long long* acc) {
+  uint64_t b = 0;
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < rows; v += (int64_t)gridDim.x * kBlock) {
+    const uint8_t lv = level[v];
+    if (lv != 0xFF && lv < L) continue;
+    const uint64_t s = offsets[v], e = offsets[v + 1];
+    if (s == e) continue;
+    uint64_t k = 0;
+    for (uint64_t j = s; j < e; ++j) {
+      ++k;
+      const uint8_t lw = level[cols[j]];
+      if (lw != 0xFF && lw < L) break;
+    }
+    b += 8 + 4 * k;
+  }
+  block_add(b, acc);
+}
+
+void bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total) {
+  Graph& g = ctx->graph;
+  BfsState& b = ctx->bfs;
+  G5_CHECK(b.have_result, "g5_bfs_model_bytes: no BFS result (call g5_run_bfs first)");
+  const g5_bfs_stats& st = b.stats;
+  hipStream_t s = ctx->stream;
+  ctx->acc.alloc(4);
+  const int64_t rows = g.n_rows;
+  double tot = 1.0;  // the root's level byte
+  for (int L = 0; L < G5_MAX_LEVELS; ++L)
+    if (per_level) per_level[L] = 0;
+  for (int L = 1; L < st.num_levels; ++L) {
+    G5_HIP(hipMemsetAsync(ctx->acc.p, 0, 8, s));
+    const unsigned grid = grid_of(rows, kBlock, 8192);
+    if (st.direction[L])
+      hipLaunchKernelGGL(k_model_td, dim3(grid), dim3(kBlock), 0, s, b.level.p, g.offsets.p, rows,
+                         (uint32_t)L, (unsigned long long*)ctx->acc.p);
+    else
+      hipLaunchKernelGGL(k_model_bu, dim3(grid), dim3(kBlock), 0, s, b.level.p, g.offsets.p, g.cols.p,
+                         rows, (uint32_t)L, (unsigned long long*)ctx->acc.p);
+    G5_HIP(hipGetLastError());
+    uint64_t h = 0;
+    G5_HIP(hipMemcpyAsync(&h, ctx->acc.p, 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    const double nq = (double)st.nq[L];
+    double bytes = (double)h + nq;  // + 1 B level write per discovered vertex
+    if (st.direction[L]) bytes += 4.0 * nq;             // queue write
+    else bytes += 3.0 * (double)g.n / 8.0;              // visited read, next read+write
+    if (per_level) per_level[L] = bytes;
+    tot += bytes;
+  }
+  if (total) *total = tot;
+}
+
+struct ReachedWeight {
+  const uint8_t* level;
+  const uint32_t* weight;
+  __device__ uint64_t operator()(size_t i) const { return level[i] != 0xFF ? weight[i] : 0; }
+};
+
+__global__ void k_make_pred(const uint32_t* __restrict__ old2new, const uint32_t* __restrict__ new2old,
+                            const uint8_t* __restrict__ level, const uint32_t* __restrict__ parent,
+                            int64_t n, int64_t rows, int64_t root_orig, int64_t* __restrict__ pred) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t nv = old2new[v];
+    int64_t out = -1;
+    if (v == root_orig) {
+      out = root_orig;
+    } else if ((int64_t)nv < rows) {
+      const uint8_t l = level[nv];
+      if (l != 0xFF) out = (int64_t)new2old[parent[nv]] | ((int64_t)l << 48);
+    }
+    pred[v] = out;
+  }
+}
+
+__global__ void k_make_levels(const uint32_t* __restrict__ old2new, const uint8_t* __restrict__ level,
+                              int64_t n, int64_t rows, int64_t root_orig, uint8_t* __restrict__ out) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t nv = old2new[v];
+    uint8_t l = 0xFF;
+    if (v == root_orig) l = 0;
+    else if ((int64_t)nv < rows) l = level[nv];
+    out[v] = l;
+  }
+}
+
+void bfs_prepare(g5_ctx* ctx) {
+  Graph& g = ctx->graph;
+  G5_CHECK(g.built, "g5_prepare_bfs: no graph (call g5_construct first)");
+  BfsState& b = ctx->bfs;
+  b.release();
+  const int64_t rows = g.n_rows;
+  b.words = (rows + 63) / 64;
+  for (auto& v : b.vis) v.alloc(b.words);
+  b.level.alloc(rows);
+  b.parent.alloc(rows);
+  for (auto& q : b.queue) q.alloc(rows);
+  b.prefix.alloc(rows + 1);
+  b.scan_tmp.alloc(scan_tmp_elems(rows + 1));
+  b.counters.alloc(C_NUM);
+  G5_HIP(hipHostMalloc((void**)&b.h_counters, C_NUM * sizeof(uint64_t), hipHostMallocDefault));
+  G5_HIP(hipEventCreate(&b.ev0));
+  G5_HIP(hipEventCreate(&b.ev1));
+  b.lev_begin.assign(G5_MAX_LEVELS, nullptr);
+  b.lev_end.assign(G5_MAX_LEVELS, nullptr);
+  for (int i = 0; i < G5_MAX_LEVELS; ++i) {
+    G5_HIP(hipEventCreate(&b.lev_begin[i]));
+    G5_HIP(hipEventCreate(&b.lev_end[i]));
+  }
+  b.prepared = true;
+}
+
+void bfs_run(g5_ctx* ctx, int64_t root) {
+  Graph& g = ctx->graph;
+  BfsState& b = ctx->bfs;
+  G5_CHECK(b.prepared, "g5_run_bfs: call g5_prepare_bfs first");
+  G5_CHECK(root >= 0 && root < g.n, "g5_run_bfs: root out of range");
+  hipStream_t s = ctx->stream;
+  const int64_t rows = g.n_rows;
+  unsigned long long* cnt = (unsigned long long*)b.counters.p;
+  b.have_result = false;
+  b.root_orig = root;
+  g5_bfs_stats& st = b.stats;
+  st = g5_bfs_stats{};
+
+  G5_HIP(hipEventRecord(b.ev0, s));
+  hipLaunchKernelGGL(k_bfs_init, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, b.level.p,
+                     b.vis[0].p, rows, b.words);
+  G5_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bfs_root, dim3(1), dim3(1), 0, s, g.old2new.p, root, rows, b.level.p,
+                     b.parent.p, b.vis[0].p, b.queue[0].p, cnt);
+  G5_HIP(hipGetLastError());
+
+  // A root without edges (reachable only through a find_roots fallback) is
+  // the whole tree: the reference's first top-down level finds nothing.
+  const bool root_active = (g.has_edge_words[root >> 6] >> (root & 63)) & 1;
+  int cur = 0, qc = 0;
+  int64_t nf = 1;
+  bool next_forward = true, growing = true;
+  int64_t prev_nq = 0, visited = 1;
+  const double na = (double)g.n_active;
+  int lvl = 0;
+  if (!root_active) {
+    st.num_levels = 2;
+    st.direction[1] = 1;
+    st.nq[1] = 0;
+  }
+  while (root_active) {
+    ++lvl;
+    G5_CHECK(lvl < 255, "g5_run_bfs: more than 254 levels (uint8 level array)");
+    const bool forward = next_forward;
+    G5_HIP(hipMemsetAsync(b.counters.p, 0, 8, s));
+    G5_HIP(hipEventRecord(b.lev_begin[lvl], s));
+    if (forward) {
+      device_scan(FrontierDegree{b.queue[qc].p, g.offsets.p, nf}, StoreU64{b.prefix.p},
+                  (size_t)nf + 1, b.scan_tmp.p, s);
+      hipLaunchKernelGGL(k_td_expand, dim3(2048), dim3(kBlock), 0, s, b.queue[qc].p, nf, b.prefix.p,
+                         g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p, b.parent.p, b.queue[qc ^ 1].p,
+                         cnt, (uint32_t)lvl);
+    } else {
+      hipLaunchKernelGGL(k_bu, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
+                         b.vis[cur].p, b.vis[cur ^ 1].p, g.first_nbr.p, g.offsets.p, g.cols.p,
+                         b.level.p, b.parent.p, b.words, (uint32_t)lvl, cnt);
+    }
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipEventRecord(b.lev_end[lvl], s));
+    G5_HIP(hipMemcpyAsync(b.h_counters, b.counters.p, 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    const int64_t nq = (int64_t)b.h_counters[0];
+    {
+      float lms = 0;
+      G5_HIP(hipEventElapsedTime(&lms, b.lev_begin[lvl], b.lev_end[lvl]));
+      st.level_ms[lvl] = lms;
+    }
+    st.direction[lvl] = forward ? 1 : 0;
+    st.nq[lvl] = nq;
+    st.num_levels = lvl + 1;
+    visited += nq;
+    if (nq == 0) break;
+    // direction switch, mpi/bfs.hpp:2799-2829
+    const int64_t unvisited = g.n_active - visited;
+    if (growing && nq > prev_nq) {
+      if (forward && (double)nq > na / 2000.0) next_forward = false;
+    } else {
+      if (!forward && unvisited < (int64_t)(na / 8.0)) {
+        next_forward = true;
+        growing = false;
+      }
+    }
+    prev_nq = nq;
+    if (forward) {
+      qc ^= 1;
+    } else {
+      if (next_forward) {
+        G5_HIP(hipMemsetAsync(b.counters.p, 0, 8, s));
+        hipLaunchKernelGGL(k_bitmap_diff_queue, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock),
+                           0, s, b.vis[cur ^ 1].p, b.vis[cur].p, b.words, b.queue[qc].p, cnt);
+        G5_HIP(hipGetLastError());
+      }
+      cur ^= 1;
+    }
+    nf = nq;
+  }
+  G5_HIP(hipEventRecord(b.ev1, s));
+  G5_HIP(hipEventSynchronize(b.ev1));
+  float ms = 0;
+  G5_HIP(hipEventElapsedTime(&ms, b.ev0, b.ev1));
+  st.device_ms = ms;
+  st.reached = visited;
+  b.have_result = true;
+}
+
+void bfs_get_pred(g5_ctx* ctx, int64_t* host_pred) {
+  Graph& g = ctx->graph;
+  BfsState& b = ctx->bfs;
+  G5_CHECK(b.have_result, "g5_get_pred: no BFS result (call g5_run_bfs first)");
+  b.pred_out.alloc(g.n);
+  hipLaunchKernelGGL(k_make_pred, dim3(grid_of(g.n, kBlock, 65536)), dim3(kBlock), 0, ctx->stream,
+                     g.old2new.p, g.new2old.p, b.level.p, b.parent.p, g.n, g.n_rows, b.root_orig,
+                     b.pred_out.p);
+  G5_HIP(hipGetLastError());
+  G5_HIP(hipMemcpyAsync(host_pred, b.pred_out.p, g.n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  G5_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void bfs_get_levels(g5_ctx* ctx, uint8_t* host_levels) {
+  Graph& g = ctx->graph;
+  BfsState& b = ctx->bfs;
+  G5_CHECK(b.have_result, "g5_get_levels: no BFS result (call g5_run_bfs first)");
+  b.level_out.alloc(g.n);
+  hipLaunchKernelGGL(k_make_levels, dim3(grid_of(g.n, kBlock, 65536)), dim3(kBlock), 0, ctx->stream,
+                     g.old2new.p, b.level.p, g.n, g.n_rows, b.root_orig, b.level_out.p);
+  G5_HIP(hipGetLastError());
+  G5_HIP(hipMemcpyAsync(host_levels, b.level_out.p, g.n, hipMemcpyDeviceToHost, ctx->stream));
+  G5_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+int64_t bfs_teps_edges(g5_ctx* ctx) {
+  Graph& g = ctx->graph;
+  BfsState& b = ctx->bfs;
+  G5_CHECK(b.have_result, "g5_teps_edges: no BFS result (call g5_run_bfs first)");
+  ctx->acc.alloc(4);
+  const uint64_t sum = device_reduce_sum(ReachedWeight{b.level.p, g.weight.p}, (size_t)g.n_rows,
+                                         ctx->acc.p, ctx->stream);
+  return (int64_t)(sum / 2);
+}
+
+}  // namespace g5

@@ -1,0 +1,114 @@
+// g5_ctx.hpp -- the per-GPU context behind the C ABI.
+#pragma once
+
+#include "g5_internal.hpp"
+
+namespace g5 {
+
+// Canonical CSR of one graph in HBM (see g5_internal.hpp for the layout).
+struct Graph {
+  bool built = false;
+  int scale = 0;             // estimated SCALE (msb of OR of all ids + 1)
+  int reorder_mode = 2;
+  int64_t n = 0;             // 2^scale
+  int64_t n_active = 0;      // vertices with >= 1 non-loop tuple
+  int64_t n_rows = 0;        // rows swept by the BFS: n_active (modes 1,2) or n (mode 0)
+  int64_t num_tuples = 0;
+  int64_t self_loops = 0;
+  int64_t directed_before = 0;
+  int64_t nnz = 0;
+  int64_t max_degree = 0;
+  int64_t max_used_vertex = 0;
+  double construct_ms = 0;
+  DevBuf<uint64_t> offsets;    // n_rows + 1
+  DevBuf<uint32_t> cols;       // nnz
+  DevBuf<uint32_t> first_nbr;  // n_rows; 0xFFFFFFFF for an empty row
+  DevBuf<uint32_t> weight;     // n_rows: non-loop degree + 2 * self loops
+  DevBuf<uint32_t> old2new;    // n
+  DevBuf<uint32_t> new2old;    // n
+  std::vector<uint64_t> has_edge_words;  // host bitmap by original id
+  void clear() {
+    built = false;
+    offsets.release();
+    cols.release();
+    first_nbr.release();
+    weight.release();
+    old2new.release();
+    new2old.release();
+    has_edge_words.clear();
+  }
+};
+
+// Single-root BFS state.
+struct BfsState {
+  bool prepared = false;
+  bool have_result = false;
+  int64_t root_orig = -1;
+  uint32_t root_new = 0;
+  int64_t words = 0;              // bitmap words covering n_rows
+  DevBuf<uint64_t> vis[2];
+  DevBuf<uint8_t> level;          // n_rows
+  DevBuf<uint32_t> parent;        // n_rows
+  DevBuf<uint32_t> queue[2];      // n_rows
+  DevBuf<uint64_t> prefix;        // n_rows + 1 (top-down edge prefix)
+  DevBuf<uint64_t> scan_tmp;
+  DevBuf<uint64_t> counters;      // device counters (see g5_bfs.hip)
+  uint64_t* h_counters = nullptr; // pinned host mirror
+  DevBuf<int64_t> pred_out;       // n, built on demand by get_pred
+  DevBuf<uint8_t> level_out;      // n, built on demand by get_levels
+  g5_bfs_stats stats{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> lev_begin, lev_end;  // per level
+  void release() {
+    for (auto& v : vis) v.release();
+    level.release();
+    parent.release();
+    for (auto& q : queue) q.release();
+    prefix.release();
+    scan_tmp.release();
+    counters.release();
+    pred_out.release();
+    level_out.release();
+    if (h_counters) (void)hipHostFree(h_counters);
+    h_counters = nullptr;
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    ev0 = ev1 = nullptr;
+    for (auto e : lev_begin) (void)hipEventDestroy(e);
+    for (auto e : lev_end) (void)hipEventDestroy(e);
+    lev_begin.clear();
+    lev_end.clear();
+    prepared = false;
+    have_result = false;
+  }
+};
+
+}  // namespace g5
+
+struct g5_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // device edge list
+  g5::DevBuf<g5_packed_edge> edges;
+  int64_t num_edges = 0;
+  double generate_ms = 0;
+  g5::Graph graph;
+  g5::BfsState bfs;
+  g5::DevBuf<uint64_t> acc;  // small reduction scratch
+};
+
+namespace g5 {
+// g5_csr.hip
+void build_graph(g5_ctx* ctx, int reorder_mode, bool release_edges);
+// g5_gen.hip
+void launch_kronecker(int device, g5_packed_edge* out, int64_t start, int64_t count, int scale,
+                      uint64_t s1, uint64_t s2, int a_num, int bc_num, hipStream_t stream);
+// g5_bfs.hip
+void bfs_prepare(g5_ctx* ctx);
+void bfs_run(g5_ctx* ctx, int64_t root_orig);
+void bfs_get_pred(g5_ctx* ctx, int64_t* host_pred);
+void bfs_get_levels(g5_ctx* ctx, uint8_t* host_levels);
+int64_t bfs_teps_edges(g5_ctx* ctx);
+void bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
+}  // namespace g5

@@ -1,0 +1,275 @@
+"""Python host side over the C ABI (include/g5.h), mirroring the reference's
+BfsBase operator interface (mpi/bfs.hpp:29-130):
+
+    ctx = G5Context()                         # one GPU
+    ctx.generate_kronecker(scale, 16)         # generate_graph_spec2010
+    ctx.construct()                           # BfsBase::construct (VR=2)
+    roots = ctx.find_roots(64)                # find_roots
+    ctx.prepare_bfs()
+    ctx.run_bfs(root)                         # timed region of mpi/main.cc:113-117
+    pred = ctx.get_pred()                     # parent | level << 48, -1 unreached
+    ctx.end_bfs()
+
+The HIP library is required: importing works everywhere, but creating a
+context without graph500_amd/lib/libg5.so or without a GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libg5.so")
+
+EDGE_DTYPE = np.dtype([("v0_low", "<u4"), ("v1_low", "<u4"), ("high", "<u4")])
+MAX_LEVELS = 256
+
+G5_OK = 0
+ERRORS = {-1: "G5_ERR_ARG", -2: "G5_ERR_STATE", -3: "G5_ERR_HIP", -4: "G5_ERR_MEM",
+          -5: "G5_ERR_LIMIT"}
+
+# Every entry point of include/g5.h (the export test checks this list against
+# the header and the library).
+EXPORTS = [
+    "g5_version", "g5_device_count", "g5_create", "g5_destroy", "g5_last_error",
+    "g5_synchronize", "g5_edges_reserve", "g5_edges_append", "g5_edges_count",
+    "g5_edges_download", "g5_edges_release", "g5_generate_kronecker", "g5_construct",
+    "g5_graph_info_get", "g5_has_edge", "g5_pred_size", "g5_find_roots", "g5_graph_download",
+    "g5_prepare_bfs", "g5_run_bfs", "g5_get_pred", "g5_get_levels", "g5_teps_edges",
+    "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_end_bfs",
+]
+
+
+class G5Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "G5_ERR"), code, msg))
+        self.code = code
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [
+        ("scale_estimate", ctypes.c_int32),
+        ("reorder_mode", ctypes.c_int32),
+        ("num_verts", ctypes.c_int64),
+        ("num_active", ctypes.c_int64),
+        ("num_tuples", ctypes.c_int64),
+        ("num_self_loops", ctypes.c_int64),
+        ("directed_before", ctypes.c_int64),
+        ("nnz", ctypes.c_int64),
+        ("max_degree", ctypes.c_int64),
+        ("max_used_vertex", ctypes.c_int64),
+        ("construct_ms", ctypes.c_double),
+        ("generate_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BfsStats(ctypes.Structure):
+    _fields_ = [
+        ("num_levels", ctypes.c_int32),
+        ("direction", ctypes.c_int32 * MAX_LEVELS),
+        ("nq", ctypes.c_int64 * MAX_LEVELS),
+        ("level_ms", ctypes.c_double * MAX_LEVELS),
+        ("device_ms", ctypes.c_double),
+        ("reached", ctypes.c_int64),
+    ]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libg5.so; raises if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libg5.so not built: run `python -m graph500_amd.build` (%s)" % path)
+    L = ctypes.CDLL(path)
+    vp, i64, u64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int
+    sig = {
+        "g5_version": ([], ctypes.c_char_p),
+        "g5_device_count": ([ctypes.POINTER(i32)], i32),
+        "g5_create": ([i32, ctypes.POINTER(vp)], i32),
+        "g5_destroy": ([vp], None),
+        "g5_last_error": ([vp], ctypes.c_char_p),
+        "g5_synchronize": ([vp], i32),
+        "g5_edges_reserve": ([vp, i64], i32),
+        "g5_edges_append": ([vp, vp, i64], i32),
+        "g5_edges_count": ([vp, ctypes.POINTER(i64)], i32),
+        "g5_edges_download": ([vp, i64, i64, vp], i32),
+        "g5_edges_release": ([vp], i32),
+        "g5_generate_kronecker": ([vp, i32, i32, u64, u64, i32, i32], i32),
+        "g5_construct": ([vp, i32, i32], i32),
+        "g5_graph_info_get": ([vp, ctypes.POINTER(GraphInfo)], i32),
+        "g5_has_edge": ([vp, i64], i32),
+        "g5_pred_size": ([vp, ctypes.POINTER(i64)], i32),
+        "g5_find_roots": ([vp, u64, u64, i32, vp, ctypes.POINTER(i32)], i32),
+        "g5_graph_download": ([vp, vp, vp, vp, vp], i32),
+        "g5_prepare_bfs": ([vp], i32),
+        "g5_run_bfs": ([vp, i64], i32),
+        "g5_get_pred": ([vp, vp], i32),
+        "g5_get_levels": ([vp, vp], i32),
+        "g5_teps_edges": ([vp, ctypes.POINTER(i64)], i32),
+        "g5_bfs_stats_get": ([vp, ctypes.POINTER(BfsStats)], i32),
+        "g5_bfs_model_bytes": ([vp, vp, ctypes.POINTER(ctypes.c_double)], i32),
+        "g5_end_bfs": ([vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class G5Context:
+    """One GPU context; methods mirror BfsBase / EdgeListStorage."""
+
+    def __init__(self, device=0):
+        self._lib = load_library()
+        n = ctypes.c_int(0)
+        self._lib.g5_device_count(ctypes.byref(n))
+        if n.value <= device:
+            raise RuntimeError("no HIP device %d (found %d)" % (device, n.value))
+        h = ctypes.c_void_p()
+        rc = self._lib.g5_create(device, ctypes.byref(h))
+        if rc != G5_OK:
+            raise G5Error(rc, "g5_create failed")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.g5_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise G5Error(rc, self._lib.g5_last_error(self._h).decode())
+        return rc
+
+    # -- edge list ---------------------------------------------------------
+    def generate_kronecker(self, scale, edgefactor=16, seed1=2, seed2=3, a_num=5700, bc_num=1900):
+        self._check(self._lib.g5_generate_kronecker(self._h, scale, edgefactor, seed1, seed2,
+                                                    a_num, bc_num))
+
+    def edges_append(self, tuples):
+        t = np.ascontiguousarray(tuples, EDGE_DTYPE)
+        self._check(self._lib.g5_edges_append(self._h, _ptr(t), len(t)))
+
+    def edges_count(self):
+        n = ctypes.c_int64(0)
+        self._check(self._lib.g5_edges_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def edges_download(self, start=0, count=None):
+        if count is None:
+            count = self.edges_count() - start
+        out = np.zeros(count, EDGE_DTYPE)
+        self._check(self._lib.g5_edges_download(self._h, start, count, _ptr(out)))
+        return out
+
+    def edges_release(self):
+        self._check(self._lib.g5_edges_release(self._h))
+
+    # -- graph -------------------------------------------------------------
+    def construct(self, reorder_mode=2, release_edges=False):
+        self._check(self._lib.g5_construct(self._h, reorder_mode, 1 if release_edges else 0))
+
+    def graph_info(self):
+        gi = GraphInfo()
+        self._check(self._lib.g5_graph_info_get(self._h, ctypes.byref(gi)))
+        return gi.as_dict()
+
+    def has_edge(self, v):
+        return bool(self._check(self._lib.g5_has_edge(self._h, v)))
+
+    def pred_size(self):
+        n = ctypes.c_int64(0)
+        self._check(self._lib.g5_pred_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def find_roots(self, num_roots, seed1=2, seed2=3):
+        out = np.zeros(max(num_roots, 1), np.int64)
+        k = ctypes.c_int(0)
+        self._check(self._lib.g5_find_roots(self._h, seed1, seed2, num_roots, _ptr(out),
+                                            ctypes.byref(k)))
+        return out[: k.value]
+
+    def graph_download(self):
+        gi = self.graph_info()
+        rows = gi["num_verts"] if gi["reorder_mode"] == 0 else gi["num_active"]
+        offsets = np.zeros(rows + 1, np.uint64)
+        cols = np.zeros(max(gi["nnz"], 1), np.uint32)
+        new2old = np.zeros(gi["num_verts"], np.uint32)
+        weight = np.zeros(max(rows, 1), np.uint32)
+        self._check(self._lib.g5_graph_download(self._h, _ptr(offsets), _ptr(cols), _ptr(new2old),
+                                                _ptr(weight)))
+        return {"offsets": offsets, "cols": cols[: gi["nnz"]], "new2old": new2old,
+                "weight": weight[:rows]}
+
+    # -- BFS ---------------------------------------------------------------
+    def prepare_bfs(self):
+        self._check(self._lib.g5_prepare_bfs(self._h))
+
+    def run_bfs(self, root):
+        self._check(self._lib.g5_run_bfs(self._h, int(root)))
+
+    def get_pred(self, out=None):
+        if out is None:
+            out = np.zeros(self.pred_size(), np.int64)
+        self._check(self._lib.g5_get_pred(self._h, _ptr(out)))
+        return out
+
+    def get_levels(self, out=None):
+        if out is None:
+            out = np.zeros(self.pred_size(), np.uint8)
+        self._check(self._lib.g5_get_levels(self._h, _ptr(out)))
+        return out
+
+    def teps_edges(self):
+        n = ctypes.c_int64(0)
+        self._check(self._lib.g5_teps_edges(self._h, ctypes.byref(n)))
+        return n.value
+
+    def bfs_stats(self):
+        st = BfsStats()
+        self._check(self._lib.g5_bfs_stats_get(self._h, ctypes.byref(st)))
+        L = st.num_levels
+        return {
+            "num_levels": L,
+            "direction": [st.direction[i] for i in range(1, L)],
+            "nq": [st.nq[i] for i in range(1, L)],
+            "level_ms": [st.level_ms[i] for i in range(1, L)],
+            "device_ms": st.device_ms,
+            "reached": st.reached,
+        }
+
+    def model_bytes(self):
+        """(per-level algorithmic bytes [level 1..], total) of the last run."""
+        per = np.zeros(MAX_LEVELS, np.float64)
+        tot = ctypes.c_double(0)
+        self._check(self._lib.g5_bfs_model_bytes(self._h, _ptr(per), ctypes.byref(tot)))
+        L = self.bfs_stats()["num_levels"]
+        return per[1:L].tolist(), tot.value
+
+    def end_bfs(self):
+        self._check(self._lib.g5_end_bfs(self._h))
+
+    def synchronize(self):
+        self._check(self._lib.g5_synchronize(self._h))

@@ -1,0 +1,157 @@
+/*
+ * g5.h -- C ABI of the MI355X-native Graph500 BFS hot path.
+ *
+ * One context (g5_ctx) owns one GPU's copy of an edge list, the CSR built from
+ * it and the BFS state.  Every entry point returns 0 on success or a negative
+ * G5_ERR_* code; g5_last_error() then gives a message.  No exception crosses
+ * this boundary.  Calls on one context must be serialized by the caller (the
+ * reference is MPI_THREAD_SINGLE, mpi/utils.hpp:1580); different contexts are
+ * independent.  Host pointers passed in are only read/written during the call.
+ *
+ * Each entry point names the reference interface it replaces (file:line,
+ * relative to the suzumura/graph500 tree).
+ */
+#ifndef G5_H
+#define G5_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define G5_OK 0
+#define G5_ERR_ARG (-1)      /* invalid argument */
+#define G5_ERR_STATE (-2)    /* call out of order (e.g. run_bfs before construct) */
+#define G5_ERR_HIP (-3)      /* HIP runtime error */
+#define G5_ERR_MEM (-4)      /* device allocation failed */
+#define G5_ERR_LIMIT (-5)    /* input beyond a documented limit */
+
+#define G5_MAX_LEVELS 256
+
+typedef struct g5_ctx g5_ctx;
+
+/* 12-byte packed tuple, identical layout to UnweightedPackedEdge
+ * (mpi/primitives.hpp:51-64): v0 = v0_low | (high & 0xFFFF) << 32,
+ * v1 = v1_low | (high >> 16) << 32. */
+typedef struct g5_packed_edge {
+  uint32_t v0_low;
+  uint32_t v1_low;
+  uint32_t high;
+} g5_packed_edge;
+
+/* Graph facts after construction (Graph2DCSR members,
+ * mpi/graph_constructor.hpp:113-143). */
+typedef struct g5_graph_info {
+  int32_t scale_estimate;   /* log_orig_global_verts_ */
+  int32_t reorder_mode;     /* VERTEX_REORDERING used */
+  int64_t num_verts;        /* 2^scale_estimate = pred_size() at 1x1 */
+  int64_t num_active;       /* num_global_verts_: vertices with >= 1 edge */
+  int64_t num_tuples;       /* input tuples */
+  int64_t num_self_loops;   /* tuples with v0 == v1 (dropped) */
+  int64_t directed_before;  /* directed entries before duplicate removal */
+  int64_t nnz;              /* num_global_edges_: directed entries after removal */
+  int64_t max_degree;       /* largest degree counting duplicates */
+  int64_t max_used_vertex;  /* find_max_used_vertex, mpi/benchmark_helper.hpp:510-528 */
+  double construct_ms;      /* device time of the last g5_construct */
+  double generate_ms;       /* device time of the last g5_generate_kronecker */
+} g5_graph_info;
+
+/* Per-run trace of the last g5_run_bfs (the reference's VERVOSE per-level
+ * report, mpi/bfs.hpp:2836-2879). Index 0 is unused; levels start at 1. */
+typedef struct g5_bfs_stats {
+  int32_t num_levels;               /* levels executed + 1 (last level finds nothing) */
+  int32_t direction[G5_MAX_LEVELS]; /* 1 = top-down, 0 = bottom-up */
+  int64_t nq[G5_MAX_LEVELS];        /* vertices discovered at that level */
+  double level_ms[G5_MAX_LEVELS];   /* device time of each level's kernels (HIP events) */
+  double device_ms;                 /* device time of the whole run (HIP events) */
+  int64_t reached;                  /* vertices with a level, root included */
+} g5_bfs_stats;
+
+/* ---- context ---- */
+const char* g5_version(void);
+int g5_device_count(int* count);
+/* Creates a context on HIP device 'device' with its own stream. */
+int g5_create(int device, g5_ctx** out);
+void g5_destroy(g5_ctx* ctx);
+/* Message of the last failing call on ctx (empty when none). */
+const char* g5_last_error(const g5_ctx* ctx);
+/* Waits for all work queued by ctx. */
+int g5_synchronize(g5_ctx* ctx);
+
+/* ---- edge list: EdgeListStorage<UnweightedPackedEdge, 8Mi>
+ *      (generator/graph_generator.hpp:35-319) held in HBM ---- */
+/* Pre-sizes the device edge list (edge_memory_size_, graph_generator.hpp:58). */
+int g5_edges_reserve(g5_ctx* ctx, int64_t capacity);
+/* Appends host tuples (EdgeListStorage::write, graph_generator.hpp:190-243). */
+int g5_edges_append(g5_ctx* ctx, const g5_packed_edge* tuples, int64_t count);
+/* Number of tuples held. */
+int g5_edges_count(g5_ctx* ctx, int64_t* count);
+/* Copies tuples [start, start+count) to the host (EdgeListStorage::read,
+ * graph_generator.hpp:120-154). */
+int g5_edges_download(g5_ctx* ctx, int64_t start, int64_t count, g5_packed_edge* out);
+/* Frees the device edge list. */
+int g5_edges_release(g5_ctx* ctx);
+/* generate_graph_spec2010 / RmatGraphGenerator::generateRange
+ * (mpi/benchmark_helper.hpp:237-314, generator/graph_generator.hpp:602-705):
+ * replaces the edge list with edgefactor*2^scale Kronecker tuples, bit-exact
+ * with the reference splittable-MRG generator.  The reference uses seeds
+ * (2,3) and a_num=5700, bc_num=1900 (per 10000). */
+int g5_generate_kronecker(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1,
+                          uint64_t seed2, int a_num, int bc_num);
+
+/* ---- graph: BfsBase::construct -> GraphConstructor2DCSR::construct
+ *      (mpi/bfs.hpp:102-113, mpi/graph_constructor.hpp:554-574) ---- */
+/* Builds the CSR from the edge list.  reorder_mode is VERTEX_REORDERING
+ * (mpi/parameters.h:38-40): 2 = degree order (ties by id), 1 = vertices with
+ * edges first, 0 = original order.  release_edges != 0 frees the device edge
+ * list afterwards (EdgeListStorage::beginRead(true), graph_generator.hpp:98). */
+int g5_construct(g5_ctx* ctx, int reorder_mode, int release_edges);
+int g5_graph_info_get(g5_ctx* ctx, g5_graph_info* out);
+/* Graph2DCSR::has_edge (mpi/graph_constructor.hpp:101-110): 1, 0 or < 0. */
+int g5_has_edge(g5_ctx* ctx, int64_t vertex);
+/* Graph2DCSR::pred_size (mpi/graph_constructor.hpp:64). */
+int g5_pred_size(g5_ctx* ctx, int64_t* size);
+/* find_roots (mpi/benchmark_helper.hpp:475-508): writes up to num_roots roots. */
+int g5_find_roots(g5_ctx* ctx, uint64_t seed1, uint64_t seed2, int num_roots, int64_t* roots,
+                  int* num_found);
+/* Copies the canonical CSR to the host (for parity checks).  Any pointer may
+ * be NULL.  offsets: num_rows+1 (num_rows = num_active for modes 1, 2 and
+ * num_verts for mode 0), cols: nnz, new2old: num_verts, weight: num_rows. */
+int g5_graph_download(g5_ctx* ctx, uint64_t* offsets, uint32_t* cols, uint32_t* new2old,
+                      uint32_t* weight);
+
+/* ---- single-root BFS: BfsBase (mpi/bfs.hpp:115-128, run_bfs :2651) ---- */
+/* prepare_bfs: allocates BFS state (mpi/bfs.hpp:115-118). */
+int g5_prepare_bfs(g5_ctx* ctx);
+/* run_bfs(root, pred): direction-optimizing BFS from an original vertex id.
+ * Synchronous: returns after the device finished, so host timing brackets it
+ * like mpi/main.cc:113-117.  Levels and parents stay in HBM. */
+int g5_run_bfs(g5_ctx* ctx, int64_t root);
+/* get_pred (mpi/bfs.hpp:122-124): writes pred_size() entries of
+ * parent | level << 48, or -1 when unreached (encoding of mpi/bfs.hpp:1203,
+ * 1271, 2470). */
+int g5_get_pred(g5_ctx* ctx, int64_t* pred);
+/* uint8 level per original vertex id, 0xFF when unreached. */
+int g5_get_levels(g5_ctx* ctx, uint8_t* levels);
+/* Traversed-edge count of the last run: input tuples with a reached endpoint,
+ * duplicates and self loops included (mpi/validate.hpp:703-721). */
+int g5_teps_edges(g5_ctx* ctx, int64_t* count);
+int g5_bfs_stats_get(g5_ctx* ctx, g5_bfs_stats* out);
+/* Algorithmic HBM bytes of the last run, per level and in total: top-down
+ * level = sum over the frontier of (4 + 8 + 4 deg) + 4 per discovered vertex;
+ * bottom-up level = 3 n / 8 + sum over unvisited vertices with edges of
+ * (8 + 4 k), k = neighbours probed in ascending order up to the first visited
+ * one; plus 1 byte per reached vertex.  No reference counterpart: it is the
+ * byte model the roofline figures are computed with (evaluated on the GPU
+ * from the final levels, outside any timed region). per_level has
+ * G5_MAX_LEVELS entries, index = level. */
+int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
+/* end_bfs (mpi/bfs.hpp:126-128): frees BFS state. */
+int g5_end_bfs(g5_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

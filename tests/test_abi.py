@@ -1,0 +1,61 @@
+"""CPU-side checks of the C ABI: the library builds, loads and exports every
+entry point include/g5.h declares; calls that need a GPU fail cleanly."""
+import ctypes
+import os
+import re
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(REPO, "include", "g5.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int|void)\s+(g5_\w+)\s*\(", txt, re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import graph500_amd
+    from graph500_amd import build
+    if not os.path.exists(build.LIB):
+        build.build()
+    return graph500_amd.load_library()
+
+
+def test_header_lists_every_binding():
+    from graph500_amd import g5
+    assert header_functions() == sorted(g5.EXPORTS)
+
+
+def test_library_exports_every_header_symbol(lib):
+    for name in header_functions():
+        assert hasattr(lib, name), name
+
+
+def test_version_and_device_query_without_gpu(lib):
+    assert b"gfx950" in lib.g5_version()
+    n = ctypes.c_int(-1)
+    assert lib.g5_device_count(ctypes.byref(n)) == 0
+    assert n.value >= 0
+
+
+def test_null_context_is_rejected(lib):
+    assert lib.g5_run_bfs(None, 0) == -1
+    assert lib.g5_construct(None, 2, 0) == -1
+    assert lib.g5_last_error(None) == b"null context"
+
+
+def test_create_fails_cleanly_on_missing_device(lib):
+    h = ctypes.c_void_p()
+    assert lib.g5_create(10_000, ctypes.byref(h)) == -1
+    assert not h.value
+
+
+def test_context_requires_gpu():
+    import graph500_amd
+    n = ctypes.c_int(0)
+    graph500_amd.load_library().g5_device_count(ctypes.byref(n))
+    if n.value == 0:
+        with pytest.raises(RuntimeError):
+            graph500_amd.G5Context(0)

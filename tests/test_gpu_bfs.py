@@ -1,0 +1,277 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle and the
+reference golden fixtures.
+
+Bar: bit-exact for tuples, CSR, roots, levels, reached sets, direction
+schedules, frontier sizes and traversed-edge counts; parents may differ from
+the reference's but must pass the validator rules (mpi/validate.hpp:506-802).
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gold(name):
+    return json.load(open(os.path.join(GOLD, name)))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import graph500_amd
+    c = graph500_amd.G5Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def s16(ctx, oracle_mod):
+    say("generate+construct SCALE 16")
+    ctx.generate_kronecker(16, 16, 2, 3)
+    edges = ctx.edges_download()
+    ctx.construct(2)
+    ctx.prepare_bfs()
+    return edges, oracle_mod.Csr(oracle_mod.generate_edges(16))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_generator_bit_exact_s16(s16, oracle_mod):
+    edges, _ = s16
+    g = gold("ref_s16_16roots.json")
+    t = edges.view(np.uint32).reshape(-1, 3)
+    assert sha(t) == g["tuples_sha256"]
+    assert np.array_equal(edges, oracle_mod.generate_edges(16))
+
+
+@pytest.mark.parametrize("scale,seeds,init", [(12, (7, 11), (5700, 1900)),
+                                              (10, (2**40 + 5, 3), (5500, 100)),
+                                              (1, (2, 3), (5700, 1900))])
+def test_generator_other_seeds_and_initiators(oracle_mod, scale, seeds, init):
+    import graph500_amd
+    say("generator", scale, seeds, init)
+    with graph500_amd.G5Context(0) as c:
+        c.generate_kronecker(scale, 16, seeds[0], seeds[1], init[0], init[1])
+        got = c.edges_download()
+    exp = oracle_mod.generate_edges(scale, 16, seeds[0], seeds[1], init[0], init[1])
+    assert np.array_equal(got, exp)
+
+
+def test_csr_byte_equal_s16(ctx, s16):
+    _, oc = s16
+    info = ctx.graph_info()
+    g = gold("ref_s16_16roots.json")
+    assert info["scale_estimate"] == 16
+    assert info["num_active"] == g["num_active"] == oc.num_active
+    assert info["nnz"] == g["nnz"] == oc.nnz
+    assert info["directed_before"] == oc.directed_before
+    assert info["num_self_loops"] == oc.num_self_loops
+    a = oc.arrays()
+    d = ctx.graph_download()
+    assert np.array_equal(d["new2old"], a["new2old"])
+    assert np.array_equal(d["offsets"], a["offsets"])
+    assert np.array_equal(d["cols"], a["cols"])
+    na = oc.num_active
+    w = a["degree_raw"][:na] + 2 * a["self_loops"][:na]
+    assert np.array_equal(d["weight"], w)
+    assert info["max_used_vertex"] == g["max_used_vertex"] == oc.max_used_vertex()
+
+
+def test_find_roots_match_reference(ctx, s16):
+    assert ctx.find_roots(16).tolist() == gold("ref_s16_16roots.json")["roots"]
+    assert ctx.find_roots(64).tolist() == gold("ref_s16_64roots.json")["roots"]
+
+
+def test_has_edge_matches_oracle(ctx, s16):
+    _, oc = s16
+    for v in [0, 1, 5, 100, 29348, 65535, 65536, -1]:
+        assert ctx.has_edge(v) == (0 <= v < oc.num_verts and oc.has_edge(v))
+
+
+def test_bfs_64_roots_bit_exact_s16(ctx, s16, oracle_mod):
+    edges, oc = s16
+    g = gold("ref_s16_64roots.json")
+    ng = oc.max_used_vertex() + 1
+    for pr in g["per_root"]:
+        say("bfs s16 root", pr["root"])
+        ctx.run_bfs(pr["root"])
+        lv = ctx.get_levels()
+        assert sha(lv) == pr["levels_sha256"], pr["root"]
+        st = ctx.bfs_stats()
+        assert "".join("T" if d else "B" for d in st["direction"]) == pr["directions"]
+        assert st["nq"] == pr["nq"]
+        assert ctx.teps_edges() == pr["nedge"] == 1048570
+        pred = ctx.get_pred()
+        reached = pred != -1
+        assert np.array_equal(reached, lv != 0xFF)
+        assert np.array_equal((pred[reached] >> 48), lv[reached].astype(np.int64))
+        ok, cnt, msg = oracle_mod.validate(edges, ng, pr["root"], pred)
+        assert ok, msg
+        assert cnt == pr["nedge"]
+
+
+@pytest.mark.parametrize("scale", [18, 20, 22])
+def test_bfs_reference_fixtures_larger_scales(scale, oracle_mod):
+    import graph500_amd
+    g = gold("ref_s%d_16roots.json" % scale)
+    say("scale", scale)
+    with graph500_amd.G5Context(0) as c:
+        c.generate_kronecker(scale, 16, 2, 3)
+        t = c.edges_download().view(np.uint32).reshape(-1, 3)
+        assert sha(t) == g["tuples_sha256"]
+        c.construct(2)
+        info = c.graph_info()
+        assert info["num_active"] == g["num_active"]
+        assert info["nnz"] == g["nnz"]
+        assert c.find_roots(16).tolist() == g["roots"]
+        c.prepare_bfs()
+        for i, pr in enumerate(g["per_root"]):
+            c.run_bfs(pr["root"])
+            lv = c.get_levels()
+            assert sha(lv) == pr["levels_sha256"], (scale, pr["root"])
+            st = c.bfs_stats()
+            assert "".join("T" if d else "B" for d in st["direction"]) == pr["directions"]
+            assert st["nq"] == pr["nq"]
+            assert c.teps_edges() == pr["nedge"]
+            if i < 2:
+                edges = c.edges_download()
+                ok, cnt, msg = oracle_mod.validate(edges, info["max_used_vertex"] + 1, pr["root"],
+                                                   c.get_pred())
+                assert ok, msg
+                assert cnt == pr["nedge"]
+
+
+def _run_small(oracle_mod, v0, v1, roots, modes=(2, 1, 0)):
+    import graph500_amd
+    e = oracle_mod.pack_edges(v0, v1)
+    oc = oracle_mod.Csr(e)
+    for mode in modes:
+        with graph500_amd.G5Context(0) as c:
+            c.edges_append(e[: len(e) // 2])
+            c.edges_append(e[len(e) // 2:])
+            c.construct(mode)
+            c.prepare_bfs()
+            for r in roots:
+                if oc.has_edge(int(r)):
+                    lv_o, _, tr = oc.bfs(int(r))
+                else:  # a root without edges is the whole tree
+                    lv_o = np.full(oc.num_verts, 0xFF, np.uint8)
+                    lv_o[int(r)] = 0
+                c.run_bfs(int(r))
+                lv = c.get_levels()
+                assert np.array_equal(lv, lv_o), (mode, r)
+                pred = c.get_pred()
+                if oc.has_edge(int(r)):
+                    # the validator only accepts roots up to the largest vertex with an edge
+                    ok, cnt, msg = oracle_mod.validate(e, oc.max_used_vertex() + 1, int(r), pred)
+                    assert ok, (mode, r, msg)
+                    st = c.bfs_stats()
+                    assert st["nq"] == tr["nq"], (mode, r)
+                    assert c.teps_edges() == cnt == tr["teps_edges"], (mode, r)
+                else:
+                    assert pred[int(r)] == int(r) and (pred != -1).sum() == 1
+
+
+def test_small_graphs_edge_cases(oracle_mod):
+    rng = np.random.default_rng(5)
+    # two components, self loops and duplicates
+    v0 = np.array([0, 1, 2, 2, 5, 5, 6, 7, 7, 9, 3, 3])
+    v1 = np.array([1, 2, 0, 2, 6, 6, 7, 5, 7, 9, 3, 4])
+    say("small: components")
+    _run_small(oracle_mod, v0, v1, [0, 5, 3, 9, 8])
+    # star with a long tail (hub row split across the grid)
+    n = 5000
+    v0 = np.concatenate([np.zeros(n - 1, np.int64), np.arange(n, n + 60)])
+    v1 = np.concatenate([np.arange(1, n), np.arange(n + 1, n + 61)])
+    say("small: star+path")
+    _run_small(oracle_mod, v0, v1, [0, 17, n, n + 30])
+    # random sparse graph
+    v0 = rng.integers(0, 3000, 20000)
+    v1 = rng.integers(0, 3000, 20000)
+    say("small: random")
+    _run_small(oracle_mod, v0, v1, [int(v0[0]), int(v1[5])])
+
+
+def test_path_graph_depth(oracle_mod):
+    n = 250
+    say("path graph")
+    _run_small(oracle_mod, np.arange(n - 1), np.arange(1, n), [0, 125], modes=(2,))
+
+
+def test_too_deep_graph_raises(oracle_mod):
+    import graph500_amd
+    n = 400
+    e = oracle_mod.pack_edges(np.arange(n - 1), np.arange(1, n))
+    with graph500_amd.G5Context(0) as c:
+        c.edges_append(e)
+        c.construct(2)
+        c.prepare_bfs()
+        with pytest.raises(graph500_amd.G5Error):
+            c.run_bfs(0)
+
+
+def test_error_paths():
+    import graph500_amd
+    E = graph500_amd.G5Error
+    with graph500_amd.G5Context(0) as c:
+        with pytest.raises(E):
+            c.prepare_bfs()  # no graph
+        with pytest.raises(E):
+            c.construct(2)  # empty edge list
+        with pytest.raises(E):
+            c.generate_kronecker(0)  # scale out of range
+        c.generate_kronecker(8)
+        with pytest.raises(E):
+            c.construct(7)  # unknown reorder mode
+        c.construct(2)
+        with pytest.raises(E):
+            c.run_bfs(0)  # not prepared
+        c.prepare_bfs()
+        with pytest.raises(E):
+            c.get_pred()  # no result yet
+        with pytest.raises(E):
+            c.run_bfs(1 << 8)  # root out of range
+        with pytest.raises(E):
+            c.run_bfs(-1)
+        r = int(c.find_roots(1)[0])
+        c.run_bfs(r)
+        assert c.get_levels()[r] == 0
+
+
+def test_model_bytes_match_oracle(ctx, s16, oracle_mod):
+    """The GPU evaluation of the byte model equals the oracle's count."""
+    _, oc = s16
+    for r in oc.find_roots(16):
+        _, _, tr = oc.bfs(int(r), want_pred=False)
+        ctx.run_bfs(int(r))
+        per, tot = ctx.model_bytes()
+        assert per == tr["bytes"], int(r)
+        assert tot == tr["total_bytes"]
+
+
+def test_model_bytes_match_oracle_s20(oracle_mod):
+    import graph500_amd
+    say("model bytes s20")
+    oc = oracle_mod.Csr(oracle_mod.generate_edges(20))
+    with graph500_amd.G5Context(0) as c:
+        c.generate_kronecker(20)
+        c.construct(2, release_edges=True)
+        c.prepare_bfs()
+        for r in oc.find_roots(4):
+            _, _, tr = oc.bfs(int(r), want_pred=False)
+            c.run_bfs(int(r))
+            per, tot = c.model_bytes()
+            assert per == tr["bytes"] and tot == tr["total_bytes"]
