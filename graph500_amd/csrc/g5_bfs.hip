@@ -9,11 +9,16 @@
 // Kernels (vertex ids are degree-ordered "new" ids, rows [0, n_rows)):
 //   top-down   k_td_expand: the frontier's rows are concatenated into one
 //              virtual edge list (exclusive scan of frontier degrees); each
-//              workgroup takes 1024-edge chunks, stages the prefix/row-base/
+//              workgroup takes kEpb-edge chunks, stages the prefix/row-base/
 //              source of the frontier vertices overlapping its chunk in LDS and
-//              binary-searches there, so hub rows are split across the whole
-//              grid.  Claims are atomicOr test-and-set on the visited bitmap;
-//              winners are appended to the next queue with one atomic per wave.
+//              binary-searches there once per lane, so hub rows are split
+//              across the whole grid.  Small levels test-and-set the visited
+//              bitmap and append winners through LDS (one global atomic per
+//              chunk); hub levels use no atomics: the visited bitmap is a
+//              read-only snapshot, new targets get level/parent with benign
+//              races, and one pass over the level array gives the frontier.
+//   queues     a top-down frontier that comes from a bitmap is rebuilt by an
+//              exclusive scan of per-word popcounts (ascending vertex order).
 //   bottom-up  k_bu: one wave per 64-vertex bitmap word, one lane per vertex.
 //              Unvisited lanes first probe their first (highest-degree)
 //              neighbour from the coalesced first_nbr array; misses with a
@@ -26,7 +31,9 @@
 
 namespace g5 {
 
-constexpr int kEpb = 1024;       // top-down edges per workgroup chunk
+constexpr int kEpt = 4;                 // top-down edges per lane and chunk
+constexpr int kEpb = kEpt * kBlock;     // top-down edges per workgroup chunk
+constexpr uint64_t kTdSmall = 1 << 20;  // top-down levels up to this many edges use the atomic path
 constexpr int kShortRow = 32;    // bottom-up: lane-sequential scan up to this length
 constexpr uint32_t kNoNbr = 0xFFFFFFFFu;
 
@@ -78,41 +85,77 @@ struct FrontierDegree {
   }
 };
 
-__device__ __forceinline__ void wave_append(bool take, uint32_t value, uint32_t* __restrict__ out,
-                                            unsigned long long* __restrict__ counter) {
-  const uint64_t m = __ballot(take);
-  if (!m) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  uint64_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-  base = __shfl(base, leader, 64);
-  if (take) out[base + __popcll(m & lanemask_lt())] = value;
+__device__ __forceinline__ bool bit_set(const uint64_t* __restrict__ bm, uint32_t v) {
+  return (bm[v >> 6] >> (v & 63)) & 1;
 }
 
+// Block-wide sum added to *acc with one atomic per workgroup.
+__device__ __forceinline__ void block_add(uint64_t v, unsigned long long* acc) {
+  __shared__ uint64_t s_part[kBlock / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int i = 0; i < kBlock / 64; ++i) t += s_part[i];
+    if (t) atomicAdd(acc, (unsigned long long)t);
+  }
+}
+
+
+// For every kEpb-edge chunk of the virtual edge list, the first and last
+// frontier index whose row overlaps it (binary searches over the prefix, one
+// lane per chunk boundary, so their latency overlaps across the grid).
+__global__ void __launch_bounds__(kBlock) k_chunk_rows(const uint64_t* __restrict__ prefix, int64_t nf,
+                                                       uint64_t nchunks, uint32_t* __restrict__ rows_out) {
+  const uint64_t E = prefix[nf];
+  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < 2 * nchunks;
+       t += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t c = t >> 1;
+    uint64_t key = c * kEpb;
+    if (t & 1) key = ((key + kEpb < E) ? key + kEpb : E) - 1;
+    int64_t lo = 0, hi = nf - 1;  // last i with prefix[i] <= key
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= key) lo = mid; else hi = mid - 1;
+    }
+    rows_out[t] = (uint32_t)lo;
+  }
+}
+
+// Top-down expansion of one level.  The frontier's rows form one virtual edge
+// list (prefix = exclusive scan of frontier degrees, E = prefix[nf]); every
+// workgroup takes kEpb-edge chunks and each lane walks kEpt consecutive edges
+// after one LDS binary search.
+//   kBulk = false (small levels): test-and-set on the visited bitmap itself
+//           (atomicOr after a plain-load filter); winners are staged in LDS
+//           and appended to 'next' with one global atomic per chunk.
+//   kBulk = true  (hub levels): no atomics.  The visited bitmap stays a
+//           read-only snapshot; an unvisited target gets level and parent
+//           unless its level is already set (racing writers store the same
+//           level and a valid parent); k_td_finish then takes the exact new
+//           set from the level array.
+// Winners write level and parent; the small path counts them in
+// counters[C_FOUND] (k_td_finish does it for the bulk path).
+template <bool kBulk>
 __global__ void __launch_bounds__(kBlock) k_td_expand(
     const uint32_t* __restrict__ queue, int64_t nf, const uint64_t* __restrict__ prefix,
-    const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols, uint64_t* vis,
-    uint8_t* __restrict__ level, uint32_t* __restrict__ parent, uint32_t* __restrict__ next,
-    unsigned long long* __restrict__ counters, uint32_t lvl) {
+    const uint32_t* __restrict__ chunk_rows, const uint64_t* __restrict__ offsets,
+    const uint32_t* __restrict__ cols, uint64_t* vis, uint8_t* level, uint32_t* __restrict__ parent,
+    uint32_t* __restrict__ next, unsigned long long* __restrict__ counters, uint32_t lvl) {
   __shared__ uint64_t s_pre[kEpb + 1];
   __shared__ uint64_t s_base[kEpb + 1];
   __shared__ uint32_t s_src[kEpb + 1];
+  __shared__ uint32_t s_q[kBulk ? 1 : kEpb];
   __shared__ int64_t s_range[2];
+  __shared__ uint32_t s_qn;
+  __shared__ uint64_t s_qbase;
   const uint64_t E = prefix[nf];
   for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
     const uint64_t e0 = chunk * kEpb;
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
-    if (threadIdx.x < 2) {
-      // last i with prefix[i] <= key, over i in [0, nf)
-      const uint64_t key = threadIdx.x == 0 ? e0 : e1 - 1;
-      int64_t lo = 0, hi = nf - 1;
-      while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= key) lo = mid; else hi = mid - 1;
-      }
-      s_range[threadIdx.x] = lo;
-    }
+    if (threadIdx.x < 2) s_range[threadIdx.x] = chunk_rows[2 * chunk + threadIdx.x];
+    if (!kBulk && threadIdx.x == 0) s_qn = 0;
     __syncthreads();
     const int64_t i0 = s_range[0];
     // Every frontier vertex has a non-empty row, so at most kEpb rows overlap
@@ -127,36 +170,62 @@ __global__ void __launch_bounds__(kBlock) k_td_expand(
       s_src[j] = u;
     }
     __syncthreads();
-    for (uint64_t eb = e0; eb < e1; eb += kBlock) {
-      const uint64_t e = eb + threadIdx.x;
-      bool claimed = false;
-      uint32_t w = 0;
-      if (e < e1) {
-        int lo = 0, hi = cnt - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_pre[mid] <= e) lo = mid; else hi = mid - 1;
-        }
-        w = cols[s_base[lo] + e];
+    const uint64_t my0 = e0 + (uint64_t)threadIdx.x * kEpt;
+    if (my0 < e1) {
+      int lo = 0, hi = cnt - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[mid] <= my0) lo = mid; else hi = mid - 1;
+      }
+      const uint64_t my1 = (my0 + kEpt < e1) ? my0 + kEpt : e1;
+      for (uint64_t e = my0; e < my1; ++e) {
+        while (lo + 1 < cnt && s_pre[lo + 1] <= e) ++lo;
+        const uint32_t w = cols[s_base[lo] + e];
         const uint64_t bit = UINT64_C(1) << (w & 63);
-        uint64_t* wp = vis + (w >> 6);
-        if (!(__atomic_load_n(wp, __ATOMIC_RELAXED) & bit)) {
-          const uint64_t old = atomicOr((unsigned long long*)wp, (unsigned long long)bit);
-          claimed = !(old & bit);
-        }
-        if (claimed) {
+        if (vis[w >> 6] & bit) continue;
+        if (kBulk) {
+          // Racing writers all store the same level and a valid parent.
+          if (level[w] != 0xFF) continue;
           level[w] = (uint8_t)lvl;
           parent[w] = s_src[lo];
+        } else {
+          const uint64_t old = atomicOr((unsigned long long*)(vis + (w >> 6)), (unsigned long long)bit);
+          if (old & bit) continue;
+          level[w] = (uint8_t)lvl;
+          parent[w] = s_src[lo];
+          s_q[atomicAdd(&s_qn, 1u)] = w;
         }
       }
-      wave_append(claimed, w, next, counters + C_FOUND);
+    }
+    if (!kBulk) {
+      __syncthreads();
+      if (threadIdx.x == 0) s_qbase = s_qn ? atomicAdd(counters + C_FOUND, (unsigned long long)s_qn) : 0;
+      __syncthreads();
+      for (uint32_t j = threadIdx.x; j < s_qn; j += kBlock) next[s_qbase + j] = s_q[j];
     }
     __syncthreads();
   }
 }
 
-__device__ __forceinline__ bool bit_set(const uint64_t* __restrict__ bm, uint32_t v) {
-  return (bm[v >> 6] >> (v & 63)) & 1;
+// After a bulk top-down level: the vertices whose level became 'lvl' join the
+// visited bitmap and form the frontier bitmap; their number goes to counters.
+__global__ void __launch_bounds__(kBlock) k_td_finish(const uint8_t* __restrict__ level, int64_t rows,
+                                                      int64_t words, uint32_t lvl, uint64_t* __restrict__ vis,
+                                                      uint64_t* __restrict__ fbits,
+                                                      unsigned long long* __restrict__ counters) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  uint64_t total = 0;
+  for (int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < words; w += nwaves) {
+    const int64_t v = w * 64 + lane;
+    const uint64_t nb = __ballot(v < rows && level[v] == (uint8_t)lvl);
+    if (lane == 0) {
+      vis[w] |= nb;
+      fbits[w] = nb;
+    }
+    total += __popcll(nb);
+  }
+  block_add(lane == 0 ? total : 0, counters + C_FOUND);
 }
 
 __global__ void __launch_bounds__(kBlock) k_bu(
@@ -246,33 +315,31 @@ __global__ void __launch_bounds__(kBlock) k_bu(
   }
 }
 
-// Vertices set in 'now' but not in 'before' -> queue (bottom-up -> top-down).
-__global__ void __launch_bounds__(kBlock) k_bitmap_diff_queue(const uint64_t* __restrict__ now,
-                                                              const uint64_t* __restrict__ before,
-                                                              int64_t words, uint32_t* __restrict__ queue,
-                                                              unsigned long long* __restrict__ counters) {
-  const int lane = threadIdx.x & 63;
-  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
-  for (int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < words; w += nwaves) {
-    const uint64_t d = now[w] & ~before[w];
-    const bool take = (d >> lane) & 1;
-    wave_append(take, (uint32_t)(w * 64 + lane), queue, counters + C_FOUND);
+// Frontier bitmap -> frontier queue in ascending vertex order: an exclusive
+// scan of per-word popcounts gives every word its output position, so no
+// atomics are involved.  The bits are 'now & ~before' (before may be null).
+struct FrontierBits {
+  const uint64_t* now;
+  const uint64_t* before;
+  __device__ uint64_t operator()(size_t w) const {
+    const uint64_t x = before ? (now[w] & ~before[w]) : now[w];
+    return (uint64_t)__popcll(x);
   }
-}
+};
+struct FrontierEmit {
+  const uint64_t* now;
+  const uint64_t* before;
+  uint32_t* queue;
+  __device__ void operator()(size_t w, uint64_t pre) const {
+    uint64_t x = before ? (now[w] & ~before[w]) : now[w];
+    while (x) {
+      queue[pre++] = (uint32_t)(w * 64 + __builtin_ctzll(x));
+      x &= x - 1;
+    }
+  }
+};
 
 // --- algorithmic byte model, evaluated from the final levels -------------
-__device__ __forceinline__ void block_add(uint64_t v, unsigned long long* acc) {
-  __shared__ uint64_t s_part[kBlock / 64];
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t t = 0;
-    for (int i = 0; i < kBlock / 64; ++i) t += s_part[i];
-    if (t) atomicAdd(acc, (unsigned long long)t);
-  }
-}
-
 // top-down level L reads the rows of the vertices at level L-1
 __global__ void __launch_bounds__(kBlock) k_model_td(const uint8_t* __restrict__ level,
                                                      const uint64_t* __restrict__ offsets, int64_t rows,
@@ -387,6 +454,7 @@ void bfs_prepare(g5_ctx* ctx) {
   b.parent.alloc(rows);
   for (auto& q : b.queue) q.alloc(rows);
   b.prefix.alloc(rows + 1);
+  b.chunk_rows.alloc(2 * ((uint64_t)g.nnz / kEpb + 2));
   b.scan_tmp.alloc(scan_tmp_elems(rows + 1));
   b.counters.alloc(C_NUM);
   G5_HIP(hipHostMalloc((void**)&b.h_counters, C_NUM * sizeof(uint64_t), hipHostMallocDefault));
@@ -425,7 +493,7 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
   // A root without edges (reachable only through a find_roots fallback) is
   // the whole tree: the reference's first top-down level finds nothing.
   const bool root_active = (g.has_edge_words[root >> 6] >> (root & 63)) & 1;
-  int cur = 0, qc = 0;
+  int cur = 0, qi = 0;
   int64_t nf = 1;
   bool next_forward = true, growing = true;
   int64_t prev_nq = 0, visited = 1;
@@ -442,12 +510,34 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     const bool forward = next_forward;
     G5_HIP(hipMemsetAsync(b.counters.p, 0, 8, s));
     G5_HIP(hipEventRecord(b.lev_begin[lvl], s));
+    bool bulk = false;
     if (forward) {
-      device_scan(FrontierDegree{b.queue[qc].p, g.offsets.p, nf}, StoreU64{b.prefix.p},
+      device_scan(FrontierDegree{b.queue[qi].p, g.offsets.p, nf}, StoreU64{b.prefix.p},
                   (size_t)nf + 1, b.scan_tmp.p, s);
-      hipLaunchKernelGGL(k_td_expand, dim3(2048), dim3(kBlock), 0, s, b.queue[qc].p, nf, b.prefix.p,
-                         g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p, b.parent.p, b.queue[qc ^ 1].p,
-                         cnt, (uint32_t)lvl);
+      G5_HIP(hipMemcpyAsync(b.h_counters + 1, b.prefix.p + nf, 8, hipMemcpyDeviceToHost, s));
+      G5_HIP(hipStreamSynchronize(s));
+      const uint64_t E = b.h_counters[1];
+      const unsigned grid = grid_of((int64_t)E, kEpb, 2048);
+      const uint64_t nchunks = (E + kEpb - 1) / kEpb;
+      G5_CHECK(2 * nchunks <= b.chunk_rows.n, "g5_run_bfs: top-down chunk table too small");
+      if (nchunks)
+        hipLaunchKernelGGL(k_chunk_rows, dim3(grid_of((int64_t)(2 * nchunks), kBlock, 65536)), dim3(kBlock),
+                           0, s, b.prefix.p, nf, nchunks, b.chunk_rows.p);
+      bulk = E > kTdSmall;
+      if (!bulk) {
+        hipLaunchKernelGGL(k_td_expand<false>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf,
+                           b.prefix.p, b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p,
+                           b.parent.p, b.queue[qi ^ 1].p, cnt, (uint32_t)lvl);
+      } else {
+        // k_td_finish makes vis[cur^1] exactly this level's vertices and adds
+        // them to vis[cur]
+        hipLaunchKernelGGL(k_td_expand<true>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf,
+                           b.prefix.p, b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p,
+                           b.parent.p, (uint32_t*)nullptr, cnt, (uint32_t)lvl);
+        G5_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_td_finish, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
+                           b.level.p, rows, b.words, (uint32_t)lvl, b.vis[cur].p, b.vis[cur ^ 1].p, cnt);
+      }
     } else {
       hipLaunchKernelGGL(k_bu, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
                          b.vis[cur].p, b.vis[cur ^ 1].p, g.first_nbr.p, g.offsets.p, g.cols.p,
@@ -479,17 +569,17 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
       }
     }
     prev_nq = nq;
-    if (forward) {
-      qc ^= 1;
-    } else {
-      if (next_forward) {
-        G5_HIP(hipMemsetAsync(b.counters.p, 0, 8, s));
-        hipLaunchKernelGGL(k_bitmap_diff_queue, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock),
-                           0, s, b.vis[cur ^ 1].p, b.vis[cur].p, b.words, b.queue[qc].p, cnt);
-        G5_HIP(hipGetLastError());
-      }
-      cur ^= 1;
+    // The next top-down level needs this level's vertices as a queue.  A small
+    // top-down level appended them to queue[qi^1]; a bulk one left them in
+    // vis[cur^1]; a bottom-up level left them as vis[cur^1] & ~vis[cur].
+    if (forward && !bulk) {
+      qi ^= 1;
+    } else if (next_forward) {
+      const uint64_t* before = forward ? nullptr : b.vis[cur].p;
+      device_scan(FrontierBits{b.vis[cur ^ 1].p, before},
+                  FrontierEmit{b.vis[cur ^ 1].p, before, b.queue[qi].p}, (size_t)b.words, b.scan_tmp.p, s);
     }
+    if (!forward) cur ^= 1;
     nf = nq;
   }
   G5_HIP(hipEventRecord(b.ev1, s));

@@ -51,6 +51,7 @@ struct BfsState {
   DevBuf<uint32_t> parent;        // n_rows
   DevBuf<uint32_t> queue[2];      // n_rows
   DevBuf<uint64_t> prefix;        // n_rows + 1 (top-down edge prefix)
+  DevBuf<uint32_t> chunk_rows;    // 2 per top-down edge chunk: first/last frontier index
   DevBuf<uint64_t> scan_tmp;
   DevBuf<uint64_t> counters;      // device counters (see g5_bfs.hip)
   uint64_t* h_counters = nullptr; // pinned host mirror
@@ -65,6 +66,7 @@ struct BfsState {
     parent.release();
     for (auto& q : queue) q.release();
     prefix.release();
+    chunk_rows.release();
     scan_tmp.release();
     counters.release();
     pred_out.release();
