@@ -29,6 +29,8 @@
 #include "g5_ctx.hpp"
 #include "g5_prims.hpp"
 
+#include <algorithm>
+
 namespace g5 {
 
 constexpr int kEpt = 4;                 // top-down edges per lane and chunk
@@ -123,80 +125,98 @@ __global__ void __launch_bounds__(kBlock) k_chunk_rows(const uint64_t* __restric
   }
 }
 
+// LDS view of the frontier rows overlapping one kEpb-edge chunk.
+struct ChunkRows {
+  uint64_t pre[kEpb + 1];   // first virtual edge index of each row
+  uint64_t base[kEpb + 1];  // offsets[u] - pre: cols index = base + virtual index
+  uint32_t src[kEpb + 1];   // the frontier vertex u
+};
+
+// Stages the rows of chunk c (every lane of the block must call it; it ends
+// with a barrier) and returns their number.  Every frontier vertex has a
+// non-empty row, so at most kEpb rows overlap a chunk; the clamp only keeps
+// LDS indexing in bounds.
+__device__ __forceinline__ int stage_chunk(uint64_t c, const uint32_t* __restrict__ chunk_rows,
+                                           const uint32_t* __restrict__ queue,
+                                           const uint64_t* __restrict__ prefix,
+                                           const uint64_t* __restrict__ offsets, ChunkRows& cr) {
+  const int64_t i0 = chunk_rows[2 * c];
+  int cnt = (int)((int64_t)chunk_rows[2 * c + 1] - i0 + 1);
+  cnt = cnt > kEpb + 1 ? kEpb + 1 : cnt;
+  for (int j = threadIdx.x; j < cnt; j += kBlock) {
+    const uint32_t u = queue[i0 + j];
+    const uint64_t p = prefix[i0 + j];
+    cr.pre[j] = p;
+    cr.base[j] = offsets[u] - p;
+    cr.src[j] = u;
+  }
+  __syncthreads();
+  return cnt;
+}
+
+// Calls f(target, source) for this lane's kEpt consecutive edges of [e0, e1):
+// one LDS binary search, then a forward walk across row boundaries.
+template <class F>
+__device__ __forceinline__ void walk_lane(uint64_t e0, uint64_t e1, int cnt, const ChunkRows& cr,
+                                          const uint32_t* __restrict__ cols, F&& f) {
+  const uint64_t my0 = e0 + (uint64_t)threadIdx.x * kEpt;
+  if (my0 >= e1) return;
+  int lo = 0, hi = cnt - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cr.pre[mid] <= my0) lo = mid; else hi = mid - 1;
+  }
+  const uint64_t my1 = (my0 + kEpt < e1) ? my0 + kEpt : e1;
+  for (uint64_t e = my0; e < my1; ++e) {
+    while (lo + 1 < cnt && cr.pre[lo + 1] <= e) ++lo;
+    f(cols[cr.base[lo] + e], cr.src[lo]);
+  }
+}
+
 // Top-down expansion of one level.  The frontier's rows form one virtual edge
 // list (prefix = exclusive scan of frontier degrees, E = prefix[nf]); every
-// workgroup takes kEpb-edge chunks and each lane walks kEpt consecutive edges
-// after one LDS binary search.
+// workgroup takes kEpb-edge chunks and each lane walks kEpt consecutive edges.
 //   kBulk = false (small levels): test-and-set on the visited bitmap itself
 //           (atomicOr after a plain-load filter); winners are staged in LDS
-//           and appended to 'next' with one global atomic per chunk.
-//   kBulk = true  (hub levels): no atomics.  The visited bitmap stays a
-//           read-only snapshot; an unvisited target gets level and parent
-//           unless its level is already set (racing writers store the same
-//           level and a valid parent); k_td_finish then takes the exact new
-//           set from the level array.
-// Winners write level and parent; the small path counts them in
-// counters[C_FOUND] (k_td_finish does it for the bulk path).
+//           and appended to 'next' with one global atomic per chunk, which
+//           also counts them in counters[C_FOUND].
+//   kBulk = true  (hub levels): no atomics and no bitmap probe (the snapshot
+//           holds few vertices at such a level): a target gets level and
+//           parent unless its level is already set (racing writers store the
+//           same level and a valid parent); k_td_finish then takes the exact
+//           new set from the level array.
 template <bool kBulk>
 __global__ void __launch_bounds__(kBlock) k_td_expand(
     const uint32_t* __restrict__ queue, int64_t nf, const uint64_t* __restrict__ prefix,
     const uint32_t* __restrict__ chunk_rows, const uint64_t* __restrict__ offsets,
     const uint32_t* __restrict__ cols, uint64_t* vis, uint8_t* level, uint32_t* __restrict__ parent,
     uint32_t* __restrict__ next, unsigned long long* __restrict__ counters, uint32_t lvl) {
-  __shared__ uint64_t s_pre[kEpb + 1];
-  __shared__ uint64_t s_base[kEpb + 1];
-  __shared__ uint32_t s_src[kEpb + 1];
+  __shared__ ChunkRows cr;
   __shared__ uint32_t s_q[kBulk ? 1 : kEpb];
-  __shared__ int64_t s_range[2];
   __shared__ uint32_t s_qn;
   __shared__ uint64_t s_qbase;
   const uint64_t E = prefix[nf];
   for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
     const uint64_t e0 = chunk * kEpb;
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
-    if (threadIdx.x < 2) s_range[threadIdx.x] = chunk_rows[2 * chunk + threadIdx.x];
     if (!kBulk && threadIdx.x == 0) s_qn = 0;
-    __syncthreads();
-    const int64_t i0 = s_range[0];
-    // Every frontier vertex has a non-empty row, so at most kEpb rows overlap
-    // a chunk; the clamp only keeps LDS indexing in bounds.
-    int cnt = (int)(s_range[1] - i0 + 1);
-    cnt = cnt > kEpb + 1 ? kEpb + 1 : cnt;
-    for (int j = threadIdx.x; j < cnt; j += kBlock) {
-      const uint32_t u = queue[i0 + j];
-      const uint64_t p = prefix[i0 + j];
-      s_pre[j] = p;
-      s_base[j] = offsets[u] - p;
-      s_src[j] = u;
-    }
-    __syncthreads();
-    const uint64_t my0 = e0 + (uint64_t)threadIdx.x * kEpt;
-    if (my0 < e1) {
-      int lo = 0, hi = cnt - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_pre[mid] <= my0) lo = mid; else hi = mid - 1;
+    const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
+    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
+      const uint64_t bit = UINT64_C(1) << (w & 63);
+      // a hub level's snapshot holds few vertices: the level test alone decides
+      if (!kBulk && (vis[w >> 6] & bit)) return;
+      if (kBulk) {
+        if (level[w] != 0xFF) return;
+        level[w] = (uint8_t)lvl;
+        parent[w] = u;
+      } else {
+        const uint64_t old = atomicOr((unsigned long long*)(vis + (w >> 6)), (unsigned long long)bit);
+        if (old & bit) return;
+        level[w] = (uint8_t)lvl;
+        parent[w] = u;
+        s_q[atomicAdd(&s_qn, 1u)] = w;
       }
-      const uint64_t my1 = (my0 + kEpt < e1) ? my0 + kEpt : e1;
-      for (uint64_t e = my0; e < my1; ++e) {
-        while (lo + 1 < cnt && s_pre[lo + 1] <= e) ++lo;
-        const uint32_t w = cols[s_base[lo] + e];
-        const uint64_t bit = UINT64_C(1) << (w & 63);
-        if (vis[w >> 6] & bit) continue;
-        if (kBulk) {
-          // Racing writers all store the same level and a valid parent.
-          if (level[w] != 0xFF) continue;
-          level[w] = (uint8_t)lvl;
-          parent[w] = s_src[lo];
-        } else {
-          const uint64_t old = atomicOr((unsigned long long*)(vis + (w >> 6)), (unsigned long long)bit);
-          if (old & bit) continue;
-          level[w] = (uint8_t)lvl;
-          parent[w] = s_src[lo];
-          s_q[atomicAdd(&s_qn, 1u)] = w;
-        }
-      }
-    }
+    });
     if (!kBulk) {
       __syncthreads();
       if (threadIdx.x == 0) s_qbase = s_qn ? atomicAdd(counters + C_FOUND, (unsigned long long)s_qn) : 0;
@@ -205,6 +225,69 @@ __global__ void __launch_bounds__(kBlock) k_td_expand(
     }
     __syncthreads();
   }
+}
+
+// First neighbour of v, in ascending (highest degree first) order, whose bit
+// is set in bm: the bottom-up probe.  Every lane of the wave calls it; lanes
+// with active=false only help.  The first neighbour comes from the coalesced
+// first_nbr array; rows up to kShortRow are then scanned by their lane, longer
+// rows by the whole wave 64 entries at a time with a ballot for the first hit.
+__device__ __forceinline__ bool first_marked_nbr(bool active, uint32_t v, const uint64_t* __restrict__ bm,
+                                                 const uint32_t* __restrict__ first_nbr,
+                                                 const uint64_t* __restrict__ offsets,
+                                                 const uint32_t* __restrict__ cols, uint32_t& p) {
+  const int lane = threadIdx.x & 63;
+  bool found = false, need = false;
+  uint64_t rb = 0, re = 0;
+  if (active) {
+    const uint32_t f = first_nbr[v];
+    if (f != kNoNbr) {
+      if (bit_set(bm, f)) {
+        found = true;
+        p = f;
+      } else {
+        rb = offsets[v] + 1;
+        re = offsets[v + 1];
+        need = rb < re;
+      }
+    }
+  }
+  if (need && re - rb <= (uint64_t)kShortRow) {
+    for (uint64_t k = rb; k < re; ++k) {
+      const uint32_t c = cols[k];
+      if (bit_set(bm, c)) {
+        found = true;
+        p = c;
+        break;
+      }
+    }
+    need = false;
+  }
+  uint64_t lm = __ballot(need);
+  while (lm) {
+    const int src = __ffsll((unsigned long long)lm) - 1;
+    lm &= lm - 1;
+    const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
+    for (uint64_t base = b; base < e; base += 64) {
+      const uint64_t k = base + lane;
+      uint32_t c = 0;
+      bool hit = false;
+      if (k < e) {
+        c = cols[k];
+        hit = bit_set(bm, c);
+      }
+      const uint64_t hm = __ballot(hit);
+      if (hm) {
+        const uint32_t hp = __shfl(c, __ffsll((unsigned long long)hm) - 1, 64);
+        if (lane == src) {
+          found = true;
+          p = hp;
+        }
+        break;
+      }
+    }
+  }
+  return found;
 }
 
 // After a bulk top-down level: the vertices whose level became 'lvl' join the
@@ -233,7 +316,6 @@ __global__ void __launch_bounds__(kBlock) k_bu(
     const uint32_t* __restrict__ first_nbr, const uint64_t* __restrict__ offsets,
     const uint32_t* __restrict__ cols, uint8_t* __restrict__ level, uint32_t* __restrict__ parent,
     int64_t words, uint32_t lvl, unsigned long long* __restrict__ counters) {
-  __shared__ uint64_t s_cnt[kBlock / 64];
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
   uint64_t found_total = 0;
@@ -244,60 +326,8 @@ __global__ void __launch_bounds__(kBlock) k_bu(
       continue;
     }
     const uint32_t v = (uint32_t)(w * 64 + lane);
-    const bool unvisited = !((cur >> lane) & 1);
-    bool found = false;
     uint32_t p = 0;
-    bool need = false;
-    uint64_t rb = 0, re = 0;
-    if (unvisited) {
-      const uint32_t f = first_nbr[v];
-      if (f != kNoNbr) {
-        if (bit_set(vis, f)) {
-          found = true;
-          p = f;
-        } else {
-          rb = offsets[v] + 1;
-          re = offsets[v + 1];
-          need = rb < re;
-        }
-      }
-    }
-    if (need && re - rb <= (uint64_t)kShortRow) {
-      for (uint64_t k = rb; k < re; ++k) {
-        const uint32_t c = cols[k];
-        if (bit_set(vis, c)) {
-          found = true;
-          p = c;
-          break;
-        }
-      }
-      need = false;
-    }
-    uint64_t lm = __ballot(need);
-    while (lm) {
-      const int src = __ffsll((unsigned long long)lm) - 1;
-      lm &= lm - 1;
-      const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
-      for (uint64_t base = b; base < e; base += 64) {
-        const uint64_t k = base + lane;
-        uint32_t c = 0;
-        bool hit = false;
-        if (k < e) {
-          c = cols[k];
-          hit = bit_set(vis, c);
-        }
-        const uint64_t hm = __ballot(hit);
-        if (hm) {
-          const int first = __ffsll((unsigned long long)hm) - 1;
-          const uint32_t hp = __shfl(c, first, 64);
-          if (lane == src) {
-            found = true;
-            p = hp;
-          }
-          break;
-        }
-      }
-    }
+    const bool found = first_marked_nbr(!((cur >> lane) & 1), v, vis, first_nbr, offsets, cols, p);
     const uint64_t fm = __ballot(found);
     if (lane == 0) vis_next[w] = cur | fm;
     if (found) {
@@ -306,13 +336,7 @@ __global__ void __launch_bounds__(kBlock) k_bu(
     }
     found_total += __popcll(fm);  // identical in every lane
   }
-  if (lane == 0) s_cnt[threadIdx.x >> 6] = found_total;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t t = 0;
-    for (int i = 0; i < kBlock / 64; ++i) t += s_cnt[i];
-    if (t) atomicAdd(counters + C_FOUND, (unsigned long long)t);
-  }
+  block_add(lane == 0 ? found_total : 0, counters + C_FOUND);
 }
 
 // Frontier bitmap -> frontier queue in ascending vertex order: an exclusive
@@ -455,7 +479,8 @@ void bfs_prepare(g5_ctx* ctx) {
   for (auto& q : b.queue) q.alloc(rows);
   b.prefix.alloc(rows + 1);
   b.chunk_rows.alloc(2 * ((uint64_t)g.nnz / kEpb + 2));
-  b.scan_tmp.alloc(scan_tmp_elems(rows + 1));
+  const size_t scan_n = (size_t)rows + 1;
+  b.scan_tmp.alloc(scan_tmp_elems(scan_n));
   b.counters.alloc(C_NUM);
   G5_HIP(hipHostMalloc((void**)&b.h_counters, C_NUM * sizeof(uint64_t), hipHostMallocDefault));
   G5_HIP(hipEventCreate(&b.ev0));
@@ -529,12 +554,12 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
                            b.prefix.p, b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p,
                            b.parent.p, b.queue[qi ^ 1].p, cnt, (uint32_t)lvl);
       } else {
-        // k_td_finish makes vis[cur^1] exactly this level's vertices and adds
-        // them to vis[cur]
         hipLaunchKernelGGL(k_td_expand<true>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf,
                            b.prefix.p, b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p,
                            b.parent.p, (uint32_t*)nullptr, cnt, (uint32_t)lvl);
         G5_HIP(hipGetLastError());
+        // k_td_finish makes vis[cur^1] exactly this level's vertices and adds
+        // them to vis[cur]
         hipLaunchKernelGGL(k_td_finish, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
                            b.level.p, rows, b.words, (uint32_t)lvl, b.vis[cur].p, b.vis[cur ^ 1].p, cnt);
       }
