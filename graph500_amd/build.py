@@ -17,7 +17,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libg5.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["g5_gen.hip", "g5_prims.hip", "g5_csr.hip", "g5_bfs.hip", "g5_api.hip"]
+SOURCES = ["g5_gen.hip", "g5_prims.hip", "g5_csr.hip", "g5_bfs.hip", "g5_msbfs.hip", "g5_validate.hip",
+           "g5_api.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

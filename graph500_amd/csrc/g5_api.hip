@@ -67,6 +67,7 @@ void g5_destroy(g5_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->bfs.release();
+  ctx->ms.release();
   ctx->graph.clear();
   ctx->edges.release();
   ctx->acc.release();
@@ -178,6 +179,7 @@ int g5_construct(g5_ctx* ctx, int reorder_mode, int release_edges) {
   return guarded(ctx, [&] {
     set_device(ctx);
     ctx->bfs.release();
+    ctx->ms.release();
     build_graph(ctx, reorder_mode, release_edges != 0);
   });
 }
@@ -336,11 +338,105 @@ int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total) {
   });
 }
 
+int g5_validate(g5_ctx* ctx, int64_t* errors, int* valid) {
+  return guarded(ctx, [&] {
+    G5_CHECK(errors && valid, "g5_validate: null output");
+    set_device(ctx);
+    bfs_validate(ctx, errors);
+    *valid = 1;
+    for (int i = 0; i < 6; ++i)
+      if (errors[i]) *valid = 0;
+  });
+}
+
+int g5_validate_pred(g5_ctx* ctx, int64_t root, const int64_t* pred, int64_t* errors, int* valid,
+                     int64_t* edge_visit_count) {
+  return guarded(ctx, [&] {
+    G5_CHECK(pred && errors && valid, "g5_validate_pred: null argument");
+    set_device(ctx);
+    const int64_t visited = pred_validate(ctx, root, pred, errors);
+    *valid = 1;
+    for (int i = 0; i < 6; ++i)
+      if (errors[i]) *valid = 0;
+    if (edge_visit_count) *edge_visit_count = visited;
+  });
+}
+
 int g5_end_bfs(g5_ctx* ctx) {
   return guarded(ctx, [&] {
     set_device(ctx);
     G5_HIP(hipStreamSynchronize(ctx->stream));
     ctx->bfs.release();
+  });
+}
+
+int g5_batch_prepare(g5_ctx* ctx) {
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    ms_prepare(ctx);
+  });
+}
+
+int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* t) {
+  return guarded(ctx, [&] {
+    G5_CHECK(t && t->alpha > 0 && t->pull_grid >= 1, "g5_batch_set_tuning: bad tuning");
+    ctx->ms.alpha = t->alpha;
+    ctx->ms.pull_grid = t->pull_grid;
+  });
+}
+
+int g5_batch_get_tuning(g5_ctx* ctx, g5_batch_tuning* t) {
+  return guarded(ctx, [&] {
+    G5_CHECK(t, "g5_batch_get_tuning: null output");
+    t->alpha = ctx->ms.alpha;
+    t->pull_grid = ctx->ms.pull_grid;
+  });
+}
+
+int g5_run_bfs_batch(g5_ctx* ctx, const int64_t* roots, int num_roots) {
+  return guarded(ctx, [&] {
+    G5_CHECK(roots, "g5_run_bfs_batch: null roots");
+    set_device(ctx);
+    ms_run(ctx, roots, num_roots);
+  });
+}
+
+int g5_batch_get_levels(g5_ctx* ctx, int lane, uint8_t* levels) {
+  return guarded(ctx, [&] {
+    G5_CHECK(levels, "g5_batch_get_levels: null output");
+    set_device(ctx);
+    ms_get_levels(ctx, lane, levels);
+  });
+}
+
+int g5_batch_teps_edges(g5_ctx* ctx, int64_t* counts) {
+  return guarded(ctx, [&] {
+    G5_CHECK(counts, "g5_batch_teps_edges: null output");
+    set_device(ctx);
+    ms_teps_edges(ctx, counts);
+  });
+}
+
+int g5_batch_stats_get(g5_ctx* ctx, g5_batch_stats* out) {
+  return guarded(ctx, [&] {
+    G5_CHECK(out, "g5_batch_stats_get: null output");
+    if (!ctx->ms.have_result) throw Error{"g5_batch_stats_get: no batch result", G5_ERR_STATE};
+    *out = ctx->ms.stats;
+  });
+}
+
+int g5_batch_model_bytes(g5_ctx* ctx, double* per_level, double* total) {
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    ms_model_bytes(ctx, per_level, total);
+  });
+}
+
+int g5_batch_end(g5_ctx* ctx) {
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    G5_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->ms.release();
   });
 }
 

@@ -27,25 +27,18 @@
 //              hit.  The visited snapshot of the previous level is read-only;
 //              the next bitmap word is written once per wave from a ballot.
 #include "g5_ctx.hpp"
+#include "g5_frontier.hpp"
 #include "g5_prims.hpp"
 
 #include <algorithm>
 
 namespace g5 {
 
-constexpr int kEpt = 4;                 // top-down edges per lane and chunk
-constexpr int kEpb = kEpt * kBlock;     // top-down edges per workgroup chunk
 constexpr uint64_t kTdSmall = 1 << 20;  // top-down levels up to this many edges use the atomic path
 constexpr int kShortRow = 32;    // bottom-up: lane-sequential scan up to this length
 constexpr uint32_t kNoNbr = 0xFFFFFFFFu;
 
 enum { C_FOUND = 0, C_AUX = 1, C_NUM = 4 };
-
-static unsigned grid_of(int64_t n, int64_t per, int64_t cap) {
-  int64_t b = (n + per - 1) / per;
-  if (b > cap) b = cap;
-  return (unsigned)(b > 0 ? b : 1);
-}
 
 
 __global__ void __launch_bounds__(kBlock) k_bfs_init(uint8_t* __restrict__ level, uint64_t* __restrict__ vis,
@@ -72,104 +65,6 @@ __global__ void k_bfs_root(const uint32_t* __restrict__ old2new, int64_t root_or
     parent[r] = r;
     vis[r >> 6] |= UINT64_C(1) << (r & 63);
     queue[0] = r;
-  }
-}
-
-// prefix input: degree of frontier vertex i (0 past the end).
-struct FrontierDegree {
-  const uint32_t* queue;
-  const uint64_t* offsets;
-  int64_t nf;
-  __device__ uint64_t operator()(size_t i) const {
-    if ((int64_t)i >= nf) return 0;
-    const uint32_t u = queue[i];
-    return offsets[u + 1] - offsets[u];
-  }
-};
-
-__device__ __forceinline__ bool bit_set(const uint64_t* __restrict__ bm, uint32_t v) {
-  return (bm[v >> 6] >> (v & 63)) & 1;
-}
-
-// Block-wide sum added to *acc with one atomic per workgroup.
-__device__ __forceinline__ void block_add(uint64_t v, unsigned long long* acc) {
-  __shared__ uint64_t s_part[kBlock / 64];
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t t = 0;
-    for (int i = 0; i < kBlock / 64; ++i) t += s_part[i];
-    if (t) atomicAdd(acc, (unsigned long long)t);
-  }
-}
-
-
-// For every kEpb-edge chunk of the virtual edge list, the first and last
-// frontier index whose row overlaps it (binary searches over the prefix, one
-// lane per chunk boundary, so their latency overlaps across the grid).
-__global__ void __launch_bounds__(kBlock) k_chunk_rows(const uint64_t* __restrict__ prefix, int64_t nf,
-                                                       uint64_t nchunks, uint32_t* __restrict__ rows_out) {
-  const uint64_t E = prefix[nf];
-  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < 2 * nchunks;
-       t += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t c = t >> 1;
-    uint64_t key = c * kEpb;
-    if (t & 1) key = ((key + kEpb < E) ? key + kEpb : E) - 1;
-    int64_t lo = 0, hi = nf - 1;  // last i with prefix[i] <= key
-    while (lo < hi) {
-      const int64_t mid = (lo + hi + 1) >> 1;
-      if (prefix[mid] <= key) lo = mid; else hi = mid - 1;
-    }
-    rows_out[t] = (uint32_t)lo;
-  }
-}
-
-// LDS view of the frontier rows overlapping one kEpb-edge chunk.
-struct ChunkRows {
-  uint64_t pre[kEpb + 1];   // first virtual edge index of each row
-  uint64_t base[kEpb + 1];  // offsets[u] - pre: cols index = base + virtual index
-  uint32_t src[kEpb + 1];   // the frontier vertex u
-};
-
-// Stages the rows of chunk c (every lane of the block must call it; it ends
-// with a barrier) and returns their number.  Every frontier vertex has a
-// non-empty row, so at most kEpb rows overlap a chunk; the clamp only keeps
-// LDS indexing in bounds.
-__device__ __forceinline__ int stage_chunk(uint64_t c, const uint32_t* __restrict__ chunk_rows,
-                                           const uint32_t* __restrict__ queue,
-                                           const uint64_t* __restrict__ prefix,
-                                           const uint64_t* __restrict__ offsets, ChunkRows& cr) {
-  const int64_t i0 = chunk_rows[2 * c];
-  int cnt = (int)((int64_t)chunk_rows[2 * c + 1] - i0 + 1);
-  cnt = cnt > kEpb + 1 ? kEpb + 1 : cnt;
-  for (int j = threadIdx.x; j < cnt; j += kBlock) {
-    const uint32_t u = queue[i0 + j];
-    const uint64_t p = prefix[i0 + j];
-    cr.pre[j] = p;
-    cr.base[j] = offsets[u] - p;
-    cr.src[j] = u;
-  }
-  __syncthreads();
-  return cnt;
-}
-
-// Calls f(target, source) for this lane's kEpt consecutive edges of [e0, e1):
-// one LDS binary search, then a forward walk across row boundaries.
-template <class F>
-__device__ __forceinline__ void walk_lane(uint64_t e0, uint64_t e1, int cnt, const ChunkRows& cr,
-                                          const uint32_t* __restrict__ cols, F&& f) {
-  const uint64_t my0 = e0 + (uint64_t)threadIdx.x * kEpt;
-  if (my0 >= e1) return;
-  int lo = 0, hi = cnt - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (cr.pre[mid] <= my0) lo = mid; else hi = mid - 1;
-  }
-  const uint64_t my1 = (my0 + kEpt < e1) ? my0 + kEpt : e1;
-  for (uint64_t e = my0; e < my1; ++e) {
-    while (lo + 1 < cnt && cr.pre[lo + 1] <= e) ++lo;
-    f(cols[cr.base[lo] + e], cr.src[lo]);
   }
 }
 
@@ -431,12 +326,6 @@ void bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total) {
   if (total) *total = tot;
 }
 
-struct ReachedWeight {
-  const uint8_t* level;
-  const uint32_t* weight;
-  __device__ uint64_t operator()(size_t i) const { return level[i] != 0xFF ? weight[i] : 0; }
-};
-
 __global__ void k_make_pred(const uint32_t* __restrict__ old2new, const uint32_t* __restrict__ new2old,
                             const uint8_t* __restrict__ level, const uint32_t* __restrict__ parent,
                             int64_t n, int64_t rows, int64_t root_orig, int64_t* __restrict__ pred) {
@@ -546,7 +435,7 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
       const uint64_t nchunks = (E + kEpb - 1) / kEpb;
       G5_CHECK(2 * nchunks <= b.chunk_rows.n, "g5_run_bfs: top-down chunk table too small");
       if (nchunks)
-        hipLaunchKernelGGL(k_chunk_rows, dim3(grid_of((int64_t)(2 * nchunks), kBlock, 65536)), dim3(kBlock),
+        hipLaunchKernelGGL(k_chunk_rows<kEpb>, dim3(grid_of((int64_t)(2 * nchunks), kBlock, 65536)), dim3(kBlock),
                            0, s, b.prefix.p, nf, nchunks, b.chunk_rows.p);
       bulk = E > kTdSmall;
       if (!bulk) {

@@ -3,6 +3,8 @@
 
 #include "g5_internal.hpp"
 
+#include <memory>
+
 namespace g5 {
 
 // Canonical CSR of one graph in HBM (see g5_internal.hpp for the layout).
@@ -85,6 +87,54 @@ struct BfsState {
   }
 };
 
+// Batched (up to 64 roots) BFS state.
+struct MsState {
+  bool prepared = false;
+  bool have_result = false;
+  int nroots = 0;
+  int nlevels = 0;
+  std::vector<int64_t> host_roots;
+  double alpha = 16.0;       // push while the frontier union has < nnz/alpha edges
+  int64_t pull_grid = 8192;  // workgroups of the pull sweep
+  DevBuf<uint64_t> seen;     // n_rows
+  DevBuf<uint64_t> nz[2];    // frontier-word-not-zero bitmaps, alternating by level
+  std::vector<std::unique_ptr<DevBuf<uint64_t>>> fr;  // frontier words per level
+  DevBuf<const uint64_t*> frp;  // device copy of the fr pointers
+  DevBuf<uint32_t> queue;
+  DevBuf<uint64_t> prefix;
+  DevBuf<uint32_t> chunk_rows;
+  DevBuf<uint64_t> scan_tmp;
+  DevBuf<uint64_t> counters;
+  DevBuf<int64_t> roots;
+  uint64_t* h_counters = nullptr;
+  g5_batch_stats stats{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> lev_begin, lev_end;
+  void release() {
+    seen.release();
+    for (auto& b : nz) b.release();
+    fr.clear();
+    frp.release();
+    queue.release();
+    prefix.release();
+    chunk_rows.release();
+    scan_tmp.release();
+    counters.release();
+    roots.release();
+    if (h_counters) (void)hipHostFree(h_counters);
+    h_counters = nullptr;
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    ev0 = ev1 = nullptr;
+    for (auto e : lev_begin) (void)hipEventDestroy(e);
+    for (auto e : lev_end) (void)hipEventDestroy(e);
+    lev_begin.clear();
+    lev_end.clear();
+    prepared = false;
+    have_result = false;
+  }
+};
+
 }  // namespace g5
 
 struct g5_ctx {
@@ -97,6 +147,7 @@ struct g5_ctx {
   double generate_ms = 0;
   g5::Graph graph;
   g5::BfsState bfs;
+  g5::MsState ms;
   g5::DevBuf<uint64_t> acc;  // small reduction scratch
 };
 
@@ -113,4 +164,13 @@ void bfs_get_pred(g5_ctx* ctx, int64_t* host_pred);
 void bfs_get_levels(g5_ctx* ctx, uint8_t* host_levels);
 int64_t bfs_teps_edges(g5_ctx* ctx);
 void bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
+// g5_validate.hip
+void bfs_validate(g5_ctx* ctx, int64_t* errors_out);
+int64_t pred_validate(g5_ctx* ctx, int64_t root, const int64_t* host_pred, int64_t* errors_out);
+// g5_msbfs.hip
+void ms_prepare(g5_ctx* ctx);
+void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots);
+void ms_get_levels(g5_ctx* ctx, int lane, uint8_t* host_levels);
+void ms_teps_edges(g5_ctx* ctx, int64_t* out);
+void ms_model_bytes(g5_ctx* ctx, double* per_level, double* total);
 }  // namespace g5

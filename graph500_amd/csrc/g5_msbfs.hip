@@ -1,0 +1,537 @@
+// g5_msbfs.hip -- batched multi-source BFS: up to 64 roots share one sweep.
+//
+// Lane r of every 64-bit word belongs to root r.  Per vertex (degree-ordered
+// new ids, rows [0, n_rows)):
+//   seen[v]        lanes that have reached v
+//   fr[l][v]       lanes for which v is at level l (the frontier of level l+1);
+//                  kept for every level, so the per-root level arrays are
+//                  rebuilt after the run without any extra store in the sweep
+// Each level is a push or a pull step, chosen from the edge count of the
+// frontier union (the direction-optimizing rule applied to the union):
+//   push  the union's rows form a chunked virtual edge list (as in the
+//         single-root top-down); target w receives front[u] & ~seen[w] with
+//         one 64-bit atomicOr, filtered by plain loads; k_ms_finish then
+//         masks the new lanes and folds them into seen
+//   pull  one lane per vertex: the lanes it still lacks (and whose frontier
+//         is not empty) are collected by OR-ing the neighbours' frontier
+//         words in ascending (highest degree first) order until every needed
+//         lane is found or the row ends; long rows are scanned by the whole
+//         wave with a wave-wide OR.  A 1-bit-per-vertex "frontier word is not
+//         zero" bitmap (n/8 bytes, cache resident) is tested first, so the
+//         8-byte frontier word is only read for frontier vertices
+// Per-lane levels equal the single-root BFS levels of the same root.
+#include "g5_ctx.hpp"
+#include "g5_frontier.hpp"
+
+#include <algorithm>
+
+namespace g5 {
+
+constexpr uint32_t kMsNoNbr = 0xFFFFFFFFu;
+constexpr int kMsShortRow = 32;
+
+enum { M_LIVE = 0, M_PAIRS = 1, M_NZ = 2, M_FDEG = 3, M_NUM = 8 };
+
+__device__ __forceinline__ uint64_t wave_or(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide OR / sums of the level counters, one atomic each per workgroup.
+__device__ __forceinline__ void ms_block_counters(uint64_t live, uint64_t pairs, uint64_t nz, uint64_t fdeg,
+                                                  unsigned long long* c) {
+  __shared__ uint64_t s_v[4][kBlock / 64];
+  live = wave_or(live);
+  pairs = wave_sum(pairs);
+  nz = wave_sum(nz);
+  fdeg = wave_sum(fdeg);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_v[0][wave] = live;
+    s_v[1][wave] = pairs;
+    s_v[2][wave] = nz;
+    s_v[3][wave] = fdeg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t a = 0, b = 0, d = 0, e = 0;
+    for (int i = 0; i < kBlock / 64; ++i) {
+      a |= s_v[0][i];
+      b += s_v[1][i];
+      d += s_v[2][i];
+      e += s_v[3][i];
+    }
+    if (a) atomicOr(c + M_LIVE, (unsigned long long)a);
+    if (b) atomicAdd(c + M_PAIRS, (unsigned long long)b);
+    if (d) atomicAdd(c + M_NZ, (unsigned long long)d);
+    if (e) atomicAdd(c + M_FDEG, (unsigned long long)e);
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBlock) k_ms_clear(uint64_t* __restrict__ a, uint64_t* __restrict__ b,
+                                                     int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    a[i] = 0;
+    b[i] = 0;
+  }
+}
+
+// Marks every active root in seen and fr0 and counts the level-0 union.
+__global__ void k_ms_roots(const uint32_t* __restrict__ old2new, const int64_t* __restrict__ roots, int nroots,
+                           int64_t rows, const uint64_t* __restrict__ offsets, uint64_t* __restrict__ seen,
+                           uint64_t* __restrict__ fr0, unsigned long long* __restrict__ c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint64_t live = 0, nz = 0, fdeg = 0;
+  for (int r = 0; r < nroots; ++r) {
+    const uint32_t v = old2new[roots[r]];
+    if ((int64_t)v >= rows) continue;  // a root without edges discovers nothing
+    const uint64_t bit = UINT64_C(1) << r;
+    if (!fr0[v]) {
+      ++nz;
+      fdeg += offsets[v + 1] - offsets[v];
+    }
+    seen[v] |= bit;
+    fr0[v] |= bit;
+    live |= bit;
+  }
+  c[M_LIVE] = live;
+  c[M_PAIRS] = 0;
+  c[M_NZ] = nz;
+  c[M_FDEG] = fdeg;
+}
+
+// Level-0 nonzero bitmap: the root vertices.
+__global__ void k_ms_roots_nz(const uint32_t* __restrict__ old2new, const int64_t* __restrict__ roots, int nroots,
+                              int64_t rows, uint64_t* __restrict__ nz0) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int r = 0; r < nroots; ++r) {
+    const uint32_t v = old2new[roots[r]];
+    if ((int64_t)v < rows) nz0[v >> 6] |= UINT64_C(1) << (v & 63);
+  }
+}
+
+// Vertices of the frontier union (set bits of the nonzero bitmap) -> queue,
+// ascending, through a scan of per-word popcounts.
+struct UnionCount {
+  const uint64_t* nz;
+  __device__ uint64_t operator()(size_t w) const { return (uint64_t)__popcll(nz[w]); }
+};
+struct UnionEmit {
+  const uint64_t* nz;
+  uint32_t* queue;
+  __device__ void operator()(size_t w, uint64_t pre) const {
+    uint64_t x = nz[w];
+    while (x) {
+      queue[pre++] = (uint32_t)(w * 64 + __builtin_ctzll(x));
+      x &= x - 1;
+    }
+  }
+};
+
+__global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__ queue, int64_t nf,
+                                                    const uint64_t* __restrict__ prefix,
+                                                    const uint32_t* __restrict__ chunk_rows,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ cols,
+                                                    const uint64_t* __restrict__ front,
+                                                    const uint64_t* __restrict__ seen, uint64_t* next) {
+  __shared__ ChunkRows cr;
+  const uint64_t E = prefix[nf];
+  for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
+    const uint64_t e0 = chunk * kEpb;
+    const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
+    const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
+    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
+      const uint64_t m = front[u] & ~seen[w];
+      if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
+        atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
+    });
+    __syncthreads();
+  }
+}
+
+// After a push: next keeps only lanes new to v, seen gains them, and
+// nz_next gets one bit per vertex whose frontier word is not zero.
+__global__ void __launch_bounds__(kBlock) k_ms_finish(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
+                                                      uint64_t* __restrict__ nz_next,
+                                                      const uint64_t* __restrict__ offsets, int64_t rows,
+                                                      unsigned long long* __restrict__ c) {
+  uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
+  const int64_t words = (rows + 63) / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t wd = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
+    const int64_t v = wd * 64 + lane;
+    uint64_t nw = 0;
+    if (v < rows) {
+      const uint64_t s = seen[v];
+      nw = next[v] & ~s;
+      next[v] = nw;
+      if (nw) {
+        seen[v] = s | nw;
+        live |= nw;
+        pairs += __popcll(nw);
+        ++nz;
+        fdeg += offsets[v + 1] - offsets[v];
+      }
+    }
+    const uint64_t b = __ballot(nw != 0);
+    if (lane == 0) nz_next[wd] = b;
+  }
+  ms_block_counters(live, pairs, nz, fdeg, c);
+}
+
+// Pull step: one lane per vertex, the wave helps with long rows.
+__global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
+                                                    const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
+                                                    uint64_t* __restrict__ nz_next,
+                                                    const uint32_t* __restrict__ first_nbr,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ cols, int64_t rows,
+                                                    uint64_t live_in, unsigned long long* __restrict__ c) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
+  const int64_t words = (rows + 63) / 64;
+  for (int64_t wd = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
+    const int64_t v = wd * 64 + lane;
+    const bool inb = v < rows;
+    const uint64_t s = inb ? seen[v] : ~UINT64_C(0);
+    const uint64_t need = ~s & live_in;
+    uint64_t acc = 0;
+    bool more = false;
+    uint64_t rb = 0, re = 0;
+    if (need) {
+      const uint32_t f = first_nbr[v];
+      if (f != kMsNoNbr) {
+        if (bit_set(nz_front, f)) acc = front[f];
+        if ((acc & need) != need) {
+          rb = offsets[v] + 1;
+          re = offsets[v + 1];
+          more = rb < re;
+        }
+      }
+    }
+    if (more && re - rb <= (uint64_t)kMsShortRow) {
+      for (uint64_t k = rb; k < re; ++k) {
+        const uint32_t w = cols[k];
+        if (!bit_set(nz_front, w)) continue;
+        acc |= front[w];
+        if ((acc & need) == need) break;
+      }
+      more = false;
+    }
+    uint64_t lm = __ballot(more);
+    while (lm) {
+      const int src = __ffsll((unsigned long long)lm) - 1;
+      lm &= lm - 1;
+      const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
+      const uint64_t nd = __shfl(need, src, 64);
+      uint64_t a = __shfl(acc, src, 64);
+      for (uint64_t base = b; base < e; base += 64) {
+        const uint64_t k = base + lane;
+        uint64_t x = 0;
+        if (k < e) {
+          const uint32_t w = cols[k];
+          if (bit_set(nz_front, w)) x = front[w];
+        }
+        a |= wave_or(x);
+        if ((a & nd) == nd) break;
+      }
+      if (lane == src) acc = a;
+    }
+    const uint64_t nw = acc & need;
+    const uint64_t b = __ballot(nw != 0);
+    if (lane == 0) nz_next[wd] = b;
+    if (inb) {
+      next[v] = nw;
+      if (nw) {
+        seen[v] = s | nw;
+        live |= nw;
+        pairs += __popcll(nw);
+        ++nz;
+        fdeg += offsets[v + 1] - offsets[v];
+      }
+    }
+  }
+  ms_block_counters(live, pairs, nz, fdeg, c);
+}
+
+// Per-root level of every original vertex: the first level whose frontier
+// word holds the lane.
+__global__ void k_ms_levels(const uint32_t* __restrict__ old2new, int64_t n, int64_t rows,
+                            const uint64_t* const* __restrict__ fr, int nlevels, int lane, int64_t root,
+                            uint8_t* __restrict__ out) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t nv = old2new[v];
+    uint8_t l = 0xFF;
+    if (v == root)
+      l = 0;  // also a root without edges, which is no row of the CSR
+    else if ((int64_t)nv < rows)
+      for (int i = 0; i < nlevels; ++i)
+        if ((fr[i][nv] >> lane) & 1) {
+          l = (uint8_t)i;
+          break;
+        }
+    out[v] = l;
+  }
+}
+
+// Traversed-edge numerators of all lanes: sum of weight over the vertices
+// each lane reached.  Vertices reached by every lane go to one accumulator.
+__global__ void __launch_bounds__(kBlock) k_ms_teps(const uint64_t* __restrict__ seen,
+                                                    const uint32_t* __restrict__ weight, int64_t rows,
+                                                    uint64_t all, unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long s_acc[65];
+  if (threadIdx.x < 65) s_acc[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t full = 0;
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < rows; v += (int64_t)gridDim.x * kBlock) {
+    const uint64_t s = seen[v];
+    const uint64_t w = weight[v];
+    if (s == all) {
+      full += w;
+    } else {
+      uint64_t x = s;
+      while (x) {
+        atomicAdd(&s_acc[__builtin_ctzll(x)], (unsigned long long)w);
+        x &= x - 1;
+      }
+    }
+  }
+  full = wave_sum(full);
+  if ((threadIdx.x & 63) == 0 && full) atomicAdd(&s_acc[64], (unsigned long long)full);
+  __syncthreads();
+  if (threadIdx.x < 65 && s_acc[threadIdx.x]) atomicAdd(acc + threadIdx.x, s_acc[threadIdx.x]);
+}
+
+struct PopCount {
+  const uint64_t* p;
+  __device__ uint64_t operator()(size_t i) const { return (uint64_t)__popcll(p[i]); }
+};
+
+// --- batched byte model (evaluated after the run from the per-level words) ---
+// pull level l: 24 n + sum over vertices with a needed lane of (8 + 12 k),
+// k = probes until every needed lane is found (the row length if not).
+__global__ void __launch_bounds__(kBlock) k_ms_model(const uint64_t* const* __restrict__ fr, int l,
+                                                     uint64_t lanes, const uint64_t* __restrict__ offsets,
+                                                     const uint32_t* __restrict__ cols, int64_t rows,
+                                                     unsigned long long* __restrict__ acc) {
+  uint64_t b = 0;
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < rows; v += (int64_t)gridDim.x * kBlock) {
+    uint64_t seen_before = 0;
+    for (int i = 0; i < l; ++i) seen_before |= fr[i][v];
+    const uint64_t need = ~seen_before & lanes;
+    const uint64_t s = offsets[v], e = offsets[v + 1];
+    if (!need || s == e) continue;
+    uint64_t a = 0, k = 0;
+    for (uint64_t j = s; j < e; ++j) {
+      ++k;
+      a |= fr[l - 1][cols[j]];
+      if ((a & need) == need) break;
+    }
+    b += 8 + 12 * k;
+  }
+  block_add(b, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Host
+// ---------------------------------------------------------------------------
+void ms_prepare(g5_ctx* ctx) {
+  Graph& g = ctx->graph;
+  G5_CHECK(g.built, "g5_batch_prepare: no graph (call g5_construct first)");
+  MsState& m = ctx->ms;
+  m.release();
+  const int64_t rows = g.n_rows;
+  m.seen.alloc(rows);
+  for (auto& b : m.nz) b.alloc((rows + 63) / 64);
+  m.queue.alloc(rows);
+  m.prefix.alloc(rows + 1);
+  m.chunk_rows.alloc(2 * ((uint64_t)g.nnz / kEpb + 2));
+  m.scan_tmp.alloc(scan_tmp_elems(rows + 1));
+  m.counters.alloc(M_NUM);
+  m.roots.alloc(64);
+  m.frp.alloc(G5_MAX_LEVELS);
+  G5_HIP(hipHostMalloc((void**)&m.h_counters, M_NUM * sizeof(uint64_t), hipHostMallocDefault));
+  G5_HIP(hipEventCreate(&m.ev0));
+  G5_HIP(hipEventCreate(&m.ev1));
+  m.lev_begin.assign(G5_MAX_LEVELS, nullptr);
+  m.lev_end.assign(G5_MAX_LEVELS, nullptr);
+  for (int i = 0; i < G5_MAX_LEVELS; ++i) {
+    G5_HIP(hipEventCreate(&m.lev_begin[i]));
+    G5_HIP(hipEventCreate(&m.lev_end[i]));
+  }
+  m.prepared = true;
+}
+
+// Frontier words of level l, allocated on first use and kept for reuse.
+static uint64_t* ms_level(MsState& m, int l, int64_t rows) {
+  while ((int)m.fr.size() <= l) m.fr.emplace_back(new DevBuf<uint64_t>());
+  m.fr[l]->alloc(rows);
+  return m.fr[l]->p;
+}
+
+void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots) {
+  Graph& g = ctx->graph;
+  MsState& m = ctx->ms;
+  G5_CHECK(m.prepared, "g5_run_bfs_batch: call g5_batch_prepare first");
+  G5_CHECK(nroots >= 1 && nroots <= 64, "g5_run_bfs_batch: 1 to 64 roots per batch");
+  for (int r = 0; r < nroots; ++r)
+    G5_CHECK(roots[r] >= 0 && roots[r] < g.n, "g5_run_bfs_batch: root out of range");
+  hipStream_t s = ctx->stream;
+  const int64_t rows = g.n_rows;
+  unsigned long long* cnt = (unsigned long long*)m.counters.p;
+  m.have_result = false;
+  m.nroots = nroots;
+  m.host_roots.assign(roots, roots + nroots);
+  g5_batch_stats& st = m.stats;
+  st = g5_batch_stats{};
+  st.num_roots = nroots;
+  G5_HIP(hipMemcpyAsync(m.roots.p, roots, nroots * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  uint64_t* fr0 = ms_level(m, 0, rows);
+
+  const int64_t words = (rows + 63) / 64;
+  G5_HIP(hipEventRecord(m.ev0, s));
+  hipLaunchKernelGGL(k_ms_clear, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, m.seen.p, fr0, rows);
+  G5_HIP(hipMemsetAsync(m.nz[0].p, 0, words * 8, s));
+  hipLaunchKernelGGL(k_ms_roots, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, nroots, rows, g.offsets.p,
+                     m.seen.p, fr0, cnt);
+  hipLaunchKernelGGL(k_ms_roots_nz, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, nroots, rows, m.nz[0].p);
+  G5_HIP(hipGetLastError());
+  G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, M_NUM * 8, hipMemcpyDeviceToHost, s));
+  G5_HIP(hipStreamSynchronize(s));
+  uint64_t live = m.h_counters[M_LIVE];
+  uint64_t nz = m.h_counters[M_NZ], fdeg = m.h_counters[M_FDEG];
+  const double push_limit = (double)g.nnz / m.alpha;
+  int l = 0;
+  while (live) {
+    ++l;
+    G5_CHECK(l < G5_MAX_LEVELS - 1, "g5_run_bfs_batch: more than 254 levels (uint8 level arrays)");
+    const uint64_t* front = m.fr[l - 1]->p;
+    uint64_t* next = ms_level(m, l, rows);
+    front = m.fr[l - 1]->p;  // the vector may have grown
+    const uint64_t* nz_front = m.nz[(l - 1) & 1].p;
+    uint64_t* nz_next = m.nz[l & 1].p;
+    const bool push = (double)fdeg < push_limit;
+    G5_HIP(hipMemsetAsync(m.counters.p, 0, M_NUM * 8, s));
+    G5_HIP(hipEventRecord(m.lev_begin[l], s));
+    if (push) {
+      device_scan(UnionCount{nz_front}, UnionEmit{nz_front, m.queue.p}, (size_t)words, m.scan_tmp.p, s);
+      const int64_t nf = (int64_t)nz;
+      device_scan(FrontierDegree{m.queue.p, g.offsets.p, nf}, StoreU64{m.prefix.p}, (size_t)nf + 1,
+                  m.scan_tmp.p, s);
+      const uint64_t nchunks = (fdeg + kEpb - 1) / kEpb;
+      G5_CHECK(2 * nchunks <= m.chunk_rows.n, "g5_run_bfs_batch: chunk table too small");
+      if (nchunks)
+        hipLaunchKernelGGL(k_chunk_rows<kEpb>, dim3(grid_of((int64_t)(2 * nchunks), kBlock, 65536)),
+                           dim3(kBlock), 0, s, m.prefix.p, nf, nchunks, m.chunk_rows.p);
+      G5_HIP(hipMemsetAsync(next, 0, rows * 8, s));
+      hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)fdeg, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p,
+                         nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, m.seen.p, next);
+      G5_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_ms_finish, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
+                         next, nz_next, g.offsets.p, rows, cnt);
+    } else {
+      hipLaunchKernelGGL(k_ms_pull, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
+                         m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows,
+                         live, cnt);
+    }
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipEventRecord(m.lev_end[l], s));
+    G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, M_NUM * 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    float lms = 0;
+    G5_HIP(hipEventElapsedTime(&lms, m.lev_begin[l], m.lev_end[l]));
+    st.level_ms[l] = lms;
+    st.direction[l] = push ? 1 : 0;
+    st.pairs[l] = (int64_t)m.h_counters[M_PAIRS];
+    st.num_levels = l + 1;
+    live = m.h_counters[M_LIVE];
+    nz = m.h_counters[M_NZ];
+    fdeg = m.h_counters[M_FDEG];
+  }
+  if (st.num_levels == 0) st.num_levels = 1;
+  G5_HIP(hipEventRecord(m.ev1, s));
+  G5_HIP(hipEventSynchronize(m.ev1));
+  float ms = 0;
+  G5_HIP(hipEventElapsedTime(&ms, m.ev0, m.ev1));
+  st.device_ms = ms;
+  m.nlevels = st.num_levels - 1 > 0 ? st.num_levels - 1 : 1;  // levels with frontier words 0..L-1
+  m.have_result = true;
+}
+
+static void ms_upload_level_ptrs(MsState& m, int nlev, hipStream_t s) {
+  std::vector<const uint64_t*> h(nlev);
+  for (int i = 0; i < nlev; ++i) h[i] = m.fr[i]->p;
+  G5_HIP(hipMemcpyAsync(m.frp.p, h.data(), nlev * sizeof(void*), hipMemcpyHostToDevice, s));
+}
+
+void ms_get_levels(g5_ctx* ctx, int lane, uint8_t* host_levels) {
+  Graph& g = ctx->graph;
+  MsState& m = ctx->ms;
+  G5_CHECK(m.have_result, "g5_batch_get_levels: no batch result");
+  G5_CHECK(lane >= 0 && lane < m.nroots, "g5_batch_get_levels: lane out of range");
+  const int nlev = m.stats.num_levels;  // frontier words exist for levels 0..num_levels-1
+  ms_upload_level_ptrs(m, nlev, ctx->stream);
+  ctx->bfs.level_out.alloc(g.n);
+  hipLaunchKernelGGL(k_ms_levels, dim3(grid_of(g.n, kBlock, 65536)), dim3(kBlock), 0, ctx->stream, g.old2new.p,
+                     g.n, g.n_rows, (const uint64_t* const*)m.frp.p, nlev, lane, m.host_roots[lane],
+                     ctx->bfs.level_out.p);
+  G5_HIP(hipGetLastError());
+  G5_HIP(hipMemcpyAsync(host_levels, ctx->bfs.level_out.p, g.n, hipMemcpyDeviceToHost, ctx->stream));
+  G5_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void ms_teps_edges(g5_ctx* ctx, int64_t* out) {
+  Graph& g = ctx->graph;
+  MsState& m = ctx->ms;
+  G5_CHECK(m.have_result, "g5_batch_teps_edges: no batch result");
+  ctx->acc.alloc(65);
+  hipStream_t s = ctx->stream;
+  G5_HIP(hipMemsetAsync(ctx->acc.p, 0, 65 * 8, s));
+  const uint64_t all = m.nroots == 64 ? ~UINT64_C(0) : ((UINT64_C(1) << m.nroots) - 1);
+  hipLaunchKernelGGL(k_ms_teps, dim3(grid_of(g.n_rows, kBlock, 4096)), dim3(kBlock), 0, s, m.seen.p,
+                     g.weight.p, g.n_rows, all, (unsigned long long*)ctx->acc.p);
+  G5_HIP(hipGetLastError());
+  uint64_t h[65];
+  G5_HIP(hipMemcpyAsync(h, ctx->acc.p, 65 * 8, hipMemcpyDeviceToHost, s));
+  G5_HIP(hipStreamSynchronize(s));
+  for (int r = 0; r < m.nroots; ++r) out[r] = (int64_t)((h[r] + h[64]) / 2);
+}
+
+void ms_model_bytes(g5_ctx* ctx, double* per_level, double* total) {
+  Graph& g = ctx->graph;
+  MsState& m = ctx->ms;
+  G5_CHECK(m.have_result, "g5_batch_model_bytes: no batch result");
+  hipStream_t s = ctx->stream;
+  const int nlev = m.stats.num_levels;
+  ms_upload_level_ptrs(m, nlev, s);
+  ctx->acc.alloc(65);
+  const uint64_t lanes = m.nroots == 64 ? ~UINT64_C(0) : ((UINT64_C(1) << m.nroots) - 1);
+  // 1 B per (root, reached vertex)
+  double tot = 0;
+  for (int l = 0; l < nlev; ++l) tot += (double)(l == 0 ? 0 : m.stats.pairs[l]);
+  // the roots placed at level 0
+  tot += (double)device_reduce_sum(PopCount{m.fr[0]->p}, (size_t)g.n_rows, ctx->acc.p, s);
+  for (int l = 0; l < G5_MAX_LEVELS; ++l)
+    if (per_level) per_level[l] = 0;
+  for (int l = 1; l < nlev; ++l) {
+    G5_HIP(hipMemsetAsync(ctx->acc.p, 0, 8, s));
+    hipLaunchKernelGGL(k_ms_model, dim3(grid_of(g.n_rows, kBlock, 8192)), dim3(kBlock), 0, s,
+                       (const uint64_t* const*)m.frp.p, l, lanes, g.offsets.p, g.cols.p, g.n_rows,
+                       (unsigned long long*)ctx->acc.p);
+    G5_HIP(hipGetLastError());
+    uint64_t h = 0;
+    G5_HIP(hipMemcpyAsync(&h, ctx->acc.p, 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    const double b = 24.0 * (double)g.n + (double)h;
+    if (per_level) per_level[l] = b;
+    tot += b;
+  }
+  if (total) *total = tot;
+}
+
+}  // namespace g5

@@ -36,7 +36,10 @@ EXPORTS = [
     "g5_edges_download", "g5_edges_release", "g5_generate_kronecker", "g5_construct",
     "g5_graph_info_get", "g5_has_edge", "g5_pred_size", "g5_find_roots", "g5_graph_download",
     "g5_prepare_bfs", "g5_run_bfs", "g5_get_pred", "g5_get_levels", "g5_teps_edges",
-    "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_end_bfs",
+    "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_validate", "g5_validate_pred", "g5_end_bfs",
+    "g5_batch_prepare", "g5_batch_set_tuning", "g5_batch_get_tuning", "g5_run_bfs_batch",
+    "g5_batch_get_levels", "g5_batch_teps_edges", "g5_batch_stats_get", "g5_batch_model_bytes",
+    "g5_batch_end",
 ]
 
 
@@ -77,6 +80,21 @@ class BfsStats(ctypes.Structure):
     ]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("num_roots", ctypes.c_int32),
+        ("num_levels", ctypes.c_int32),
+        ("direction", ctypes.c_int32 * MAX_LEVELS),
+        ("pairs", ctypes.c_int64 * MAX_LEVELS),
+        ("level_ms", ctypes.c_double * MAX_LEVELS),
+        ("device_ms", ctypes.c_double),
+    ]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_double), ("pull_grid", ctypes.c_int64)]
+
+
 _lib = None
 
 
@@ -115,7 +133,18 @@ def load_library(path=LIB_PATH):
         "g5_teps_edges": ([vp, ctypes.POINTER(i64)], i32),
         "g5_bfs_stats_get": ([vp, ctypes.POINTER(BfsStats)], i32),
         "g5_bfs_model_bytes": ([vp, vp, ctypes.POINTER(ctypes.c_double)], i32),
+        "g5_validate": ([vp, vp, ctypes.POINTER(i32)], i32),
+        "g5_validate_pred": ([vp, i64, vp, vp, ctypes.POINTER(i32), ctypes.POINTER(i64)], i32),
         "g5_end_bfs": ([vp], i32),
+        "g5_batch_prepare": ([vp], i32),
+        "g5_batch_set_tuning": ([vp, ctypes.POINTER(BatchTuning)], i32),
+        "g5_batch_get_tuning": ([vp, ctypes.POINTER(BatchTuning)], i32),
+        "g5_run_bfs_batch": ([vp, vp, i32], i32),
+        "g5_batch_get_levels": ([vp, i32, vp], i32),
+        "g5_batch_teps_edges": ([vp, vp], i32),
+        "g5_batch_stats_get": ([vp, ctypes.POINTER(BatchStats)], i32),
+        "g5_batch_model_bytes": ([vp, vp, ctypes.POINTER(ctypes.c_double)], i32),
+        "g5_batch_end": ([vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -268,8 +297,88 @@ class G5Context:
         L = self.bfs_stats()["num_levels"]
         return per[1:L].tolist(), tot.value
 
+    VALIDATION_RULES = ["root", "self_parent", "depth", "edge", "parent_edge", "range"]
+
+    def validate(self):
+        """(valid, {rule: violations}) of the last run, checked on the GPU."""
+        err = np.zeros(6, np.int64)
+        ok = ctypes.c_int(0)
+        self._check(self._lib.g5_validate(self._h, _ptr(err), ctypes.byref(ok)))
+        return bool(ok.value), dict(zip(self.VALIDATION_RULES, err.tolist()))
+
+    def validate_pred(self, root, pred):
+        """(valid, {rule: violations}, traversed edges) of a pred array of
+        pred_size() int64 entries (parent | level << 48, -1 unreached),
+        checked on the GPU against the resident graph."""
+        pred = np.ascontiguousarray(pred, dtype=np.int64)
+        if pred.shape != (self.pred_size(),):
+            raise ValueError("pred must hold pred_size() = %d entries" % self.pred_size())
+        err = np.zeros(6, np.int64)
+        ok = ctypes.c_int(0)
+        visited = ctypes.c_int64(0)
+        self._check(self._lib.g5_validate_pred(self._h, int(root), _ptr(pred), _ptr(err), ctypes.byref(ok),
+                                               ctypes.byref(visited)))
+        return bool(ok.value), dict(zip(self.VALIDATION_RULES, err.tolist())), int(visited.value)
+
     def end_bfs(self):
         self._check(self._lib.g5_end_bfs(self._h))
+
+    # -- batched BFS (up to 64 roots per sweep) ---------------------------------
+    def batch_prepare(self):
+        self._check(self._lib.g5_batch_prepare(self._h))
+
+    def batch_set_tuning(self, alpha=None, pull_grid=None):
+        t = BatchTuning()
+        self._check(self._lib.g5_batch_get_tuning(self._h, ctypes.byref(t)))
+        if alpha is not None:
+            t.alpha = alpha
+        if pull_grid is not None:
+            t.pull_grid = pull_grid
+        self._check(self._lib.g5_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def batch_tuning(self):
+        t = BatchTuning()
+        self._check(self._lib.g5_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {"alpha": t.alpha, "pull_grid": t.pull_grid}
+
+    def run_bfs_batch(self, roots):
+        r = np.ascontiguousarray(roots, np.int64)
+        self._check(self._lib.g5_run_bfs_batch(self._h, _ptr(r), len(r)))
+
+    def batch_levels(self, lane, out=None):
+        if out is None:
+            out = np.zeros(self.pred_size(), np.uint8)
+        self._check(self._lib.g5_batch_get_levels(self._h, lane, _ptr(out)))
+        return out
+
+    def batch_teps_edges(self):
+        st = self.batch_stats()
+        out = np.zeros(st["num_roots"], np.int64)
+        self._check(self._lib.g5_batch_teps_edges(self._h, _ptr(out)))
+        return out
+
+    def batch_stats(self):
+        st = BatchStats()
+        self._check(self._lib.g5_batch_stats_get(self._h, ctypes.byref(st)))
+        L = st.num_levels
+        return {
+            "num_roots": st.num_roots,
+            "num_levels": L,
+            "direction": [st.direction[i] for i in range(1, L)],
+            "pairs": [st.pairs[i] for i in range(1, L)],
+            "level_ms": [st.level_ms[i] for i in range(1, L)],
+            "device_ms": st.device_ms,
+        }
+
+    def batch_model_bytes(self):
+        per = np.zeros(MAX_LEVELS, np.float64)
+        tot = ctypes.c_double(0)
+        self._check(self._lib.g5_batch_model_bytes(self._h, _ptr(per), ctypes.byref(tot)))
+        L = self.batch_stats()["num_levels"]
+        return per[1:L].tolist(), tot.value
+
+    def batch_end(self):
+        self._check(self._lib.g5_batch_end(self._h))
 
     def synchronize(self):
         self._check(self._lib.g5_synchronize(self._h))

@@ -68,6 +68,22 @@ typedef struct g5_bfs_stats {
   int64_t reached;                  /* vertices with a level, root included */
 } g5_bfs_stats;
 
+/* Trace of the last g5_run_bfs_batch.  Index 0 is unused; levels start at 1. */
+typedef struct g5_batch_stats {
+  int32_t num_roots;
+  int32_t num_levels;               /* levels executed + 1 (last level finds nothing) */
+  int32_t direction[G5_MAX_LEVELS]; /* 1 = push (top-down), 0 = pull (bottom-up) */
+  int64_t pairs[G5_MAX_LEVELS];     /* (root, vertex) pairs discovered at that level */
+  double level_ms[G5_MAX_LEVELS];   /* device time per level (HIP events) */
+  double device_ms;                 /* device time of the whole batch */
+} g5_batch_stats;
+
+/* Tuning of the batched BFS (set by the autotuner, graph500_amd/autotune.py). */
+typedef struct g5_batch_tuning {
+  double alpha;       /* push while the frontier union has fewer than nnz/alpha edges */
+  int64_t pull_grid;  /* workgroups of the pull sweep (256 threads each) */
+} g5_batch_tuning;
+
 /* ---- context ---- */
 const char* g5_version(void);
 int g5_device_count(int* count);
@@ -147,8 +163,41 @@ int g5_bfs_stats_get(g5_ctx* ctx, g5_bfs_stats* out);
  * from the final levels, outside any timed region). per_level has
  * G5_MAX_LEVELS entries, index = level. */
 int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
+/* Validates the last run on the GPU with the rules of BfsValidation::validate
+ * (mpi/validate.hpp:506-802), checking every edge of the CSR (all non-loop
+ * input tuples, both directions).  errors[6] receives the number of
+ * violations of: root is its own parent at level 0, no other self parent,
+ * level = parent level + 1, edge endpoints both (un)reached with levels at
+ * most one apart, parent is a neighbour, value in range.  *valid is 1 when
+ * all are zero. */
+int g5_validate(g5_ctx* ctx, int64_t* errors, int* valid);
+/* The same checks on a caller's pred array of pred_size() entries
+ * (validate_bfs_result, mpi/validate.hpp:1008-1019), with the traversed-edge
+ * count of its reached set in *edge_visit_count (may be NULL). */
+int g5_validate_pred(g5_ctx* ctx, int64_t root, const int64_t* pred, int64_t* errors, int* valid,
+                     int64_t* edge_visit_count);
 /* end_bfs (mpi/bfs.hpp:126-128): frees BFS state. */
 int g5_end_bfs(g5_ctx* ctx);
+
+/* ---- batched BFS: up to 64 roots in one bit-packed sweep ----
+ * No counterpart in the reference, which runs its roots one after another
+ * (mpi/main.cc:105-150): this is the batch form of BfsBase::run_bfs whose
+ * per-root levels equal g5_run_bfs's for the same root. */
+int g5_batch_prepare(g5_ctx* ctx);
+int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* tuning);
+int g5_batch_get_tuning(g5_ctx* ctx, g5_batch_tuning* tuning);
+/* Runs 1..64 roots (original ids) at once; synchronous. */
+int g5_run_bfs_batch(g5_ctx* ctx, const int64_t* roots, int num_roots);
+/* uint8 level per original vertex id for root 'lane' of the last batch. */
+int g5_batch_get_levels(g5_ctx* ctx, int lane, uint8_t* levels);
+/* Traversed-edge count of every root of the last batch (num_roots values). */
+int g5_batch_teps_edges(g5_ctx* ctx, int64_t* counts);
+int g5_batch_stats_get(g5_ctx* ctx, g5_batch_stats* out);
+/* Algorithmic bytes of the last batch under the bit-packed pull model: per
+ * level 24 n + sum over vertices lacking some root of (8 + 12 k), k = probes
+ * until every lacking root is found; plus 1 byte per (root, reached vertex). */
+int g5_batch_model_bytes(g5_ctx* ctx, double* per_level, double* total);
+int g5_batch_end(g5_ctx* ctx);
 
 #ifdef __cplusplus
 }
