@@ -1,23 +1,33 @@
 #!/usr/bin/env python3
-"""Graph500 BFS hot path on one MI355X: harmonic-mean GTEPS at Kronecker
-SCALE 28, edgefactor 16 (seeds 2,3; initiator 0.57/0.19/0.19/0.05).
+"""Graph500 BFS hot path on one MI355X at Kronecker SCALE 28, edgefactor 16
+(seeds 2,3; initiator 0.57/0.19/0.19/0.05): 64-root batched BFS against
+single-root BFS, GTEPS.
 
-    python bench.py [--gpus 1] [--steps 64] [--warmup 2] [--scale 28]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--scale 28]
 
-A step is one single-root direction-optimizing BFS (g5_run_bfs) over the
-graph resident in HBM; steps cycle over the 64 roots chosen by find_roots.
-Generation and CSR construction run on the GPU before the timed region, as in
-the reference driver (mpi/main.cc:50-117, only run_bfs is timed).  Each root
-is timed with the host clock around the synchronous call, like MPI_Wtime
+A step is one 64-root batched BFS (g5_run_bfs_batch) over the graph resident
+in HBM, for the 64 roots find_roots draws (mpi/benchmark_helper.hpp:475-508):
+one edge sweep per level serves all 64 searches, each root's levels bit-exact
+with its single-root run.  `value` = sum over roots of traversed edges / batch
+time (host clock around the synchronous call), the batched analogue of the
+harmonic mean (mpi/logfile.h:74-82).  Generation and CSR construction run on
+the GPU before the timed region, as in the reference driver (mpi/main.cc:
+50-117, only run_bfs is timed).
+
+`single_root` times the drop-in path, g5_run_bfs, once per root (after
+warm-up), with the host clock around each synchronous call like MPI_Wtime
 around run_bfs; GTEPS = 1 / mean(t_i / e_i) with e_i the validator's
-traversed-edge count (mpi/logfile.h:74-82).
+traversed-edge count.
 
-Prints ONE JSON line.  `roofline` is for the dominant kernel (the bottom-up
-sweep k_bu): algorithmic bytes of its launches (byte model evaluated on the
-GPU from the final levels, outside timing; it agrees with the CPU oracle's
-count, see tests) over their HIP-event durations on the library stream.
-`cpu_baseline` times the reference CPU runnable (oracle/_ref/runnable) on a
-bounded sample on this host, or the oracle port when that cannot run.
+`roofline` is for the dominant kernel of the step (the batched pull sweep
+k_ms_pull): algorithmic bytes of its launches (byte model evaluated on the
+GPU from the final levels, outside timing; equal to the CPU oracle's count,
+see tests) over their HIP-event durations on the library stream.  `traffic`
+is the HBM bytes per launch measured by rocprof (FETCH_SIZE + WRITE_SIZE,
+scaled by the same counters on a plain copy) when a summary for this kernel
+and workload is present under profiles/, else null.  `cpu_baseline` times
+the reference CPU runnable (oracle/_ref/runnable) on a bounded sample on this
+host, or the oracle port when that cannot run.
 
 Other modes:  --copy-calibration  runs a plain 1 GiB device copy (for scaling
 rocprof FETCH_SIZE/WRITE_SIZE counters).
@@ -96,16 +106,75 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
 
 
+def traffic_from_profile(kernel, scale):
+    """HBM bytes per launch of `kernel` from the committed rocprof PMC summary
+    (profiles/<kernel>_s<scale>_traffic.txt, line 'traffic_bytes_per_launch: X')."""
+    path = os.path.join(HERE, "profiles", "%s_s%d_traffic.txt" % (kernel, scale))
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def single_root_leg(ctx, roots, warmup):
+    """Each root once through g5_run_bfs; host clock per call."""
+    ctx.prepare_bfs()
+    for i in range(warmup):
+        ctx.run_bfs(int(roots[i % len(roots)]))
+    ctx.synchronize()
+    wall, dev, edges, tot_bytes, bu_bytes, bu_ms, bu_launches = [], [], [], 0.0, 0.0, 0.0, 0
+    for i, r in enumerate(roots):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.run_bfs(int(r))
+        t1 = time.perf_counter()
+        st = ctx.bfs_stats()
+        e = ctx.teps_edges()
+        per, tot = ctx.model_bytes()
+        wall.append(t1 - t0)
+        dev.append(st["device_ms"] / 1e3)
+        edges.append(e)
+        tot_bytes += tot
+        for L, d in enumerate(st["direction"]):
+            if d == 0:
+                bu_bytes += per[L]
+                bu_ms += st["level_ms"][L]
+                bu_launches += 1
+        if i % 16 == 0:
+            log("single root %d: %.3f ms %.1f GTEPS %s" % (
+                r, 1e3 * (t1 - t0), e / (t1 - t0) / 1e9,
+                "".join("T" if d else "B" for d in st["direction"])))
+    ctx.end_bfs()
+    hmean = len(wall) / sum(w / e for w, e in zip(wall, edges)) / 1e9
+    hmean_dev = len(dev) / sum(w / e for w, e in zip(dev, edges)) / 1e9
+    bu_gbs = bu_bytes / (bu_ms / 1e3) / 1e9 if bu_ms > 0 else 0.0
+    return {
+        "gteps_hmean_wall": round(hmean, 3),
+        "gteps_hmean_device": round(hmean_dev, 3),
+        "roots": len(roots),
+        "mean_wall_ms": round(1e3 * sum(wall) / len(wall), 4),
+        "mean_device_ms": round(1e3 * sum(dev) / len(dev), 4),
+        "bytes_per_edge": round(tot_bytes / sum(edges), 4),
+        "roofline_frac_whole_bfs": round((tot_bytes / sum(dev)) / (HBM_PEAK_GBS * 1e9), 4),
+        "traversed_edges_per_root": int(edges[0]),
+        "k_bu": {"achieved_gbs": round(bu_gbs, 1), "frac": round(bu_gbs / HBM_PEAK_GBS, 4),
+                 "launches": bu_launches,
+                 "algorithmic_bytes_per_launch": round(bu_bytes / max(bu_launches, 1)),
+                 "avg_launch_ms": round(bu_ms / max(bu_launches, 1), 4)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=28)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-scale", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-single-root", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     a = ap.parse_args()
 
@@ -127,58 +196,62 @@ def main():
     log("graph: n_active %d nnz %d gen %.1f ms construct %.1f ms" %
         (info["num_active"], info["nnz"], info["generate_ms"], info["construct_ms"]))
     roots = ctx.find_roots(64)
-    ctx.prepare_bfs()
-    for i in range(a.warmup):
-        ctx.run_bfs(int(roots[i % len(roots)]))
-    ctx.synchronize()
 
-    wall, dev, edges, bu_bytes, bu_ms, tot_bytes = [], [], [], 0.0, 0.0, 0.0
-    per_root = []
+    single = None
+    if not a.no_single_root:
+        single = single_root_leg(ctx, roots, a.warmup)
+        log("single root: %.1f GTEPS hmean" % single["gteps_hmean_wall"])
+
+    ctx.batch_prepare()
+    for _ in range(a.warmup):
+        ctx.run_bfs_batch(roots)
+    ctx.synchronize()
+    edges = ctx.batch_teps_edges()
+    total_edges = int(edges.sum())
+    per, tot = ctx.batch_model_bytes()  # the schedule is the same every step
+    wall, dev, pull_ms, pull_launches = [], [], 0.0, 0
+    pull_bytes = 0.0
     t_all = time.perf_counter()
     for i in range(a.steps):
-        r = int(roots[i % len(roots)])
         ctx.synchronize()
         t0 = time.perf_counter()
-        ctx.run_bfs(r)
+        ctx.run_bfs_batch(roots)
         t1 = time.perf_counter()
-        st = ctx.bfs_stats()
-        e = ctx.teps_edges()
-        per, tot = ctx.model_bytes()
+        st = ctx.batch_stats()
         wall.append(t1 - t0)
         dev.append(st["device_ms"] / 1e3)
-        edges.append(e)
-        tot_bytes += tot
         for L, d in enumerate(st["direction"]):
             if d == 0:
-                bu_bytes += per[L]
-                bu_ms += st["level_ms"][L]
-        per_root.append({"root": r, "ms": 1e3 * (t1 - t0), "edges": e,
-                         "dirs": "".join("T" if d else "B" for d in st["direction"])})
-        if i % 8 == 0:
-            log("root %d: %.3f ms %.1f GTEPS %s" % (r, 1e3 * (t1 - t0), e / (t1 - t0) / 1e9,
-                                                    per_root[-1]["dirs"]))
+                pull_ms += st["level_ms"][L]
+                pull_bytes += per[L]
+                pull_launches += 1
+        log("batch step %d: %.3f ms %.1f GTEPS" % (i, 1e3 * (t1 - t0), total_edges / (t1 - t0) / 1e9))
     t_all = time.perf_counter() - t_all
-    hmean = len(wall) / sum(w / e for w, e in zip(wall, edges)) / 1e9
-    hmean_dev = len(dev) / sum(w / e for w, e in zip(dev, edges)) / 1e9
-    bu_gbs = bu_bytes / (bu_ms / 1e3) / 1e9 if bu_ms > 0 else 0.0
-    bu_launches = sum(p["dirs"].count("B") for p in per_root)
+    st = ctx.batch_stats()
+    assert (ctx.batch_teps_edges() == edges).all()
+    mean_wall = sum(wall) / len(wall)
+    gteps = total_edges / mean_wall / 1e9
+    pull_gbs = pull_bytes / (pull_ms / 1e3) / 1e9 if pull_ms > 0 else 0.0
+    per_launch = pull_bytes / max(pull_launches, 1)
+    traffic = traffic_from_profile("k_ms_pull", a.scale)
     out = {
         "metric": METRIC,
-        "value": round(hmean, 3),
+        "value": round(gteps, 3),
         "unit": "GTEPS",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(1e3 * sum(wall) / len(wall), 4),
+        "ms_per_step": round(1e3 * mean_wall, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
+        "dtype": "u64",
         "data": "synthetic: Kronecker graph from the Graph500 splittable-MRG generator "
                 "(seeds 2,3), generated and built on the GPU",
         "config": {
-            "workload": "single-root direction-optimizing BFS, Kronecker SCALE %d edgefactor %d, "
-                        "64 find_roots roots cycled, VERTEX_REORDERING=2" % (a.scale, a.edgefactor),
+            "workload": "64-root batched BFS (one bit per root in u64 per-vertex words), Kronecker "
+                        "SCALE %d edgefactor %d, the 64 find_roots roots, VERTEX_REORDERING=2"
+                        % (a.scale, a.edgefactor),
             "scale": a.scale,
             "edgefactor": a.edgefactor,
             "num_active": info["num_active"],
@@ -187,31 +260,34 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_bu (bottom-up sweep)",
-            "achieved": round(bu_gbs, 1),
+            "kernel": "k_ms_pull (batched bottom-up sweep)",
+            "achieved": round(pull_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(bu_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "launches": bu_launches,
-            "algorithmic_bytes_per_launch": round(bu_bytes / max(bu_launches, 1)),
-            "avg_launch_ms": round(bu_ms / max(bu_launches, 1), 4),
+            "frac": round(pull_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "launches": pull_launches,
+            "algorithmic_bytes_per_launch": round(per_launch),
+            "avg_launch_ms": round(pull_ms / max(pull_launches, 1), 4),
         },
-        "single_root": {
-            "gteps_hmean_wall": round(hmean, 3),
-            "gteps_hmean_device": round(hmean_dev, 3),
-            "bfs_bytes_per_edge": round(tot_bytes / sum(edges), 4),
-            "bfs_roofline_frac": round((tot_bytes / sum(dev)) / (HBM_PEAK_GBS * 1e9), 4),
-            "traversed_edges_per_root": edges[0],
-            "mean_wall_ms": round(1e3 * sum(wall) / len(wall), 4),
-            "mean_device_ms": round(1e3 * sum(dev) / len(dev), 4),
+        "batch": {
+            "gteps": round(gteps, 3),
+            "gteps_device": round(total_edges / (sum(dev) / len(dev)) / 1e9, 3),
+            "traversed_edges_sum": total_edges,
+            "bytes_per_edge": round(tot / total_edges, 4),
+            "roofline_frac_whole_batch": round((tot / (sum(dev) / len(dev))) / (HBM_PEAK_GBS * 1e9), 4),
+            "levels": "".join("P" if d else "U" for d in st["direction"]),
+            "level_ms": [round(x, 3) for x in st["level_ms"]],
+            "tuning": ctx.batch_tuning(),
             "timed_region_s": round(t_all, 3),
         },
+        "single_root": single,
+        "batch_vs_single_root": (round(gteps / single["gteps_hmean_wall"], 3) if single else None),
         "graph": {"generate_ms": round(info["generate_ms"], 2),
                   "construct_ms": round(info["construct_ms"], 2)},
-        "batch": None,
         "serve": None,
     }
+    ctx.batch_end()
     ctx.close()
     if not a.no_cpu_baseline:
         log("cpu baseline")

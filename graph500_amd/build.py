@@ -1,9 +1,10 @@
-"""Builds graph500_amd/lib/libg5.so (HIP, gfx950) from graph500_amd/csrc.
+"""Builds graph500_amd/lib/libg5.so (HIP, gfx950) from graph500_amd/csrc and
+the C++ benchmark driver graph500_amd/lib/g5bench over it.
 
     python -m graph500_amd.build          # or build() from Python
 
 Every .hip source is compiled for gfx950 only, objects go to
-graph500_amd/_build/, the shared library to graph500_amd/lib/.
+graph500_amd/_build/, the shared library and the driver to graph500_amd/lib/.
 """
 import concurrent.futures
 import os
@@ -15,7 +16,10 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libg5.so")
+DRIVER = os.path.join(LIBDIR, "g5bench")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 
 SOURCES = ["g5_gen.hip", "g5_prims.hip", "g5_csr.hip", "g5_bfs.hip", "g5_msbfs.hip", "g5_validate.hip",
            "g5_api.hip"]
@@ -31,7 +35,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # host double arithmetic of find_roots is evaluated as written (no FMA)
     "-ffp-contract=off",
-    "-I" + os.path.join(os.path.dirname(HERE), "include"),
+    "-I" + INCLUDE,
 ]
 
 
@@ -58,7 +62,19 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, LIB)
+    _build_driver()
     return LIB
+
+
+def _build_driver():
+    """g5bench: host-only C++ over the C ABI (include/g5_bfs.hpp)."""
+    tmp = DRIVER + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, os.path.join(HERE, "driver", "g5bench.cc"),
+           "-o", tmp, "-L" + LIBDIR, "-lg5", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(tmp, DRIVER)
 
 
 if __name__ == "__main__":

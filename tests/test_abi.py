@@ -59,3 +59,50 @@ def test_context_requires_gpu():
     if n.value == 0:
         with pytest.raises(RuntimeError):
             graph500_amd.G5Context(0)
+
+
+def test_cpp_shim_compiles_against_reference_shaped_edge_list(tmp_path):
+    """include/g5_bfs.hpp compiles stand-alone with an edge list that has the
+    EdgeListStorage read interface (generator/graph_generator.hpp:98-180)."""
+    import subprocess
+    src = tmp_path / "shim.cc"
+    src.write_text(r'''
+#include "g5_bfs.hpp"
+struct Tuple { uint32_t a, b, c; };
+struct List {
+  typedef Tuple edge_type;
+  int beginRead(bool) { return 1; }
+  int read(Tuple** p) { static Tuple t[2]; *p = t; return 0; }
+  void endRead() {}
+};
+void f(g5::BfsOnGPU& b, List* l, int64_t* pred) {
+  b.construct(l);
+  b.prepare_bfs();
+  int64_t roots[4];
+  b.find_roots(roots, 4);
+  if (b.graph_.has_edge(roots[0])) b.run_bfs(roots[0], pred);
+  b.get_pred(pred);
+  int64_t n = 0;
+  (void)b.validate(roots[0], pred, &n);
+  (void)b.graph_.pred_size();
+  b.end_bfs();
+}
+''')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I",
+                        os.path.join(REPO, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_driver_reports_missing_gpu():
+    """g5bench fails with a message (exit 1), not a crash, without a device."""
+    import subprocess
+    from graph500_amd import build
+    if not os.path.exists(build.DRIVER):
+        build.build()
+    n = ctypes.c_int(0)
+    import graph500_amd
+    graph500_amd.load_library().g5_device_count(ctypes.byref(n))
+    if n.value:
+        pytest.skip("a GPU is present")
+    r = subprocess.run([build.DRIVER, "10"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "g5_create" in r.stderr

@@ -275,3 +275,19 @@ def test_model_bytes_match_oracle_s20(oracle_mod):
             c.run_bfs(int(r))
             per, tot = c.model_bytes()
             assert per == tr["bytes"] and tot == tr["total_bytes"]
+
+
+@pytest.mark.parametrize("extra", [[], ["--host-edges"]])
+def test_driver_g5bench_s16(extra):
+    """The C++ driver over the BfsOnGPU shim runs the main.cc flow: 16 roots,
+    every pred array valid, traversed edges = pf_nedge[16] (mpi/utils.hpp:3664)."""
+    import subprocess
+    from graph500_amd import build
+    say("g5bench 16", extra)
+    r = subprocess.run([build.DRIVER, "16", "--roots", "16", *extra], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    kv = dict(l.split(":", 1) for l in r.stdout.splitlines() if ":" in l and not l.startswith("="))
+    assert int(kv["NBFS"]) == 16
+    assert float(kv["min_nedge"]) == float(kv["max_nedge"]) == 1048570
+    assert float(kv["harmonic_mean_TEPS"]) > 0
