@@ -19,8 +19,9 @@ warm-up), with the host clock around each synchronous call like MPI_Wtime
 around run_bfs; GTEPS = 1 / mean(t_i / e_i) with e_i the validator's
 traversed-edge count.
 
-`roofline` is for the dominant kernel of the step (the batched pull sweep
-k_ms_pull): algorithmic bytes of its launches (byte model evaluated on the
+`roofline` is for the dominant kernel of the step (the kind of batch level
+with the most device time: push, vertex-parallel pull or edge-parallel
+sweep): algorithmic bytes of its launches (byte model evaluated on the
 GPU from the final levels, outside timing; equal to the CPU oracle's count,
 see tests) over their HIP-event durations on the library stream.  `traffic`
 is the HBM bytes per launch measured by rocprof (FETCH_SIZE + WRITE_SIZE,
@@ -29,8 +30,11 @@ and workload is present under profiles/, else null.  `cpu_baseline` times
 the reference CPU runnable (oracle/_ref/runnable) on a bounded sample on this
 host, or the oracle port when that cannot run.
 
-Other modes:  --copy-calibration  runs a plain 1 GiB device copy (for scaling
-rocprof FETCH_SIZE/WRITE_SIZE counters).
+Other modes:
+  --copy-calibration      a plain 1 GiB device copy (for scaling rocprof
+                          FETCH_SIZE/WRITE_SIZE counters)
+  --rocprof-summary DB    per-kernel time, or per-kernel counter averages, of a
+                          rocprofv3 results database (the text kept in profiles/)
 """
 import argparse
 import json
@@ -46,6 +50,12 @@ sys.path.insert(0, HERE)
 
 METRIC = "GTEPS hmean SCALE 28 ef16: 64-root batch vs single-root; serve p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+# batch level kinds (g5_batch_stats.direction) -> (profile name, kernel description)
+BATCH_KERNELS = {
+    0: ("k_ms_pull", "k_ms_pull (batched vertex-parallel pull)"),
+    1: ("k_ms_push", "k_ms_push + k_ms_finish (batched push)"),
+    2: ("k_ms_sweep", "k_ms_sweep + k_ms_finish (batched edge-parallel pull sweep)"),
+}
 
 
 def log(*a):
@@ -104,6 +114,34 @@ def copy_calibration():
     b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+
+
+def rocprof_summary(db, top=30):
+    """Text summary of a rocprofv3 SQLite results database: per-kernel calls,
+    total and average duration from a --kernel-trace run, or per-kernel
+    dispatch-averaged counter values from a --pmc run."""
+    import collections
+    import sqlite3
+    cur = sqlite3.connect(db).cursor()
+    tables = {r[0] for r in cur.execute("select name from sqlite_master where type in ('table','view')")}
+    lines = []
+    if "counters_collection" in tables and cur.execute("select count(*) from counters_collection").fetchone()[0]:
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        for k, c, v in cur.execute("select kernel_name, counter_name, value from counters_collection"):
+            key = (k.split("(")[0], c)
+            agg[key][0] += 1
+            agg[key][1] += v
+        lines.append("%-48s %-12s %10s %18s" % ("kernel", "counter", "dispatches", "avg per dispatch"))
+        for (k, c), (n, tot) in sorted(agg.items(), key=lambda x: -x[1][1])[:top]:
+            lines.append("%-48s %-12s %10d %18.1f" % (k[:48], c, n, tot / n))
+    else:
+        rows = cur.execute("select name, total_calls, total_duration, average, percentage "
+                           "from top_kernels").fetchall()
+        lines.append("%-48s %8s %12s %12s %7s" % ("kernel", "calls", "total ms", "avg us", "pct"))
+        for name, calls, tot, avg, pct in rows[:top]:
+            lines.append("%-48s %8d %12.3f %12.2f %7.2f" % (name.split("(")[0][:48], calls, tot / 1e6,
+                                                             avg / 1e3, pct))
+    return "\n".join(lines)
 
 
 def traffic_from_profile(kernel, scale):
@@ -176,8 +214,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single-root", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--rocprof-summary", metavar="DB")
     a = ap.parse_args()
 
+    if a.rocprof_summary:
+        print(rocprof_summary(a.rocprof_summary))
+        return 0
     if a.copy_calibration:
         copy_calibration()
         return 0
@@ -209,8 +251,9 @@ def main():
     edges = ctx.batch_teps_edges()
     total_edges = int(edges.sum())
     per, tot = ctx.batch_model_bytes()  # the schedule is the same every step
-    wall, dev, pull_ms, pull_launches = [], [], 0.0, 0
-    pull_bytes = 0.0
+    wall, dev = [], []
+    # per level kind (push / vertex-parallel pull / edge-parallel sweep): launches, ms, model bytes
+    kinds = {d: [0, 0.0, 0.0] for d in BATCH_KERNELS}
     t_all = time.perf_counter()
     for i in range(a.steps):
         ctx.synchronize()
@@ -221,19 +264,20 @@ def main():
         wall.append(t1 - t0)
         dev.append(st["device_ms"] / 1e3)
         for L, d in enumerate(st["direction"]):
-            if d == 0:
-                pull_ms += st["level_ms"][L]
-                pull_bytes += per[L]
-                pull_launches += 1
+            kinds[d][0] += 1
+            kinds[d][1] += st["level_ms"][L]
+            kinds[d][2] += per[L]
         log("batch step %d: %.3f ms %.1f GTEPS" % (i, 1e3 * (t1 - t0), total_edges / (t1 - t0) / 1e9))
     t_all = time.perf_counter() - t_all
     st = ctx.batch_stats()
     assert (ctx.batch_teps_edges() == edges).all()
     mean_wall = sum(wall) / len(wall)
     gteps = total_edges / mean_wall / 1e9
-    pull_gbs = pull_bytes / (pull_ms / 1e3) / 1e9 if pull_ms > 0 else 0.0
-    per_launch = pull_bytes / max(pull_launches, 1)
-    traffic = traffic_from_profile("k_ms_pull", a.scale)
+    dom = max(kinds, key=lambda d: kinds[d][1])  # the level kind with the most device time
+    dom_launches, dom_ms, dom_bytes = kinds[dom]
+    dom_gbs = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    per_launch = dom_bytes / max(dom_launches, 1)
+    traffic = traffic_from_profile(BATCH_KERNELS[dom][0], a.scale)
     out = {
         "metric": METRIC,
         "value": round(gteps, 3),
@@ -260,15 +304,15 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_ms_pull (batched bottom-up sweep)",
-            "achieved": round(pull_gbs, 1),
+            "kernel": BATCH_KERNELS[dom][1],
+            "achieved": round(dom_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(pull_gbs / HBM_PEAK_GBS, 4),
+            "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "launches": pull_launches,
+            "launches": dom_launches,
             "algorithmic_bytes_per_launch": round(per_launch),
-            "avg_launch_ms": round(pull_ms / max(pull_launches, 1), 4),
+            "avg_launch_ms": round(dom_ms / max(dom_launches, 1), 4),
         },
         "batch": {
             "gteps": round(gteps, 3),
@@ -276,7 +320,7 @@ def main():
             "traversed_edges_sum": total_edges,
             "bytes_per_edge": round(tot / total_edges, 4),
             "roofline_frac_whole_batch": round((tot / (sum(dev) / len(dev))) / (HBM_PEAK_GBS * 1e9), 4),
-            "levels": "".join("P" if d else "U" for d in st["direction"]),
+            "levels": "".join("UPS"[d] for d in st["direction"]),
             "level_ms": [round(x, 3) for x in st["level_ms"]],
             "tuning": ctx.batch_tuning(),
             "timed_region_s": round(t_all, 3),

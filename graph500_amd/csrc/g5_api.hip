@@ -379,9 +379,18 @@ int g5_batch_prepare(g5_ctx* ctx) {
 
 int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* t) {
   return guarded(ctx, [&] {
-    G5_CHECK(t && t->alpha > 0 && t->pull_grid >= 1, "g5_batch_set_tuning: bad tuning");
+    G5_CHECK(t && t->alpha > 0 && t->pull_grid >= 1 && t->pull_grid <= (1 << 20) && t->sweep_below >= 0 &&
+                 t->sweep_grid >= 1 && t->sweep_grid <= (1 << 20) && (t->persistent == 0 || t->persistent == 1) &&
+                 t->persist_levels >= 2 && t->persist_levels <= G5_MAX_LEVELS - 2 && t->persist_grid >= 0 &&
+                 t->persist_grid <= (1 << 20),
+             "g5_batch_set_tuning: bad tuning");
+    ctx->ms.persistent = t->persistent;
+    ctx->ms.persist_levels = t->persist_levels;
+    ctx->ms.persist_grid = t->persist_grid;
     ctx->ms.alpha = t->alpha;
     ctx->ms.pull_grid = t->pull_grid;
+    ctx->ms.sweep_below = t->sweep_below;
+    ctx->ms.sweep_grid = t->sweep_grid;
   });
 }
 
@@ -390,6 +399,11 @@ int g5_batch_get_tuning(g5_ctx* ctx, g5_batch_tuning* t) {
     G5_CHECK(t, "g5_batch_get_tuning: null output");
     t->alpha = ctx->ms.alpha;
     t->pull_grid = ctx->ms.pull_grid;
+    t->sweep_below = ctx->ms.sweep_below;
+    t->sweep_grid = ctx->ms.sweep_grid;
+    t->persistent = ctx->ms.persistent;
+    t->persist_levels = ctx->ms.persist_levels;
+    t->persist_grid = ctx->ms.persist_grid;
   });
 }
 

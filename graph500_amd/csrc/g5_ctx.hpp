@@ -95,7 +95,26 @@ struct MsState {
   int nlevels = 0;
   std::vector<int64_t> host_roots;
   double alpha = 16.0;       // push while the frontier union has < nnz/alpha edges
-  int64_t pull_grid = 8192;  // workgroups of the pull sweep
+  int64_t pull_grid = 8192;  // workgroups of the vertex-parallel pull
+  double sweep_below = 0.25; // edge-parallel sweep while reached pairs < this x roots x rows
+  int64_t sweep_grid = 4096; // workgroups of the edge-parallel sweep
+  uint64_t sweep_chunks = 0;
+  DevBuf<uint32_t> sweep_rows;  // first/last row of every sweep chunk of the CSR
+  // persistent level-fused path (one hipGraph replay per batch)
+  int persistent = 1;
+  int persist_levels = 16;     // frontier-word arrays preallocated for it
+  int64_t persist_grid = 0;    // 0: every co-resident workgroup
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_levels = 0;
+  double graph_alpha = 0, graph_sweep = 0;
+  int64_t graph_grid = 0;
+  unsigned p_grid = 0;
+  int clock_khz = 100000;
+  DevBuf<uint64_t*> p_fr;
+  DevBuf<uint64_t> p_cnt, p_misc;
+  DevBuf<int64_t> p_ts;
+  DevBuf<int> p_dir;
+  uint64_t* h_persist = nullptr;
   DevBuf<uint64_t> seen;     // n_rows
   DevBuf<uint64_t> nz[2];    // frontier-word-not-zero bitmaps, alternating by level
   std::vector<std::unique_ptr<DevBuf<uint64_t>>> fr;  // frontier words per level
@@ -121,6 +140,18 @@ struct MsState {
     scan_tmp.release();
     counters.release();
     roots.release();
+    sweep_rows.release();
+    sweep_chunks = 0;
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    graph_levels = 0;
+    p_fr.release();
+    p_cnt.release();
+    p_misc.release();
+    p_ts.release();
+    p_dir.release();
+    if (h_persist) (void)hipHostFree(h_persist);
+    h_persist = nullptr;
     if (h_counters) (void)hipHostFree(h_counters);
     h_counters = nullptr;
     if (ev0) (void)hipEventDestroy(ev0);

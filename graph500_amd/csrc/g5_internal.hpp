@@ -42,6 +42,11 @@ struct Error {
     if (!(cond)) throw ::g5::Error{std::string(text)}; \
   } while (0)
 
+#define G5_CHECK_CODE(cond, code, text)                       \
+  do {                                                        \
+    if (!(cond)) throw ::g5::Error{std::string(text), (code)}; \
+  } while (0)
+
 constexpr int kWave = 64;
 
 // ---------------------------------------------------------------------------

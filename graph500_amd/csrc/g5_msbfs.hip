@@ -85,6 +85,7 @@ __global__ void k_ms_roots(const uint32_t* __restrict__ old2new, const int64_t* 
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint64_t live = 0, nz = 0, fdeg = 0;
   for (int r = 0; r < nroots; ++r) {
+    if (roots[r] < 0) continue;  // unused lane
     const uint32_t v = old2new[roots[r]];
     if ((int64_t)v >= rows) continue;  // a root without edges discovers nothing
     const uint64_t bit = UINT64_C(1) << r;
@@ -107,6 +108,7 @@ __global__ void k_ms_roots_nz(const uint32_t* __restrict__ old2new, const int64_
                               int64_t rows, uint64_t* __restrict__ nz0) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   for (int r = 0; r < nroots; ++r) {
+    if (roots[r] < 0) continue;
     const uint32_t v = old2new[roots[r]];
     if ((int64_t)v < rows) nz0[v >> 6] |= UINT64_C(1) << (v & 63);
   }
@@ -152,17 +154,18 @@ __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__
   }
 }
 
-// After a push: next keeps only lanes new to v, seen gains them, and
-// nz_next gets one bit per vertex whose frontier word is not zero.
-__global__ void __launch_bounds__(kBlock) k_ms_finish(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
-                                                      uint64_t* __restrict__ nz_next,
-                                                      const uint64_t* __restrict__ offsets, int64_t rows,
-                                                      unsigned long long* __restrict__ c) {
+// After a push or a sweep: next keeps only lanes new to v, seen gains them,
+// and nz_next gets one bit per vertex whose frontier word is not zero.
+// Blocks bid of nblk take 64-vertex words in a grid-stride loop.
+__device__ __forceinline__ void ms_finish_body(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
+                                               uint64_t* __restrict__ nz_next, const uint64_t* __restrict__ offsets,
+                                               int64_t rows, unsigned long long* __restrict__ c, unsigned bid,
+                                               unsigned nblk) {
   uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
   const int64_t words = (rows + 63) / 64;
   const int lane = threadIdx.x & 63;
-  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
-  for (int64_t wd = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
+  const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
+  for (int64_t wd = (int64_t)bid * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
     const int64_t v = wd * 64 + lane;
     uint64_t nw = 0;
     if (v < rows) {
@@ -183,19 +186,25 @@ __global__ void __launch_bounds__(kBlock) k_ms_finish(uint64_t* __restrict__ see
   ms_block_counters(live, pairs, nz, fdeg, c);
 }
 
+__global__ void __launch_bounds__(kBlock) k_ms_finish(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
+                                                      uint64_t* __restrict__ nz_next,
+                                                      const uint64_t* __restrict__ offsets, int64_t rows,
+                                                      unsigned long long* __restrict__ c) {
+  ms_finish_body(seen, next, nz_next, offsets, rows, c, blockIdx.x, gridDim.x);
+}
+
 // Pull step: one lane per vertex, the wave helps with long rows.
-__global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
-                                                    const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
-                                                    uint64_t* __restrict__ nz_next,
-                                                    const uint32_t* __restrict__ first_nbr,
-                                                    const uint64_t* __restrict__ offsets,
-                                                    const uint32_t* __restrict__ cols, int64_t rows,
-                                                    uint64_t live_in, unsigned long long* __restrict__ c) {
+__device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
+                                             const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
+                                             uint64_t* __restrict__ nz_next, const uint32_t* __restrict__ first_nbr,
+                                             const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols,
+                                             int64_t rows, uint64_t live_in, unsigned long long* __restrict__ c,
+                                             unsigned bid, unsigned nblk) {
   const int lane = threadIdx.x & 63;
-  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
   uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
   const int64_t words = (rows + 63) / 64;
-  for (int64_t wd = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
+  for (int64_t wd = (int64_t)bid * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
     const int64_t v = wd * 64 + lane;
     const bool inb = v < rows;
     const uint64_t s = inb ? seen[v] : ~UINT64_C(0);
@@ -257,6 +266,309 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
     }
   }
   ms_block_counters(live, pairs, nz, fdeg, c);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
+                                                    const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
+                                                    uint64_t* __restrict__ nz_next,
+                                                    const uint32_t* __restrict__ first_nbr,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ cols, int64_t rows,
+                                                    uint64_t live_in, unsigned long long* __restrict__ c) {
+  ms_pull_body(seen, front, nz_front, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x,
+               gridDim.x);
+}
+
+// Dense pull step as an edge-parallel sweep over the whole CSR: each lane
+// takes kSweepEpt consecutive column entries (a wave reads 2 KiB of cols in
+// one coalesced pass), ORs the frontier words of the neighbours whose
+// nonzero bit is set into its row's LDS accumulator, and the block writes
+// one word per row: a plain store for rows that lie inside the chunk, an
+// atomicOr for the (at most two) rows that cross its ends, whose words were
+// cleared beforehand by k_ms_sweep_clear.  No early exit: this is the form
+// for levels at which most rows would be scanned to the end anyway.  Rows
+// are those of modes 1 and 2 (every row has at least one entry), so at most
+// kSweepEpb rows overlap a chunk.  next then holds, per vertex, the lanes of
+// its neighbours' frontier words; k_ms_finish keeps the new ones.
+constexpr int kSweepEpt = 8;
+constexpr int kSweepEpb = kSweepEpt * kBlock;
+
+__device__ __forceinline__ void ms_sweep_clear_body(const uint32_t* __restrict__ chunk_rows, uint64_t nchunks,
+                                                    const uint64_t* __restrict__ offsets, uint64_t* __restrict__ next,
+                                                    unsigned bid, unsigned nblk) {
+  for (uint64_t c = (uint64_t)bid * kBlock + threadIdx.x; c < nchunks; c += (uint64_t)nblk * kBlock) {
+    const uint32_t r0 = chunk_rows[2 * c], r1 = chunk_rows[2 * c + 1];
+    const uint64_t e0 = c * kSweepEpb;
+    if (offsets[r0] < e0) next[r0] = 0;                  // first row began in an earlier chunk
+    if (offsets[r1 + 1] > e0 + kSweepEpb) next[r1] = 0;  // last row goes on into a later chunk
+  }
+}
+
+// LDS of one sweep chunk: row starts (rows r0 .. r0+nr) and row accumulators.
+struct SweepLds {
+  uint64_t off[kSweepEpb + 2];
+  unsigned long long acc[kSweepEpb + 1];
+};
+
+// Stages the row starts of chunk c; returns the row count (every lane calls
+// it; ends with a barrier).
+__device__ __forceinline__ int sweep_stage(uint64_t chunk, const uint32_t* __restrict__ chunk_rows,
+                                           const uint64_t* __restrict__ offsets, SweepLds& L, bool clear_acc) {
+  const uint32_t r0 = chunk_rows[2 * chunk];
+  int nr = (int)(chunk_rows[2 * chunk + 1] - r0 + 1);
+  nr = nr > kSweepEpb ? kSweepEpb : nr;  // bound for LDS; rows are non-empty in modes 1 and 2
+  for (int i = threadIdx.x; i <= nr; i += kBlock) {
+    L.off[i] = offsets[r0 + i];
+    if (clear_acc && i < nr) L.acc[i] = 0;
+  }
+  __syncthreads();
+  return nr;
+}
+
+// Last staged row starting at or before edge e (LDS binary search).
+__device__ __forceinline__ int sweep_row_of(const SweepLds& L, int nr, uint64_t e) {
+  int lo = 0, hi = nr - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L.off[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void ms_sweep_body(const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
+                                              const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols,
+                                              const uint64_t* __restrict__ front,
+                                              const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
+                                              SweepLds& L, unsigned bid, unsigned nblk) {
+  for (uint64_t chunk = bid; chunk * kSweepEpb < nnz; chunk += nblk) {
+    const uint64_t e0 = chunk * kSweepEpb;
+    const uint64_t e1 = (e0 + kSweepEpb < nnz) ? e0 + kSweepEpb : nnz;
+    const uint32_t r0 = chunk_rows[2 * chunk];
+    const int nr = sweep_stage(chunk, chunk_rows, offsets, L, true);
+    const uint64_t my0 = e0 + (uint64_t)threadIdx.x * kSweepEpt;
+    if (my0 < e1) {
+      int lo = sweep_row_of(L, nr, my0);
+      const uint64_t my1 = (my0 + kSweepEpt < e1) ? my0 + kSweepEpt : e1;
+      uint32_t w[kSweepEpt];
+#pragma unroll
+      for (int j = 0; j < kSweepEpt; ++j) w[j] = (my0 + j < my1) ? cols[my0 + j] : 0;
+      uint64_t x[kSweepEpt];
+#pragma unroll
+      for (int j = 0; j < kSweepEpt; ++j) x[j] = (my0 + j < my1 && bit_set(nz_front, w[j])) ? front[w[j]] : 0;
+      uint64_t acc = 0;
+      for (int j = 0; j < kSweepEpt && my0 + j < my1; ++j) {
+        while (lo + 1 < nr && L.off[lo + 1] <= my0 + j) {
+          if (acc) atomicOr(&L.acc[lo], (unsigned long long)acc);
+          acc = 0;
+          ++lo;
+        }
+        acc |= x[j];
+      }
+      if (acc) atomicOr(&L.acc[lo], (unsigned long long)acc);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nr; i += kBlock) {
+      const uint64_t a = L.acc[i];
+      if (L.off[i] >= e0 && L.off[i + 1] <= e1)
+        next[r0 + i] = a;
+      else if (a)
+        atomicOr((unsigned long long*)(next + r0 + i), (unsigned long long)a);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ms_sweep_clear(const uint32_t* __restrict__ chunk_rows,
+                                                           uint64_t nchunks, const uint64_t* __restrict__ offsets,
+                                                           uint64_t* __restrict__ next) {
+  ms_sweep_clear_body(chunk_rows, nchunks, offsets, next, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ms_sweep(const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
+                                                     const uint64_t* __restrict__ offsets,
+                                                     const uint32_t* __restrict__ cols,
+                                                     const uint64_t* __restrict__ front,
+                                                     const uint64_t* __restrict__ nz_front,
+                                                     uint64_t* __restrict__ next) {
+  __shared__ SweepLds L;
+  ms_sweep_body(chunk_rows, nnz, offsets, cols, front, nz_front, next, L, blockIdx.x, gridDim.x);
+}
+
+// Push step over the CSR in sweep chunks, for the persistent kernel (no
+// frontier queue, so no scan): a chunk whose rows hold no frontier vertex is
+// skipped after a look at its nonzero-bitmap words; otherwise each lane takes
+// kSweepEpt consecutive entries and, for those whose row u is in the
+// frontier, sends front[u] & ~seen[w] to the target w with a filtered
+// 64-bit atomicOr.  next must be zero beforehand; seen is not written.
+__device__ __forceinline__ void ms_push_csr_body(const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
+                                                 const uint64_t* __restrict__ offsets,
+                                                 const uint32_t* __restrict__ cols,
+                                                 const uint64_t* __restrict__ front,
+                                                 const uint64_t* __restrict__ nz_front,
+                                                 const uint64_t* __restrict__ seen, uint64_t* next, SweepLds& L,
+                                                 unsigned bid, unsigned nblk) {
+  for (uint64_t chunk = bid; chunk * kSweepEpb < nnz; chunk += nblk) {
+    const uint32_t r0 = chunk_rows[2 * chunk], r1 = chunk_rows[2 * chunk + 1];
+    // any frontier row in [r0, r1]?
+    bool any = false;
+    for (uint32_t wd = (r0 >> 6) + threadIdx.x; wd <= (r1 >> 6); wd += kBlock) {
+      uint64_t x = nz_front[wd];
+      if (wd == (r0 >> 6)) x &= ~UINT64_C(0) << (r0 & 63);
+      if (wd == (r1 >> 6) && (r1 & 63) != 63) x &= (UINT64_C(1) << ((r1 & 63) + 1)) - 1;
+      any |= x != 0;
+    }
+    if (!__syncthreads_or(any)) continue;
+    const uint64_t e0 = chunk * kSweepEpb;
+    const uint64_t e1 = (e0 + kSweepEpb < nnz) ? e0 + kSweepEpb : nnz;
+    const int nr = sweep_stage(chunk, chunk_rows, offsets, L, false);
+    const uint64_t my0 = e0 + (uint64_t)threadIdx.x * kSweepEpt;
+    if (my0 < e1) {
+      int lo = sweep_row_of(L, nr, my0);
+      const uint64_t my1 = (my0 + kSweepEpt < e1) ? my0 + kSweepEpt : e1;
+      uint32_t u = r0 + lo;
+      uint64_t fu = bit_set(nz_front, u) ? front[u] : 0;
+      for (uint64_t e = my0; e < my1; ++e) {
+        if (lo + 1 < nr && L.off[lo + 1] <= e) {
+          while (lo + 1 < nr && L.off[lo + 1] <= e) ++lo;
+          u = r0 + lo;
+          fu = bit_set(nz_front, u) ? front[u] : 0;
+        }
+        if (!fu) continue;
+        const uint32_t w = cols[e];
+        const uint64_t m = fu & ~seen[w];
+        if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
+          atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void ms_zero_body(uint64_t* __restrict__ a, int64_t n, unsigned bid, unsigned nblk) {
+  for (int64_t i = (int64_t)bid * kBlock + threadIdx.x; i < n; i += (int64_t)nblk * kBlock) a[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent level-fused batch: one launch runs every level, the grid meeting
+// at a barrier between the phases of a level.  All blocks read the same level
+// counters after a barrier and so take the same direction decision; the loop
+// ends when no lane discovers anything, when the preallocated frontier-word
+// arrays run out (the host loop continues from there), or when a barrier wait
+// exceeds its deadline (then every block leaves and the host reports it).
+// ---------------------------------------------------------------------------
+struct GridBarrier {
+  unsigned count;
+  unsigned gen;
+  unsigned abort;
+  unsigned pad;
+};
+
+// Returns false (in every thread of the block) once any block has given up.
+__device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned nblk, long long deadline) {
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const unsigned g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    if (__hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nblk - 1) {
+      __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+            wall_clock64() > deadline) {
+          __hip_atomic_store(&b->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __threadfence();
+    if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+struct MsPersistArgs {
+  const uint64_t* offsets;
+  const uint32_t* cols;
+  const uint32_t* first_nbr;
+  const uint32_t* sweep_rows;  // first/last row per sweep chunk
+  int64_t rows;
+  uint64_t nnz;
+  uint64_t sweep_chunks;
+  uint64_t* seen;
+  uint64_t* const* fr;          // frontier words of levels 0 .. max_levels-1
+  uint64_t* nz0;
+  uint64_t* nz1;
+  unsigned long long* cnt;      // [max_levels][M_NUM]; level 0 filled by k_ms_roots, rest zero
+  long long* ts;                // [max_levels] wall clock at the end of each level
+  int* dir;                     // [max_levels] level kind (1 push, 0 pull, 2 sweep)
+  int* done_levels;             // last level completed
+  GridBarrier* bar;
+  double push_limit;            // nnz / alpha
+  double sweep_below;           // sweep while reached pairs < sweep_below x roots x rows
+  int max_levels;
+  long long timeout_ticks;      // per-barrier wait bound (wall clock ticks)
+};
+
+__global__ void __launch_bounds__(kBlock) k_ms_persistent(MsPersistArgs a) {
+  __shared__ SweepLds L;
+  const unsigned bid = blockIdx.x, nblk = gridDim.x;
+  const int64_t words = (a.rows + 63) / 64;
+  if (bid == 0 && threadIdx.x == 0) a.ts[0] = wall_clock64();
+  uint64_t live = __hip_atomic_load(&a.cnt[M_LIVE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const double roots_live = (double)__popcll(live);
+  const double sweep_limit = a.sweep_below * roots_live * (double)a.rows;
+  double reached = roots_live;
+  int l = 0;
+  while (live && l + 1 < a.max_levels) {
+    const unsigned long long* cp = a.cnt + (size_t)l * M_NUM;
+    const uint64_t fdeg = __hip_atomic_load(&cp[M_FDEG], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ++l;
+    unsigned long long* c = a.cnt + (size_t)l * M_NUM;
+    const uint64_t* front = a.fr[l - 1];
+    uint64_t* next = a.fr[l];
+    const uint64_t* nz_front = ((l - 1) & 1) ? a.nz1 : a.nz0;
+    uint64_t* nz_next = (l & 1) ? a.nz1 : a.nz0;
+    const bool push = (double)fdeg < a.push_limit;
+    const bool sweep = !push && a.sweep_chunks && reached < sweep_limit;
+    const long long deadline = wall_clock64() + a.timeout_ticks;
+    if (push) {
+      ms_zero_body(next, a.rows, bid, nblk);
+      if (!grid_barrier(a.bar, nblk, deadline)) return;
+      ms_push_csr_body(a.sweep_rows, a.nnz, a.offsets, a.cols, front, nz_front, a.seen, next, L, bid, nblk);
+      if (!grid_barrier(a.bar, nblk, deadline)) return;
+      ms_finish_body(a.seen, next, nz_next, a.offsets, a.rows, c, bid, nblk);
+    } else if (sweep) {
+      ms_sweep_clear_body(a.sweep_rows, a.sweep_chunks, a.offsets, next, bid, nblk);
+      if (!grid_barrier(a.bar, nblk, deadline)) return;
+      ms_sweep_body(a.sweep_rows, a.nnz, a.offsets, a.cols, front, nz_front, next, L, bid, nblk);
+      if (!grid_barrier(a.bar, nblk, deadline)) return;
+      ms_finish_body(a.seen, next, nz_next, a.offsets, a.rows, c, bid, nblk);
+    } else {
+      ms_pull_body(a.seen, front, nz_front, next, nz_next, a.first_nbr, a.offsets, a.cols, a.rows, live, c, bid,
+                   nblk);
+    }
+    if (!grid_barrier(a.bar, nblk, deadline)) return;
+    live = __hip_atomic_load(&c[M_LIVE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    reached += (double)__hip_atomic_load(&c[M_PAIRS], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (bid == 0 && threadIdx.x == 0) {
+      a.ts[l] = wall_clock64();
+      a.dir[l] = push ? 1 : (sweep ? 2 : 0);
+      *a.done_levels = l;
+    }
+  }
+  (void)words;
+}
+
+// Zero fill (graph-capturable, unlike a memset node).
+__global__ void __launch_bounds__(kBlock) k_ms_fill(uint64_t* __restrict__ a, int64_t n) {
+  ms_zero_body(a, n, blockIdx.x, gridDim.x);
 }
 
 // Per-root level of every original vertex: the first level whose frontier
@@ -355,6 +667,14 @@ void ms_prepare(g5_ctx* ctx) {
   m.counters.alloc(M_NUM);
   m.roots.alloc(64);
   m.frp.alloc(G5_MAX_LEVELS);
+  if (g.reorder_mode != 0 && g.nnz > 0) {
+    // row range of every sweep chunk of the CSR (static for the graph)
+    m.sweep_chunks = ((uint64_t)g.nnz + kSweepEpb - 1) / kSweepEpb;
+    m.sweep_rows.alloc(2 * m.sweep_chunks);
+    hipLaunchKernelGGL(k_chunk_rows<kSweepEpb>, dim3(grid_of((int64_t)(2 * m.sweep_chunks), kBlock, 65536)),
+                       dim3(kBlock), 0, ctx->stream, g.offsets.p, rows, m.sweep_chunks, m.sweep_rows.p);
+    G5_HIP(hipGetLastError());
+  }
   G5_HIP(hipHostMalloc((void**)&m.h_counters, M_NUM * sizeof(uint64_t), hipHostMallocDefault));
   G5_HIP(hipEventCreate(&m.ev0));
   G5_HIP(hipEventCreate(&m.ev1));
@@ -374,48 +694,30 @@ static uint64_t* ms_level(MsState& m, int l, int64_t rows) {
   return m.fr[l]->p;
 }
 
-void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots) {
+// Host-driven levels from level l0 on (counters of level l0 - 1 in live /
+// nz / fdeg, reached pairs so far in reached): one or more launches and one
+// small device-to-host copy per level.
+static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint64_t fdeg, double reached,
+                           double roots_live) {
   Graph& g = ctx->graph;
   MsState& m = ctx->ms;
-  G5_CHECK(m.prepared, "g5_run_bfs_batch: call g5_batch_prepare first");
-  G5_CHECK(nroots >= 1 && nroots <= 64, "g5_run_bfs_batch: 1 to 64 roots per batch");
-  for (int r = 0; r < nroots; ++r)
-    G5_CHECK(roots[r] >= 0 && roots[r] < g.n, "g5_run_bfs_batch: root out of range");
   hipStream_t s = ctx->stream;
   const int64_t rows = g.n_rows;
-  unsigned long long* cnt = (unsigned long long*)m.counters.p;
-  m.have_result = false;
-  m.nroots = nroots;
-  m.host_roots.assign(roots, roots + nroots);
-  g5_batch_stats& st = m.stats;
-  st = g5_batch_stats{};
-  st.num_roots = nroots;
-  G5_HIP(hipMemcpyAsync(m.roots.p, roots, nroots * sizeof(int64_t), hipMemcpyHostToDevice, s));
-  uint64_t* fr0 = ms_level(m, 0, rows);
-
   const int64_t words = (rows + 63) / 64;
-  G5_HIP(hipEventRecord(m.ev0, s));
-  hipLaunchKernelGGL(k_ms_clear, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, m.seen.p, fr0, rows);
-  G5_HIP(hipMemsetAsync(m.nz[0].p, 0, words * 8, s));
-  hipLaunchKernelGGL(k_ms_roots, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, nroots, rows, g.offsets.p,
-                     m.seen.p, fr0, cnt);
-  hipLaunchKernelGGL(k_ms_roots_nz, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, nroots, rows, m.nz[0].p);
-  G5_HIP(hipGetLastError());
-  G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, M_NUM * 8, hipMemcpyDeviceToHost, s));
-  G5_HIP(hipStreamSynchronize(s));
-  uint64_t live = m.h_counters[M_LIVE];
-  uint64_t nz = m.h_counters[M_NZ], fdeg = m.h_counters[M_FDEG];
+  unsigned long long* cnt = (unsigned long long*)m.counters.p;
+  g5_batch_stats& st = m.stats;
   const double push_limit = (double)g.nnz / m.alpha;
-  int l = 0;
+  const double sweep_limit = m.sweep_below * roots_live * (double)rows;
+  int l = l0 - 1;
   while (live) {
     ++l;
     G5_CHECK(l < G5_MAX_LEVELS - 1, "g5_run_bfs_batch: more than 254 levels (uint8 level arrays)");
-    const uint64_t* front = m.fr[l - 1]->p;
     uint64_t* next = ms_level(m, l, rows);
-    front = m.fr[l - 1]->p;  // the vector may have grown
+    const uint64_t* front = m.fr[l - 1]->p;  // after ms_level: the vector may have grown
     const uint64_t* nz_front = m.nz[(l - 1) & 1].p;
     uint64_t* nz_next = m.nz[l & 1].p;
     const bool push = (double)fdeg < push_limit;
+    const bool sweep = !push && m.sweep_chunks && reached < sweep_limit;
     G5_HIP(hipMemsetAsync(m.counters.p, 0, M_NUM * 8, s));
     G5_HIP(hipEventRecord(m.lev_begin[l], s));
     if (push) {
@@ -434,6 +736,14 @@ void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots) {
       G5_HIP(hipGetLastError());
       hipLaunchKernelGGL(k_ms_finish, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
                          next, nz_next, g.offsets.p, rows, cnt);
+    } else if (sweep) {
+      hipLaunchKernelGGL(k_ms_sweep_clear, dim3(grid_of((int64_t)m.sweep_chunks, kBlock, 4096)), dim3(kBlock), 0,
+                         s, m.sweep_rows.p, m.sweep_chunks, g.offsets.p, next);
+      hipLaunchKernelGGL(k_ms_sweep, dim3(grid_of((int64_t)m.sweep_chunks, 1, m.sweep_grid)), dim3(kBlock), 0, s,
+                         m.sweep_rows.p, (uint64_t)g.nnz, g.offsets.p, g.cols.p, front, nz_front, next);
+      G5_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_ms_finish, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
+                         next, nz_next, g.offsets.p, rows, cnt);
     } else {
       hipLaunchKernelGGL(k_ms_pull, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
                          m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows,
@@ -446,14 +756,186 @@ void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots) {
     float lms = 0;
     G5_HIP(hipEventElapsedTime(&lms, m.lev_begin[l], m.lev_end[l]));
     st.level_ms[l] = lms;
-    st.direction[l] = push ? 1 : 0;
+    st.direction[l] = push ? 1 : (sweep ? 2 : 0);
     st.pairs[l] = (int64_t)m.h_counters[M_PAIRS];
+    reached += (double)m.h_counters[M_PAIRS];
     st.num_levels = l + 1;
     live = m.h_counters[M_LIVE];
     nz = m.h_counters[M_NZ];
     fdeg = m.h_counters[M_FDEG];
   }
-  if (st.num_levels == 0) st.num_levels = 1;
+}
+
+// Sets up (once per graph and tuning) the persistent path's buffers and the
+// hipGraph holding the clears, the root placement and the persistent launch.
+static void ms_persistent_setup(g5_ctx* ctx) {
+  Graph& g = ctx->graph;
+  MsState& m = ctx->ms;
+  hipStream_t s = ctx->stream;
+  const int64_t rows = g.n_rows;
+  const int64_t words = (rows + 63) / 64;
+  const int K = m.persist_levels;
+  if (m.graph_exec && m.graph_levels == K && m.graph_alpha == m.alpha && m.graph_sweep == m.sweep_below &&
+      m.graph_grid == m.persist_grid)
+    return;
+  if (m.graph_exec) {
+    (void)hipGraphExecDestroy(m.graph_exec);
+    m.graph_exec = nullptr;
+  }
+  std::vector<uint64_t*> lev(K);
+  for (int l = 0; l < K; ++l) lev[l] = ms_level(m, l, rows);
+  m.p_fr.alloc(K);
+  G5_HIP(hipMemcpyAsync(m.p_fr.p, lev.data(), K * sizeof(uint64_t*), hipMemcpyHostToDevice, s));
+  G5_HIP(hipStreamSynchronize(s));  // lev is pageable and local
+  m.p_cnt.alloc((size_t)K * M_NUM);
+  m.p_ts.alloc(K);
+  m.p_dir.alloc(K);
+  m.p_misc.alloc(4);  // [0]: done level, [1..2]: grid barrier
+  if (!m.h_persist) {
+    // counters [levels][M_NUM] | clock [levels] | done level + barrier (4) | roots (64)
+    G5_HIP(hipHostMalloc((void**)&m.h_persist, ((size_t)G5_MAX_LEVELS * (M_NUM + 2) + 64) * sizeof(uint64_t),
+                         hipHostMallocDefault));
+  }
+  // co-resident grid: occupancy x compute units, capped by the tuning
+  int per_cu = 0, cus = 0;
+  G5_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ms_persistent, kBlock, 0));
+  G5_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  G5_CHECK(per_cu >= 1, "g5_run_bfs_batch: persistent kernel does not fit a compute unit");
+  int64_t grid = (int64_t)per_cu * cus;
+  if (m.persist_grid > 0 && m.persist_grid < grid) grid = m.persist_grid;
+  m.p_grid = (unsigned)grid;
+  int rate_khz = 0;
+  G5_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+  m.clock_khz = rate_khz > 0 ? rate_khz : 100000;
+
+  MsPersistArgs a{};
+  a.offsets = g.offsets.p;
+  a.cols = g.cols.p;
+  a.first_nbr = g.first_nbr.p;
+  a.sweep_rows = m.sweep_rows.p;
+  a.rows = rows;
+  a.nnz = (uint64_t)g.nnz;
+  a.sweep_chunks = m.sweep_chunks;
+  a.seen = m.seen.p;
+  a.fr = m.p_fr.p;
+  a.nz0 = m.nz[0].p;
+  a.nz1 = m.nz[1].p;
+  a.cnt = (unsigned long long*)m.p_cnt.p;
+  a.ts = (long long*)m.p_ts.p;
+  a.dir = (int*)m.p_dir.p;
+  a.done_levels = (int*)m.p_misc.p;
+  a.bar = (GridBarrier*)(m.p_misc.p + 1);
+  a.push_limit = (double)g.nnz / m.alpha;
+  a.sweep_below = m.sweep_below;
+  a.max_levels = K;
+  a.timeout_ticks = (long long)m.clock_khz * 1000LL * 10;  // 10 s per barrier wait
+
+  hipGraph_t graph = nullptr;
+  G5_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(k_ms_fill, dim3(grid_of((int64_t)K * M_NUM, kBlock, 64)), dim3(kBlock), 0, s, m.p_cnt.p,
+                     (int64_t)K * M_NUM);
+  hipLaunchKernelGGL(k_ms_fill, dim3(1), dim3(kBlock), 0, s, m.p_misc.p, (int64_t)4);
+  hipLaunchKernelGGL(k_ms_clear, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, m.seen.p, lev[0], rows);
+  hipLaunchKernelGGL(k_ms_fill, dim3(grid_of(words, kBlock, 1024)), dim3(kBlock), 0, s, m.nz[0].p, words);
+  hipLaunchKernelGGL(k_ms_roots, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, 64, rows, g.offsets.p, m.seen.p,
+                     lev[0], (unsigned long long*)m.p_cnt.p);
+  hipLaunchKernelGGL(k_ms_roots_nz, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, 64, rows, m.nz[0].p);
+  hipLaunchKernelGGL(k_ms_persistent, dim3(m.p_grid), dim3(kBlock), 0, s, a);
+  const hipError_t le = hipGetLastError();
+  const hipError_t ce = hipStreamEndCapture(s, &graph);
+  G5_HIP(le);
+  G5_HIP(ce);
+  const hipError_t ie = hipGraphInstantiate(&m.graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  G5_HIP(ie);
+  m.graph_levels = K;
+  m.graph_alpha = m.alpha;
+  m.graph_sweep = m.sweep_below;
+  m.graph_grid = m.persist_grid;
+}
+
+// One replay of the batch graph; levels beyond the preallocated arrays go on
+// in the host loop.
+static void ms_run_persistent(g5_ctx* ctx, const int64_t* roots, int nroots) {
+  Graph& g = ctx->graph;
+  MsState& m = ctx->ms;
+  hipStream_t s = ctx->stream;
+  ms_persistent_setup(ctx);
+  const int K = m.persist_levels;
+  int64_t* h_roots = (int64_t*)(m.h_persist + (size_t)G5_MAX_LEVELS * (M_NUM + 2));
+  for (int r = 0; r < 64; ++r) h_roots[r] = r < nroots ? roots[r] : -1;
+  G5_HIP(hipMemcpyAsync(m.roots.p, h_roots, 64 * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  G5_HIP(hipEventRecord(m.ev0, s));
+  G5_HIP(hipGraphLaunch(m.graph_exec, s));
+  uint64_t* h_cnt = m.h_persist;
+  int64_t* h_ts = (int64_t*)(m.h_persist + (size_t)G5_MAX_LEVELS * M_NUM);
+  uint64_t* h_misc = m.h_persist + (size_t)G5_MAX_LEVELS * (M_NUM + 1);
+  G5_HIP(hipMemcpyAsync(h_cnt, m.p_cnt.p, (size_t)K * M_NUM * 8, hipMemcpyDeviceToHost, s));
+  G5_HIP(hipMemcpyAsync(h_ts, m.p_ts.p, (size_t)K * 8, hipMemcpyDeviceToHost, s));
+  G5_HIP(hipMemcpyAsync(h_misc, m.p_misc.p, 4 * 8, hipMemcpyDeviceToHost, s));
+  std::vector<int> h_dir(K);
+  G5_HIP(hipMemcpyAsync(h_dir.data(), m.p_dir.p, (size_t)K * sizeof(int), hipMemcpyDeviceToHost, s));
+  G5_HIP(hipStreamSynchronize(s));
+  const GridBarrier* hb = (const GridBarrier*)(h_misc + 1);
+  G5_CHECK_CODE(!hb->abort, G5_ERR_HIP,
+                "g5_run_bfs_batch: a grid barrier of the persistent kernel timed out (blocks not co-resident?)");
+  const int done = (int)(int32_t)(h_misc[0] & 0xFFFFFFFFu);
+  g5_batch_stats& st = m.stats;
+  const double tick_ms = 1.0 / (double)m.clock_khz;
+  double reached = (double)__builtin_popcountll(h_cnt[M_LIVE]);
+  const double roots_live = reached;
+  for (int l = 1; l <= done; ++l) {
+    st.direction[l] = h_dir[l];
+    st.pairs[l] = (int64_t)h_cnt[(size_t)l * M_NUM + M_PAIRS];
+    st.level_ms[l] = (double)(h_ts[l] - h_ts[l - 1]) * tick_ms;
+    reached += (double)st.pairs[l];
+  }
+  st.num_levels = done + 1;
+  const uint64_t* last = h_cnt + (size_t)done * M_NUM;
+  if (last[M_LIVE]) {
+    // out of preallocated levels: continue level by level from the host
+    ms_host_levels(ctx, done + 1, last[M_LIVE], last[M_NZ], last[M_FDEG], reached, roots_live);
+  }
+  (void)g;
+}
+
+void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots) {
+  Graph& g = ctx->graph;
+  MsState& m = ctx->ms;
+  G5_CHECK(m.prepared, "g5_run_bfs_batch: call g5_batch_prepare first");
+  G5_CHECK(nroots >= 1 && nroots <= 64, "g5_run_bfs_batch: 1 to 64 roots per batch");
+  for (int r = 0; r < nroots; ++r)
+    G5_CHECK(roots[r] >= 0 && roots[r] < g.n, "g5_run_bfs_batch: root out of range");
+  hipStream_t s = ctx->stream;
+  const int64_t rows = g.n_rows;
+  unsigned long long* cnt = (unsigned long long*)m.counters.p;
+  m.have_result = false;
+  m.nroots = nroots;
+  m.host_roots.assign(roots, roots + nroots);
+  g5_batch_stats& st = m.stats;
+  st = g5_batch_stats{};
+  st.num_roots = nroots;
+  st.num_levels = 1;
+
+  if (m.persistent && m.sweep_chunks) {
+    ms_run_persistent(ctx, roots, nroots);
+  } else {
+    uint64_t* fr0 = ms_level(m, 0, rows);
+    const int64_t words = (rows + 63) / 64;
+    G5_HIP(hipMemcpyAsync(m.roots.p, roots, nroots * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    G5_HIP(hipEventRecord(m.ev0, s));
+    hipLaunchKernelGGL(k_ms_clear, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, m.seen.p, fr0, rows);
+    G5_HIP(hipMemsetAsync(m.nz[0].p, 0, words * 8, s));
+    hipLaunchKernelGGL(k_ms_roots, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, nroots, rows, g.offsets.p,
+                       m.seen.p, fr0, cnt);
+    hipLaunchKernelGGL(k_ms_roots_nz, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, nroots, rows, m.nz[0].p);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, M_NUM * 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    const uint64_t live = m.h_counters[M_LIVE];
+    const double roots_live = (double)__builtin_popcountll(live);
+    ms_host_levels(ctx, 1, live, m.h_counters[M_NZ], m.h_counters[M_FDEG], roots_live, roots_live);
+  }
   G5_HIP(hipEventRecord(m.ev1, s));
   G5_HIP(hipEventSynchronize(m.ev1));
   float ms = 0;

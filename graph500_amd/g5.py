@@ -92,7 +92,9 @@ class BatchStats(ctypes.Structure):
 
 
 class BatchTuning(ctypes.Structure):
-    _fields_ = [("alpha", ctypes.c_double), ("pull_grid", ctypes.c_int64)]
+    _fields_ = [("alpha", ctypes.c_double), ("pull_grid", ctypes.c_int64), ("sweep_below", ctypes.c_double),
+                ("sweep_grid", ctypes.c_int64), ("persistent", ctypes.c_int32), ("persist_levels", ctypes.c_int32),
+                ("persist_grid", ctypes.c_int64)]
 
 
 _lib = None
@@ -327,19 +329,25 @@ class G5Context:
     def batch_prepare(self):
         self._check(self._lib.g5_batch_prepare(self._h))
 
-    def batch_set_tuning(self, alpha=None, pull_grid=None):
+    TUNING_FIELDS = ("alpha", "pull_grid", "sweep_below", "sweep_grid", "persistent", "persist_levels",
+                     "persist_grid")
+
+    def batch_set_tuning(self, **kw):
+        """Sets any of TUNING_FIELDS; the others keep their values."""
+        bad = set(kw) - set(self.TUNING_FIELDS)
+        if bad:
+            raise TypeError("unknown tuning fields %s" % sorted(bad))
         t = BatchTuning()
         self._check(self._lib.g5_batch_get_tuning(self._h, ctypes.byref(t)))
-        if alpha is not None:
-            t.alpha = alpha
-        if pull_grid is not None:
-            t.pull_grid = pull_grid
+        for name, val in kw.items():
+            if val is not None:
+                setattr(t, name, val)
         self._check(self._lib.g5_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def batch_tuning(self):
         t = BatchTuning()
         self._check(self._lib.g5_batch_get_tuning(self._h, ctypes.byref(t)))
-        return {"alpha": t.alpha, "pull_grid": t.pull_grid}
+        return {name: getattr(t, name) for name in self.TUNING_FIELDS}
 
     def run_bfs_batch(self, roots):
         r = np.ascontiguousarray(roots, np.int64)

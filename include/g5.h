@@ -72,7 +72,8 @@ typedef struct g5_bfs_stats {
 typedef struct g5_batch_stats {
   int32_t num_roots;
   int32_t num_levels;               /* levels executed + 1 (last level finds nothing) */
-  int32_t direction[G5_MAX_LEVELS]; /* 1 = push (top-down), 0 = pull (bottom-up) */
+  int32_t direction[G5_MAX_LEVELS]; /* 1 = push (top-down), 0 = vertex-parallel pull (bottom-up),
+                                       2 = edge-parallel pull sweep */
   int64_t pairs[G5_MAX_LEVELS];     /* (root, vertex) pairs discovered at that level */
   double level_ms[G5_MAX_LEVELS];   /* device time per level (HIP events) */
   double device_ms;                 /* device time of the whole batch */
@@ -80,8 +81,17 @@ typedef struct g5_batch_stats {
 
 /* Tuning of the batched BFS (set by the autotuner, graph500_amd/autotune.py). */
 typedef struct g5_batch_tuning {
-  double alpha;       /* push while the frontier union has fewer than nnz/alpha edges */
-  int64_t pull_grid;  /* workgroups of the pull sweep (256 threads each) */
+  double alpha;        /* push while the frontier union has fewer than nnz/alpha edges */
+  int64_t pull_grid;   /* workgroups of the vertex-parallel pull (256 threads each) */
+  double sweep_below;  /* pull levels use the edge-parallel sweep while the (root, vertex)
+                          pairs reached so far are below sweep_below x roots x rows
+                          (reorder modes 1 and 2; 0 disables the sweep) */
+  int64_t sweep_grid;  /* workgroups of the edge-parallel sweep */
+  int32_t persistent;      /* 1: all levels in one persistent kernel launch, replayed as a hipGraph
+                              (reorder modes 1 and 2); 0: one launch sequence per level */
+  int32_t persist_levels;  /* levels the persistent launch covers (frontier arrays kept in HBM);
+                              deeper searches continue level by level */
+  int64_t persist_grid;    /* workgroups of the persistent launch; 0 = all co-resident ones */
 } g5_batch_tuning;
 
 /* ---- context ---- */

@@ -38,20 +38,29 @@ def s16ctx():
     c.close()
 
 
-@pytest.mark.parametrize("alpha", [16.0, 1e-9, 1e12])  # default, pull only, push only
-def test_batch_64_roots_match_reference_s16(s16ctx, alpha):
+# push while the union's edges < nnz / alpha: default; pull only (vertex-parallel);
+# pull only (edge-parallel sweep); push only; each through the persistent
+# graph-replayed kernel and through per-level launches
+@pytest.mark.parametrize("persistent", [1, 0])
+@pytest.mark.parametrize("alpha,sweep", [(16.0, 0.25), (1e12, 0.0), (1e12, 2.0), (1e-9, 0.25)])
+def test_batch_64_roots_match_reference_s16(s16ctx, alpha, sweep, persistent):
     g = gold("ref_s16_64roots.json")
     roots = [p["root"] for p in g["per_root"]]
-    s16ctx.batch_set_tuning(alpha=alpha)
-    say("batch s16 alpha", alpha)
+    s16ctx.batch_set_tuning(alpha=alpha, sweep_below=sweep, persistent=persistent)
+    say("batch s16 alpha", alpha, "sweep", sweep, "persistent", persistent)
     s16ctx.run_bfs_batch(roots)
+    st = s16ctx.batch_stats()
+    if alpha > 1e6:
+        assert set(st["direction"]) == ({2} if sweep > 1 else {0}), st["direction"]
+    if alpha < 1:
+        assert set(st["direction"]) == {1}, st["direction"]
     teps = s16ctx.batch_teps_edges()
     for i, pr in enumerate(g["per_root"]):
         assert sha(s16ctx.batch_levels(i)) == pr["levels_sha256"], (alpha, pr["root"])
         assert teps[i] == pr["nedge"]
     st = s16ctx.batch_stats()
     assert sum(st["pairs"]) + 64 == sum(sum(p["level_histogram"]) for p in g["per_root"])
-    s16ctx.batch_set_tuning(alpha=16.0)
+    s16ctx.batch_set_tuning(alpha=16.0, sweep_below=0.25, persistent=1)
 
 
 def test_batch_model_bytes_match_oracle(s16ctx, oracle_mod):
@@ -92,8 +101,8 @@ def test_batch_partial_and_odd_roots(oracle_mod):
                 assert np.array_equal(c.batch_levels(i), lv), (i, r)
 
 
-@pytest.mark.parametrize("scale", [20])
-def test_batch_matches_single_root_gpu(scale):
+@pytest.mark.parametrize("scale,sweep", [(20, 0.25), (20, 2.0)])
+def test_batch_matches_single_root_gpu(scale, sweep):
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
     with graph500_amd.G5Context(0) as c:
@@ -101,7 +110,8 @@ def test_batch_matches_single_root_gpu(scale):
         c.construct(2, release_edges=True)
         roots = c.find_roots(64)
         c.batch_prepare()
-        say("batch s%d" % scale)
+        c.batch_set_tuning(sweep_below=sweep)
+        say("batch s%d sweep %g" % (scale, sweep))
         c.run_bfs_batch(roots)
         teps = c.batch_teps_edges()
         for i, pr in enumerate(g["per_root"]):
@@ -112,3 +122,84 @@ def test_batch_matches_single_root_gpu(scale):
             c.run_bfs(int(roots[i]))
             assert np.array_equal(c.get_levels(), c.batch_levels(i)), i
             assert c.teps_edges() == teps[i]
+
+
+def test_batch_sweep_and_csr_push_small_graphs(oracle_mod):
+    """The edge-parallel sweep and the persistent CSR push on reorder modes
+    1 and 2, on graphs whose rows
+    are shorter than a chunk, longer than a chunk, and of degree one."""
+    import graph500_amd
+    O = oracle_mod
+    n = 5000
+    star = (np.concatenate([np.zeros(n - 1, np.int64), np.arange(n, n + 60)]),
+            np.concatenate([np.arange(1, n), np.arange(n + 1, n + 61)]))
+    rng = np.random.default_rng(3)
+    rnd = (rng.integers(0, 3000, 20000), rng.integers(0, 3000, 20000))
+    for v0, v1 in (star, rnd):
+        e = O.pack_edges(v0, v1)
+        oc = O.Csr(e)
+        roots = [r for r in range(0, 3000, 97) if oc.has_edge(r)][:20]
+        for mode in (1, 2):
+            with graph500_amd.G5Context(0) as c:
+                c.edges_append(e)
+                c.construct(mode)
+                c.batch_prepare()
+                for persistent, alpha in ((0, 1e12), (1, 1e12), (0, 1e-9), (1, 1e-9)):
+                    c.batch_set_tuning(alpha=alpha, sweep_below=2.0, persistent=persistent)
+                    say("sweep small mode", mode, len(e), "persistent", persistent, "alpha", alpha)
+                    c.run_bfs_batch(roots)
+                    teps = c.batch_teps_edges()
+                    for i, r in enumerate(roots):
+                        lv, _, tr = oc.bfs(int(r), want_pred=False)
+                        assert np.array_equal(c.batch_levels(i), lv), (mode, r, persistent)
+                        assert teps[i] == tr["teps_edges"]
+
+
+def test_batch_graph_replays_with_changing_roots(oracle_mod):
+    """The persistent kernel's hipGraph is replayed for successive batches
+    with different root sets (and eager single-root work in between); every
+    replay must equal the oracle."""
+    import graph500_amd
+    O = oracle_mod
+    e = O.generate_edges(14)
+    oc = O.Csr(e)
+    cand = oc.find_roots(64 + 40)
+    with graph500_amd.G5Context(0) as c:
+        c.edges_append(e)
+        c.construct(2)
+        c.prepare_bfs()
+        c.batch_prepare()
+        c.batch_set_tuning(persistent=1)
+        for k, rs in enumerate((cand[:64], cand[40:], cand[7:20], cand[:64])):
+            say("replay", k, len(rs))
+            c.run_bfs_batch(rs)
+            teps = c.batch_teps_edges()
+            for i, r in enumerate(rs):
+                lv, _, tr = oc.bfs(int(r), want_pred=False)
+                assert np.array_equal(c.batch_levels(i), lv), (k, i, r)
+                assert teps[i] == tr["teps_edges"], (k, i)
+            c.run_bfs(int(rs[0]))  # eager work between replays
+            assert np.array_equal(c.get_levels(), c.batch_levels(0))
+
+
+def test_batch_persistent_deeper_than_its_levels(oracle_mod):
+    """A path of 200 vertices needs ~200 levels: the persistent launch covers
+    persist_levels of them and the host loop the rest."""
+    import graph500_amd
+    O = oracle_mod
+    n = 200
+    e = O.pack_edges(np.arange(n - 1), np.arange(1, n))
+    oc = O.Csr(e)
+    roots = [0, 50, 199, 100]
+    with graph500_amd.G5Context(0) as c:
+        c.edges_append(e)
+        c.construct(2)
+        c.batch_prepare()
+        for pl in (4, 16):
+            c.batch_set_tuning(persistent=1, persist_levels=pl)
+            say("deep path persist_levels", pl)
+            c.run_bfs_batch(roots)
+            for i, r in enumerate(roots):
+                lv, _, _ = oc.bfs(int(r), want_pred=False)
+                assert np.array_equal(c.batch_levels(i), lv), (pl, r)
+            assert c.batch_stats()["num_levels"] > pl
