@@ -14,6 +14,9 @@ harmonic mean (mpi/logfile.h:74-82).  Generation and CSR construction run on
 the GPU before the timed region, as in the reference driver (mpi/main.cc:
 50-117, only run_bfs is timed).
 
+Before the timed steps the autotuner (graph500_amd/autotune.py) picks the
+batch's launch shapes and switch thresholds on this graph and these roots.
+
 `single_root` times the drop-in path, g5_run_bfs, once per root (after
 warm-up), with the host clock around each synchronous call like MPI_Wtime
 around run_bfs; GTEPS = 1 / mean(t_i / e_i) with e_i the validator's
@@ -138,9 +141,8 @@ def rocprof_summary(db, top=30):
         rows = cur.execute("select name, total_calls, total_duration, average, percentage "
                            "from top_kernels").fetchall()
         lines.append("%-48s %8s %12s %12s %7s" % ("kernel", "calls", "total ms", "avg us", "pct"))
-        for name, calls, tot, avg, pct in rows[:top]:
-            lines.append("%-48s %8d %12.3f %12.2f %7.2f" % (name.split("(")[0][:48], calls, tot / 1e6,
-                                                             avg / 1e3, pct))
+        for name, calls, tot, avg, pct in rows[:top]:  # the view's durations are in microseconds
+            lines.append("%-48s %8d %12.3f %12.2f %7.2f" % (name.split("(")[0][:48], calls, tot / 1e3, avg, pct))
     return "\n".join(lines)
 
 
@@ -213,6 +215,8 @@ def main():
     ap.add_argument("--cpu-scale", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single-root", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune-budget", type=float, default=30.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--rocprof-summary", metavar="DB")
     a = ap.parse_args()
@@ -245,6 +249,12 @@ def main():
         log("single root: %.1f GTEPS hmean" % single["gteps_hmean_wall"])
 
     ctx.batch_prepare()
+    tuned = None
+    if not a.no_autotune:
+        from graph500_amd import autotune
+        tuned = autotune.autotune(ctx, roots, reps=2, budget_s=a.autotune_budget, use_cache=False, log=log)
+        log("autotune: %s -> %.3f ms (%d trials, %.1f s)" % (tuned["tuning"], tuned["ms"], tuned["trials"],
+                                                             tuned["seconds"]))
     for _ in range(a.warmup):
         ctx.run_bfs_batch(roots)
     ctx.synchronize()
@@ -323,6 +333,7 @@ def main():
             "levels": "".join("UPS"[d] for d in st["direction"]),
             "level_ms": [round(x, 3) for x in st["level_ms"]],
             "tuning": ctx.batch_tuning(),
+            "autotune": ({"trials": tuned["trials"], "seconds": tuned["seconds"]} if tuned else None),
             "timed_region_s": round(t_all, 3),
         },
         "single_root": single,

@@ -101,13 +101,18 @@ struct MsState {
   uint64_t sweep_chunks = 0;
   DevBuf<uint32_t> sweep_rows;  // first/last row of every sweep chunk of the CSR
   // persistent level-fused path (one hipGraph replay per batch)
-  int persistent = 1;
+  int persistent = 0;
   int persist_levels = 16;     // frontier-word arrays preallocated for it
   int64_t persist_grid = 0;    // 0: every co-resident workgroup
+  int persist_waves = 8;       // register budget of the persistent kernel: 4, 6 or 8 waves per SIMD
+  int pull_filter = 0;         // pull tests the nonzero bitmap before a frontier word
+  double beta = 50.0;          // pull whenever fewer than 1/beta of the (root, vertex) pairs are unreached
   hipGraphExec_t graph_exec = nullptr;
   int graph_levels = 0;
   double graph_alpha = 0, graph_sweep = 0;
   int64_t graph_grid = 0;
+  int graph_waves = 0, graph_filter = -1;
+  double graph_beta = -1;
   unsigned p_grid = 0;
   int clock_khz = 100000;
   DevBuf<uint64_t*> p_fr;
@@ -115,6 +120,7 @@ struct MsState {
   DevBuf<int64_t> p_ts;
   DevBuf<int> p_dir;
   uint64_t* h_persist = nullptr;
+  alignas(16) unsigned char p_args_storage[256];  // kernel argument block of the persistent launch
   DevBuf<uint64_t> seen;     // n_rows
   DevBuf<uint64_t> nz[2];    // frontier-word-not-zero bitmaps, alternating by level
   std::vector<std::unique_ptr<DevBuf<uint64_t>>> fr;  // frontier words per level

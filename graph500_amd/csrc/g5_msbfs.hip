@@ -24,6 +24,7 @@
 #include "g5_frontier.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace g5 {
 
@@ -194,12 +195,20 @@ __global__ void __launch_bounds__(kBlock) k_ms_finish(uint64_t* __restrict__ see
 }
 
 // Pull step: one lane per vertex, the wave helps with long rows.
+// Frontier word of neighbour w: with the filter, read only when w's bit in
+// the nonzero bitmap is set (one more dependent load, far fewer 8-byte reads).
+__device__ __forceinline__ uint64_t front_of(const uint64_t* __restrict__ front,
+                                             const uint64_t* __restrict__ nz_front, bool filter, uint32_t w) {
+  if (filter && !bit_set(nz_front, w)) return 0;
+  return front[w];
+}
+
 __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
                                              const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
                                              uint64_t* __restrict__ nz_next, const uint32_t* __restrict__ first_nbr,
                                              const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols,
-                                             int64_t rows, uint64_t live_in, unsigned long long* __restrict__ c,
-                                             unsigned bid, unsigned nblk) {
+                                             int64_t rows, uint64_t live_in, bool filter,
+                                             unsigned long long* __restrict__ c, unsigned bid, unsigned nblk) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
   uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
@@ -215,7 +224,7 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
     if (need) {
       const uint32_t f = first_nbr[v];
       if (f != kMsNoNbr) {
-        if (bit_set(nz_front, f)) acc = front[f];
+        acc = front_of(front, nz_front, filter, f);
         if ((acc & need) != need) {
           rb = offsets[v] + 1;
           re = offsets[v + 1];
@@ -224,10 +233,14 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
       }
     }
     if (more && re - rb <= (uint64_t)kMsShortRow) {
-      for (uint64_t k = rb; k < re; ++k) {
-        const uint32_t w = cols[k];
-        if (!bit_set(nz_front, w)) continue;
-        acc |= front[w];
+      // four independent probes per step (entries past the row repeat the first)
+      for (uint64_t k = rb; k < re; k += 4) {
+        const uint32_t w0 = cols[k];
+        const uint32_t w1 = k + 1 < re ? cols[k + 1] : w0;
+        const uint32_t w2 = k + 2 < re ? cols[k + 2] : w0;
+        const uint32_t w3 = k + 3 < re ? cols[k + 3] : w0;
+        acc |= front_of(front, nz_front, filter, w0) | front_of(front, nz_front, filter, w1) |
+               front_of(front, nz_front, filter, w2) | front_of(front, nz_front, filter, w3);
         if ((acc & need) == need) break;
       }
       more = false;
@@ -239,13 +252,14 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
       const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
       const uint64_t nd = __shfl(need, src, 64);
       uint64_t a = __shfl(acc, src, 64);
-      for (uint64_t base = b; base < e; base += 64) {
+      // 128 entries per step, two independent probes per lane
+      for (uint64_t base = b; base < e; base += 128) {
         const uint64_t k = base + lane;
+        const uint32_t w0 = k < e ? cols[k] : 0;
+        const uint32_t w1 = k + 64 < e ? cols[k + 64] : 0;
         uint64_t x = 0;
-        if (k < e) {
-          const uint32_t w = cols[k];
-          if (bit_set(nz_front, w)) x = front[w];
-        }
+        if (k < e) x = front_of(front, nz_front, filter, w0);
+        if (k + 64 < e) x |= front_of(front, nz_front, filter, w1);
         a |= wave_or(x);
         if ((a & nd) == nd) break;
       }
@@ -274,9 +288,9 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
                                                     const uint32_t* __restrict__ first_nbr,
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ cols, int64_t rows,
-                                                    uint64_t live_in, unsigned long long* __restrict__ c) {
-  ms_pull_body(seen, front, nz_front, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x,
-               gridDim.x);
+                                                    uint64_t live_in, int filter, unsigned long long* __restrict__ c) {
+  ms_pull_body(seen, front, nz_front, next, nz_next, first_nbr, offsets, cols, rows, live_in, filter != 0, c,
+               blockIdx.x, gridDim.x);
 }
 
 // Dense pull step as an edge-parallel sweep over the whole CSR: each lane
@@ -290,7 +304,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
 // are those of modes 1 and 2 (every row has at least one entry), so at most
 // kSweepEpb rows overlap a chunk.  next then holds, per vertex, the lanes of
 // its neighbours' frontier words; k_ms_finish keeps the new ones.
-constexpr int kSweepEpt = 8;
+constexpr int kSweepEpt = 4;
 constexpr int kSweepEpb = kSweepEpt * kBlock;
 
 __device__ __forceinline__ void ms_sweep_clear_body(const uint32_t* __restrict__ chunk_rows, uint64_t nchunks,
@@ -304,9 +318,11 @@ __device__ __forceinline__ void ms_sweep_clear_body(const uint32_t* __restrict__
   }
 }
 
-// LDS of one sweep chunk: row starts (rows r0 .. r0+nr) and row accumulators.
+// LDS of one sweep chunk: row starts relative to the chunk's first entry e0
+// (clamped to -1 before it and kSweepEpb + 1 after it, which keeps every
+// comparison below exact) and row accumulators.
 struct SweepLds {
-  uint64_t off[kSweepEpb + 2];
+  int32_t off[kSweepEpb + 2];
   unsigned long long acc[kSweepEpb + 1];
 };
 
@@ -315,18 +331,21 @@ struct SweepLds {
 __device__ __forceinline__ int sweep_stage(uint64_t chunk, const uint32_t* __restrict__ chunk_rows,
                                            const uint64_t* __restrict__ offsets, SweepLds& L, bool clear_acc) {
   const uint32_t r0 = chunk_rows[2 * chunk];
+  const int64_t e0 = (int64_t)(chunk * kSweepEpb);
   int nr = (int)(chunk_rows[2 * chunk + 1] - r0 + 1);
   nr = nr > kSweepEpb ? kSweepEpb : nr;  // bound for LDS; rows are non-empty in modes 1 and 2
   for (int i = threadIdx.x; i <= nr; i += kBlock) {
-    L.off[i] = offsets[r0 + i];
+    int64_t d = (int64_t)offsets[r0 + i] - e0;
+    d = d < -1 ? -1 : (d > kSweepEpb + 1 ? kSweepEpb + 1 : d);
+    L.off[i] = (int32_t)d;
     if (clear_acc && i < nr) L.acc[i] = 0;
   }
   __syncthreads();
   return nr;
 }
 
-// Last staged row starting at or before edge e (LDS binary search).
-__device__ __forceinline__ int sweep_row_of(const SweepLds& L, int nr, uint64_t e) {
+// Last staged row starting at or before relative entry e (LDS binary search).
+__device__ __forceinline__ int sweep_row_of(const SweepLds& L, int nr, int e) {
   int lo = 0, hi = nr - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -345,13 +364,14 @@ __device__ __forceinline__ void ms_sweep_body(const uint32_t* __restrict__ chunk
     const uint64_t e1 = (e0 + kSweepEpb < nnz) ? e0 + kSweepEpb : nnz;
     const uint32_t r0 = chunk_rows[2 * chunk];
     const int nr = sweep_stage(chunk, chunk_rows, offsets, L, true);
-    const uint64_t my0 = e0 + (uint64_t)threadIdx.x * kSweepEpt;
-    if (my0 < e1) {
+    const int len = (int)(e1 - e0);
+    const int my0 = threadIdx.x * kSweepEpt;  // relative to e0
+    if (my0 < len) {
       int lo = sweep_row_of(L, nr, my0);
-      const uint64_t my1 = (my0 + kSweepEpt < e1) ? my0 + kSweepEpt : e1;
+      const int my1 = (my0 + kSweepEpt < len) ? my0 + kSweepEpt : len;
       uint32_t w[kSweepEpt];
 #pragma unroll
-      for (int j = 0; j < kSweepEpt; ++j) w[j] = (my0 + j < my1) ? cols[my0 + j] : 0;
+      for (int j = 0; j < kSweepEpt; ++j) w[j] = (my0 + j < my1) ? cols[e0 + my0 + j] : 0;
       uint64_t x[kSweepEpt];
 #pragma unroll
       for (int j = 0; j < kSweepEpt; ++j) x[j] = (my0 + j < my1 && bit_set(nz_front, w[j])) ? front[w[j]] : 0;
@@ -369,7 +389,7 @@ __device__ __forceinline__ void ms_sweep_body(const uint32_t* __restrict__ chunk
     __syncthreads();
     for (int i = threadIdx.x; i < nr; i += kBlock) {
       const uint64_t a = L.acc[i];
-      if (L.off[i] >= e0 && L.off[i + 1] <= e1)
+      if (L.off[i] >= 0 && L.off[i + 1] <= len)
         next[r0 + i] = a;
       else if (a)
         atomicOr((unsigned long long*)(next + r0 + i), (unsigned long long)a);
@@ -421,20 +441,21 @@ __device__ __forceinline__ void ms_push_csr_body(const uint32_t* __restrict__ ch
     const uint64_t e0 = chunk * kSweepEpb;
     const uint64_t e1 = (e0 + kSweepEpb < nnz) ? e0 + kSweepEpb : nnz;
     const int nr = sweep_stage(chunk, chunk_rows, offsets, L, false);
-    const uint64_t my0 = e0 + (uint64_t)threadIdx.x * kSweepEpt;
-    if (my0 < e1) {
+    const int len = (int)(e1 - e0);
+    const int my0 = threadIdx.x * kSweepEpt;  // relative to e0
+    if (my0 < len) {
       int lo = sweep_row_of(L, nr, my0);
-      const uint64_t my1 = (my0 + kSweepEpt < e1) ? my0 + kSweepEpt : e1;
+      const int my1 = (my0 + kSweepEpt < len) ? my0 + kSweepEpt : len;
       uint32_t u = r0 + lo;
       uint64_t fu = bit_set(nz_front, u) ? front[u] : 0;
-      for (uint64_t e = my0; e < my1; ++e) {
+      for (int e = my0; e < my1; ++e) {
         if (lo + 1 < nr && L.off[lo + 1] <= e) {
           while (lo + 1 < nr && L.off[lo + 1] <= e) ++lo;
           u = r0 + lo;
           fu = bit_set(nz_front, u) ? front[u] : 0;
         }
         if (!fu) continue;
-        const uint32_t w = cols[e];
+        const uint32_t w = cols[e0 + e];
         const uint64_t m = fu & ~seen[w];
         if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
           atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
@@ -512,11 +533,16 @@ struct MsPersistArgs {
   GridBarrier* bar;
   double push_limit;            // nnz / alpha
   double sweep_below;           // sweep while reached pairs < sweep_below x roots x rows
+  int pull_filter;              // pull reads a frontier word only after its nonzero bit
+  double pull_unseen_frac;      // pull whenever fewer than this fraction of pairs are unreached
   int max_levels;
   long long timeout_ticks;      // per-barrier wait bound (wall clock ticks)
 };
 
-__global__ void __launch_bounds__(kBlock) k_ms_persistent(MsPersistArgs a) {
+// kWaves: waves per SIMD the compiler must fit (register budget 512 / kWaves).
+template <int kWaves>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, kWaves)))
+k_ms_persistent(MsPersistArgs a) {
   __shared__ SweepLds L;
   const unsigned bid = blockIdx.x, nblk = gridDim.x;
   const int64_t words = (a.rows + 63) / 64;
@@ -524,6 +550,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_persistent(MsPersistArgs a) {
   uint64_t live = __hip_atomic_load(&a.cnt[M_LIVE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const double roots_live = (double)__popcll(live);
   const double sweep_limit = a.sweep_below * roots_live * (double)a.rows;
+  const double total_pairs = roots_live * (double)a.rows;
   double reached = roots_live;
   int l = 0;
   while (live && l + 1 < a.max_levels) {
@@ -535,7 +562,9 @@ __global__ void __launch_bounds__(kBlock) k_ms_persistent(MsPersistArgs a) {
     uint64_t* next = a.fr[l];
     const uint64_t* nz_front = ((l - 1) & 1) ? a.nz1 : a.nz0;
     uint64_t* nz_next = (l & 1) ? a.nz1 : a.nz0;
-    const bool push = (double)fdeg < a.push_limit;
+    // push a small frontier union, unless nearly every pair is reached (then
+    // a pull touches little beyond the seen words)
+    const bool push = (double)fdeg < a.push_limit && total_pairs - reached >= a.pull_unseen_frac * total_pairs;
     const bool sweep = !push && a.sweep_chunks && reached < sweep_limit;
     const long long deadline = wall_clock64() + a.timeout_ticks;
     if (push) {
@@ -551,8 +580,8 @@ __global__ void __launch_bounds__(kBlock) k_ms_persistent(MsPersistArgs a) {
       if (!grid_barrier(a.bar, nblk, deadline)) return;
       ms_finish_body(a.seen, next, nz_next, a.offsets, a.rows, c, bid, nblk);
     } else {
-      ms_pull_body(a.seen, front, nz_front, next, nz_next, a.first_nbr, a.offsets, a.cols, a.rows, live, c, bid,
-                   nblk);
+      ms_pull_body(a.seen, front, nz_front, next, nz_next, a.first_nbr, a.offsets, a.cols, a.rows, live,
+                   a.pull_filter != 0, c, bid, nblk);
     }
     if (!grid_barrier(a.bar, nblk, deadline)) return;
     live = __hip_atomic_load(&c[M_LIVE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -708,6 +737,8 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
   g5_batch_stats& st = m.stats;
   const double push_limit = (double)g.nnz / m.alpha;
   const double sweep_limit = m.sweep_below * roots_live * (double)rows;
+  const double total_pairs = roots_live * (double)rows;
+  const double unseen_frac = m.beta > 0 ? 1.0 / m.beta : 0.0;
   int l = l0 - 1;
   while (live) {
     ++l;
@@ -716,7 +747,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     const uint64_t* front = m.fr[l - 1]->p;  // after ms_level: the vector may have grown
     const uint64_t* nz_front = m.nz[(l - 1) & 1].p;
     uint64_t* nz_next = m.nz[l & 1].p;
-    const bool push = (double)fdeg < push_limit;
+    const bool push = (double)fdeg < push_limit && total_pairs - reached >= unseen_frac * total_pairs;
     const bool sweep = !push && m.sweep_chunks && reached < sweep_limit;
     G5_HIP(hipMemsetAsync(m.counters.p, 0, M_NUM * 8, s));
     G5_HIP(hipEventRecord(m.lev_begin[l], s));
@@ -747,7 +778,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     } else {
       hipLaunchKernelGGL(k_ms_pull, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
                          m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows,
-                         live, cnt);
+                         live, m.pull_filter, cnt);
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(m.lev_end[l], s));
@@ -776,7 +807,8 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   const int64_t words = (rows + 63) / 64;
   const int K = m.persist_levels;
   if (m.graph_exec && m.graph_levels == K && m.graph_alpha == m.alpha && m.graph_sweep == m.sweep_below &&
-      m.graph_grid == m.persist_grid)
+      m.graph_grid == m.persist_grid && m.graph_waves == m.persist_waves && m.graph_filter == m.pull_filter &&
+      m.graph_beta == m.beta)
     return;
   if (m.graph_exec) {
     (void)hipGraphExecDestroy(m.graph_exec);
@@ -798,7 +830,13 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   }
   // co-resident grid: occupancy x compute units, capped by the tuning
   int per_cu = 0, cus = 0;
-  G5_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ms_persistent, kBlock, 0));
+  const void* kern = nullptr;
+  switch (m.persist_waves) {
+    case 4: kern = (const void*)k_ms_persistent<4>; break;
+    case 6: kern = (const void*)k_ms_persistent<6>; break;
+    default: kern = (const void*)k_ms_persistent<8>; break;
+  }
+  G5_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
   G5_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
   G5_CHECK(per_cu >= 1, "g5_run_bfs_batch: persistent kernel does not fit a compute unit");
   int64_t grid = (int64_t)per_cu * cus;
@@ -827,9 +865,14 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   a.bar = (GridBarrier*)(m.p_misc.p + 1);
   a.push_limit = (double)g.nnz / m.alpha;
   a.sweep_below = m.sweep_below;
+  a.pull_filter = m.pull_filter;
+  a.pull_unseen_frac = m.beta > 0 ? 1.0 / m.beta : 0.0;
   a.max_levels = K;
-  a.timeout_ticks = (long long)m.clock_khz * 1000LL * 10;  // 10 s per barrier wait
+  a.timeout_ticks = (long long)m.clock_khz * 1000LL * 5;  // 5 s per barrier wait
 
+  // The graph: clears and root placement captured from the stream, then the
+  // persistent kernel as a cooperative kernel node (the runtime checks that
+  // the whole grid is co-resident, which the grid barrier relies on).
   hipGraph_t graph = nullptr;
   G5_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(k_ms_fill, dim3(grid_of((int64_t)K * M_NUM, kBlock, 64)), dim3(kBlock), 0, s, m.p_cnt.p,
@@ -840,11 +883,37 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   hipLaunchKernelGGL(k_ms_roots, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, 64, rows, g.offsets.p, m.seen.p,
                      lev[0], (unsigned long long*)m.p_cnt.p);
   hipLaunchKernelGGL(k_ms_roots_nz, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, 64, rows, m.nz[0].p);
-  hipLaunchKernelGGL(k_ms_persistent, dim3(m.p_grid), dim3(kBlock), 0, s, a);
   const hipError_t le = hipGetLastError();
   const hipError_t ce = hipStreamEndCapture(s, &graph);
   G5_HIP(le);
   G5_HIP(ce);
+  static_assert(sizeof(MsPersistArgs) <= sizeof(m.p_args_storage), "argument block too large");
+  std::memcpy(m.p_args_storage, &a, sizeof(a));
+  size_t nnodes = 0;
+  hipError_t ge = hipGraphGetNodes(graph, nullptr, &nnodes);
+  std::vector<hipGraphNode_t> deps(nnodes);
+  if (ge == hipSuccess && nnodes) ge = hipGraphGetNodes(graph, deps.data(), &nnodes);
+  hipGraphNode_t knode = nullptr;
+  if (ge == hipSuccess) {
+    hipKernelNodeParams kp{};
+    kp.func = (void*)kern;
+    kp.gridDim = dim3(m.p_grid);
+    kp.blockDim = dim3(kBlock);
+    kp.sharedMemBytes = 0;
+    void* kargs[] = {(void*)m.p_args_storage};
+    kp.kernelParams = kargs;
+    kp.extra = nullptr;
+    ge = hipGraphAddKernelNode(&knode, graph, deps.data(), nnodes, &kp);
+  }
+  if (ge == hipSuccess) {
+    hipLaunchAttributeValue v{};
+    v.cooperative = 1;
+    ge = hipGraphKernelNodeSetAttribute(knode, hipLaunchAttributeCooperative, &v);
+  }
+  if (ge != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    G5_HIP(ge);
+  }
   const hipError_t ie = hipGraphInstantiate(&m.graph_exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   G5_HIP(ie);
@@ -852,6 +921,9 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   m.graph_alpha = m.alpha;
   m.graph_sweep = m.sweep_below;
   m.graph_grid = m.persist_grid;
+  m.graph_waves = m.persist_waves;
+  m.graph_filter = m.pull_filter;
+  m.graph_beta = m.beta;
 }
 
 // One replay of the batch graph; levels beyond the preallocated arrays go on
@@ -891,6 +963,8 @@ static void ms_run_persistent(g5_ctx* ctx, const int64_t* roots, int nroots) {
     reached += (double)st.pairs[l];
   }
   st.num_levels = done + 1;
+  st.persistent_grid = (int32_t)m.p_grid;
+  st.persistent_levels = done;
   const uint64_t* last = h_cnt + (size_t)done * M_NUM;
   if (last[M_LIVE]) {
     // out of preallocated levels: continue level by level from the host

@@ -88,13 +88,16 @@ class BatchStats(ctypes.Structure):
         ("pairs", ctypes.c_int64 * MAX_LEVELS),
         ("level_ms", ctypes.c_double * MAX_LEVELS),
         ("device_ms", ctypes.c_double),
+        ("persistent_grid", ctypes.c_int32),
+        ("persistent_levels", ctypes.c_int32),
     ]
 
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("alpha", ctypes.c_double), ("pull_grid", ctypes.c_int64), ("sweep_below", ctypes.c_double),
                 ("sweep_grid", ctypes.c_int64), ("persistent", ctypes.c_int32), ("persist_levels", ctypes.c_int32),
-                ("persist_grid", ctypes.c_int64)]
+                ("persist_grid", ctypes.c_int64), ("persist_waves", ctypes.c_int32), ("pull_filter", ctypes.c_int32),
+                ("beta", ctypes.c_double)]
 
 
 _lib = None
@@ -330,7 +333,7 @@ class G5Context:
         self._check(self._lib.g5_batch_prepare(self._h))
 
     TUNING_FIELDS = ("alpha", "pull_grid", "sweep_below", "sweep_grid", "persistent", "persist_levels",
-                     "persist_grid")
+                     "persist_grid", "persist_waves", "pull_filter", "beta")
 
     def batch_set_tuning(self, **kw):
         """Sets any of TUNING_FIELDS; the others keep their values."""
@@ -376,6 +379,8 @@ class G5Context:
             "pairs": [st.pairs[i] for i in range(1, L)],
             "level_ms": [st.level_ms[i] for i in range(1, L)],
             "device_ms": st.device_ms,
+            "persistent_grid": st.persistent_grid,
+            "persistent_levels": st.persistent_levels,
         }
 
     def batch_model_bytes(self):
@@ -387,6 +392,10 @@ class G5Context:
 
     def batch_end(self):
         self._check(self._lib.g5_batch_end(self._h))
+
+    def version(self):
+        """g5_version(): library build and GPU architecture."""
+        return self._lib.g5_version().decode()
 
     def synchronize(self):
         self._check(self._lib.g5_synchronize(self._h))

@@ -77,6 +77,8 @@ typedef struct g5_batch_stats {
   int64_t pairs[G5_MAX_LEVELS];     /* (root, vertex) pairs discovered at that level */
   double level_ms[G5_MAX_LEVELS];   /* device time per level (HIP events) */
   double device_ms;                 /* device time of the whole batch */
+  int32_t persistent_grid;          /* workgroups of the persistent launch (0: per-level launches) */
+  int32_t persistent_levels;        /* levels run inside the persistent launch */
 } g5_batch_stats;
 
 /* Tuning of the batched BFS (set by the autotuner, graph500_amd/autotune.py). */
@@ -92,6 +94,11 @@ typedef struct g5_batch_tuning {
   int32_t persist_levels;  /* levels the persistent launch covers (frontier arrays kept in HBM);
                               deeper searches continue level by level */
   int64_t persist_grid;    /* workgroups of the persistent launch; 0 = all co-resident ones */
+  int32_t persist_waves;   /* register budget of the persistent kernel: 4, 6 or 8 waves per SIMD */
+  int32_t pull_filter;     /* 1: the pull reads a neighbour's frontier word only when its bit in a
+                              1-bit-per-vertex "frontier word is not zero" map is set */
+  double beta;             /* pull (never push) once fewer than 1/beta of the (root, vertex) pairs
+                              are unreached; 0 disables */
 } g5_batch_tuning;
 
 /* ---- context ---- */

@@ -1,0 +1,70 @@
+"""CPU checks of the autotuner's search and cache with a stand-in context
+(no GPU): it keeps the fastest configuration, refuses a configuration that
+changes the traversed-edge counts, and reuses its cache."""
+import numpy as np
+import pytest
+
+from graph500_amd import autotune
+
+
+class FakeCtx:
+    """Batch time is a fixed function of the tuning; optionally one knob
+    value corrupts the edge counts."""
+
+    def __init__(self, bad=None):
+        self.t = {"alpha": 16.0, "pull_grid": 8192, "sweep_below": 0.25, "sweep_grid": 4096, "persistent": 0,
+                  "persist_levels": 16, "persist_grid": 0, "persist_waves": 8, "pull_filter": 0, "beta": 50.0}
+        self.bad = bad
+        self.runs = 0
+
+    def graph_info(self):
+        return {"scale_estimate": 10, "num_active": 900, "nnz": 30000, "reorder_mode": 2}
+
+    def version(self):
+        return "fake gfx950"
+
+    def batch_tuning(self):
+        return dict(self.t)
+
+    def batch_set_tuning(self, **kw):
+        self.t.update(kw)
+
+    def run_bfs_batch(self, roots):
+        self.runs += 1
+
+    def batch_stats(self):
+        t = self.t
+        ms = 10.0 + abs(t["alpha"] - 32.0) / 8 + (0 if t["beta"] == 20.0 else 1) + t["pull_filter"] \
+            + (2 if t["persistent"] else 0) + (0 if t["pull_grid"] == 16384 else 0.5)
+        return {"device_ms": ms}
+
+    def batch_teps_edges(self):
+        e = np.arange(4, dtype=np.int64)
+        if self.bad and self.t.get(self.bad[0]) == self.bad[1]:
+            e[0] += 1
+        return e
+
+
+def test_descent_finds_the_best_per_knob(tmp_path):
+    c = FakeCtx()
+    res = autotune.autotune(c, [1, 2], reps=1, cache=str(tmp_path / "t.json"))
+    assert res["tuning"]["alpha"] == 32.0
+    assert res["tuning"]["beta"] == 20.0
+    assert res["tuning"]["pull_filter"] == 0
+    assert res["tuning"]["persistent"] == 0
+    assert res["tuning"]["pull_grid"] == 16384
+    assert c.batch_tuning() == res["tuning"]
+
+
+def test_cache_is_reused(tmp_path):
+    path = str(tmp_path / "t.json")
+    first = autotune.autotune(FakeCtx(), [1], reps=1, cache=path)
+    c = FakeCtx()
+    again = autotune.autotune(c, [1], cache=path)
+    assert again["cached"] and again["tuning"] == first["tuning"]
+    assert c.runs == 0 and c.batch_tuning() == first["tuning"]
+
+
+def test_changed_results_are_refused(tmp_path):
+    with pytest.raises(RuntimeError):
+        autotune.autotune(FakeCtx(bad=("alpha", 8.0)), [1], reps=1, cache=str(tmp_path / "t.json"))

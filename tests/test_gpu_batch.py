@@ -42,12 +42,13 @@ def s16ctx():
 # pull only (edge-parallel sweep); push only; each through the persistent
 # graph-replayed kernel and through per-level launches
 @pytest.mark.parametrize("persistent", [1, 0])
-@pytest.mark.parametrize("alpha,sweep", [(16.0, 0.25), (1e12, 0.0), (1e12, 2.0), (1e-9, 0.25)])
-def test_batch_64_roots_match_reference_s16(s16ctx, alpha, sweep, persistent):
+@pytest.mark.parametrize("alpha,sweep,filt,beta", [(16.0, 0.25, 0, 50.0), (16.0, 0.25, 1, 0.0), (1e12, 0.0, 0, 0.0),
+                                                   (1e12, 0.0, 1, 0.0), (1e12, 2.0, 0, 0.0), (1e-9, 0.25, 0, 0.0)])
+def test_batch_64_roots_match_reference_s16(s16ctx, alpha, sweep, filt, beta, persistent):
     g = gold("ref_s16_64roots.json")
     roots = [p["root"] for p in g["per_root"]]
-    s16ctx.batch_set_tuning(alpha=alpha, sweep_below=sweep, persistent=persistent)
-    say("batch s16 alpha", alpha, "sweep", sweep, "persistent", persistent)
+    s16ctx.batch_set_tuning(alpha=alpha, sweep_below=sweep, persistent=persistent, pull_filter=filt, beta=beta)
+    say("batch s16 alpha", alpha, "sweep", sweep, "filter", filt, "beta", beta, "persistent", persistent)
     s16ctx.run_bfs_batch(roots)
     st = s16ctx.batch_stats()
     if alpha > 1e6:
@@ -60,7 +61,7 @@ def test_batch_64_roots_match_reference_s16(s16ctx, alpha, sweep, persistent):
         assert teps[i] == pr["nedge"]
     st = s16ctx.batch_stats()
     assert sum(st["pairs"]) + 64 == sum(sum(p["level_histogram"]) for p in g["per_root"])
-    s16ctx.batch_set_tuning(alpha=16.0, sweep_below=0.25, persistent=1)
+    s16ctx.batch_set_tuning(alpha=16.0, sweep_below=0.25, persistent=0, pull_filter=0, beta=50.0)
 
 
 def test_batch_model_bytes_match_oracle(s16ctx, oracle_mod):
@@ -145,7 +146,7 @@ def test_batch_sweep_and_csr_push_small_graphs(oracle_mod):
                 c.construct(mode)
                 c.batch_prepare()
                 for persistent, alpha in ((0, 1e12), (1, 1e12), (0, 1e-9), (1, 1e-9)):
-                    c.batch_set_tuning(alpha=alpha, sweep_below=2.0, persistent=persistent)
+                    c.batch_set_tuning(alpha=alpha, sweep_below=2.0, persistent=persistent, beta=0.0)
                     say("sweep small mode", mode, len(e), "persistent", persistent, "alpha", alpha)
                     c.run_bfs_batch(roots)
                     teps = c.batch_teps_edges()
@@ -203,3 +204,29 @@ def test_batch_persistent_deeper_than_its_levels(oracle_mod):
                 lv, _, _ = oc.bfs(int(r), want_pred=False)
                 assert np.array_equal(c.batch_levels(i), lv), (pl, r)
             assert c.batch_stats()["num_levels"] > pl
+
+
+def test_autotune_keeps_results(oracle_mod, tmp_path):
+    """The autotuner's pick (measured on the GPU) still gives the oracle's
+    levels; a second call reuses the cached pick."""
+    import graph500_amd
+    from graph500_amd import autotune
+    O = oracle_mod
+    e = O.generate_edges(14)
+    oc = O.Csr(e)
+    roots = oc.find_roots(64)
+    cache = str(tmp_path / "tuning.json")
+    with graph500_amd.G5Context(0) as c:
+        c.edges_append(e)
+        c.construct(2)
+        c.batch_prepare()
+        say("autotune s14")
+        res = autotune.autotune(c, roots, reps=1, budget_s=20, cache=cache)
+        assert not res["cached"] and res["trials"] > 5
+        assert c.batch_tuning() == res["tuning"]
+        c.run_bfs_batch(roots)
+        for i in range(0, 64, 9):
+            lv, _, _ = oc.bfs(int(roots[i]), want_pred=False)
+            assert np.array_equal(c.batch_levels(i), lv), i
+        again = autotune.autotune(c, roots, cache=cache)
+        assert again["cached"] and again["tuning"] == res["tuning"]
