@@ -33,6 +33,11 @@ and workload is present under profiles/, else null.  `cpu_baseline` times
 the reference CPU runnable (oracle/_ref/runnable) on a bounded sample on this
 host, or the oracle port when that cannot run.
 
+`serve` runs the serving loop (g5_server_*) over resident graphs of SCALE
+18, 22, 25 and 27 at swept Poisson rates: p50/p99 latency from arrival to a
+complete answer, sustained GTEPS, and a sample of answers checked against
+single-root runs.
+
 Other modes:
   --copy-calibration      a plain 1 GiB device copy (for scaling rocprof
                           FETCH_SIZE/WRITE_SIZE counters)
@@ -204,6 +209,61 @@ def single_root_leg(ctx, roots, warmup):
     }
 
 
+def serve_leg(scales, rates, max_queries, edgefactor, seed=1):
+    """Serving loop over resident graphs: Poisson arrivals at each rate, each
+    query on graph s with probability proportional to 2^-((27 - s) / 2) and a
+    uniform random root with edges.  Per rate: latency p50/p99 (arrival to
+    complete answer), sustained GTEPS (traversed edges / wall), and a sample
+    of kept answers compared with single-root runs of the same roots."""
+    import numpy as np
+    import graph500_amd
+    ctxs = []
+    for sc in scales:
+        c = graph500_amd.G5Context(0)
+        c.generate_kronecker(sc, edgefactor, 2, 3)
+        c.construct(2, release_edges=True)
+        c.prepare_bfs()
+        ctxs.append(c)
+        log("serve graph SCALE %d: n_active %d" % (sc, c.graph_info()["num_active"]))
+    w = np.array([2.0 ** (-(27 - sc) / 2.0) for sc in scales])
+    w /= w.sum()
+    rng = np.random.default_rng(seed)
+    has = []
+    for c in ctxs:
+        n = c.pred_size()
+        cand = rng.integers(0, n, size=min(n, 200000))
+        has.append(np.array([v for v in cand[:20000] if c.has_edge(int(v))], np.int64))
+    runs = []
+    with graph500_amd.G5Server(ctxs, answer_buffers=128) as srv:
+        for rate in rates:
+            nq = int(min(max_queries, max(64, rate * 1.0)))
+            q = np.zeros(nq, graph500_amd.QUERY_DTYPE)
+            q["graph"] = rng.choice(len(scales), size=nq, p=w)
+            q["root"] = [has[g][rng.integers(0, len(has[g]))] for g in q["graph"]]
+            q["arrival_s"] = np.cumsum(rng.exponential(1.0 / rate, size=nq))
+            check = rng.choice(nq, size=min(4, nq), replace=False)
+            q["keep"][check] = 1
+            res, rep = srv.run(q, timeout_s=300)
+            lat = (res["done_s"] - q["arrival_s"]) * 1e3
+            ok = True
+            for i in check:
+                g, r = int(q["graph"][i]), int(q["root"][i])
+                lv = srv.levels(res["request"][i], ctxs[g].pred_size())
+                srv.release(res["request"][i])
+                ctxs[g].run_bfs(r)
+                ok &= bool(np.array_equal(lv, ctxs[g].get_levels())) and int(res["edges"][i]) == ctxs[g].teps_edges()
+            run = {"rate_qps": rate, "queries": nq, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                   "served_qps": round(nq / rep["wall_s"], 1),
+                   "gteps": round(rep["edges"] / rep["wall_s"] / 1e9, 2),
+                   "max_lanes": rep["max_active_lanes"], "sample_bit_exact": ok}
+            log("serve %s" % run)
+            runs.append(run)
+    for c in ctxs:
+        c.close()
+    return {"graphs_scale": list(scales), "answer_buffers_per_graph": 128, "runs": runs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +276,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single-root", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-scales", type=lambda x: [int(v) for v in x.split(",")], default=[18, 22, 25, 27])
+    ap.add_argument("--serve-rates", type=lambda x: [float(v) for v in x.split(",")],
+                    default=[100.0, 1000.0, 5000.0, 20000.0])
+    ap.add_argument("--serve-max-queries", type=int, default=3000)
     ap.add_argument("--autotune-budget", type=float, default=30.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--rocprof-summary", metavar="DB")
@@ -344,6 +409,8 @@ def main():
     }
     ctx.batch_end()
     ctx.close()
+    if not a.no_serve:
+        out["serve"] = serve_leg(a.serve_scales, a.serve_rates, a.serve_max_queries, a.edgefactor)
     if not a.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_threads, a.cpu_scale)

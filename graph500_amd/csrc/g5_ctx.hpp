@@ -100,6 +100,7 @@ struct MsState {
   int64_t sweep_grid = 4096; // workgroups of the edge-parallel sweep
   uint64_t sweep_chunks = 0;
   DevBuf<uint32_t> sweep_rows;  // first/last row of every sweep chunk of the CSR
+  DevBuf<uint32_t> chunk_list;  // sweep chunks holding a frontier row
   // persistent level-fused path (one hipGraph replay per batch)
   int persistent = 0;
   int persist_levels = 16;     // frontier-word arrays preallocated for it
@@ -107,6 +108,8 @@ struct MsState {
   int persist_waves = 8;       // register budget of the persistent kernel: 4, 6 or 8 waves per SIMD
   int pull_filter = 0;         // pull tests the nonzero bitmap before a frontier word
   double beta = 50.0;          // pull whenever fewer than 1/beta of the (root, vertex) pairs are unreached
+  double compact_below = 0.0;  // pull from a compact frontier when it spans fewer than this x rows vertices
+  DevBuf<uint64_t> pack, compact;
   hipGraphExec_t graph_exec = nullptr;
   int graph_levels = 0;
   double graph_alpha = 0, graph_sweep = 0;
@@ -148,6 +151,9 @@ struct MsState {
     roots.release();
     sweep_rows.release();
     sweep_chunks = 0;
+    chunk_list.release();
+    pack.release();
+    compact.release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
     graph_levels = 0;

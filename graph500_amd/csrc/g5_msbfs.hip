@@ -22,54 +22,12 @@
 // Per-lane levels equal the single-root BFS levels of the same root.
 #include "g5_ctx.hpp"
 #include "g5_frontier.hpp"
+#include "g5_ms_device.hpp"
 
 #include <algorithm>
 #include <cstring>
 
 namespace g5 {
-
-constexpr uint32_t kMsNoNbr = 0xFFFFFFFFu;
-constexpr int kMsShortRow = 32;
-
-enum { M_LIVE = 0, M_PAIRS = 1, M_NZ = 2, M_FDEG = 3, M_NUM = 8 };
-
-__device__ __forceinline__ uint64_t wave_or(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Block-wide OR / sums of the level counters, one atomic each per workgroup.
-__device__ __forceinline__ void ms_block_counters(uint64_t live, uint64_t pairs, uint64_t nz, uint64_t fdeg,
-                                                  unsigned long long* c) {
-  __shared__ uint64_t s_v[4][kBlock / 64];
-  live = wave_or(live);
-  pairs = wave_sum(pairs);
-  nz = wave_sum(nz);
-  fdeg = wave_sum(fdeg);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-    s_v[0][wave] = live;
-    s_v[1][wave] = pairs;
-    s_v[2][wave] = nz;
-    s_v[3][wave] = fdeg;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t a = 0, b = 0, d = 0, e = 0;
-    for (int i = 0; i < kBlock / 64; ++i) {
-      a |= s_v[0][i];
-      b += s_v[1][i];
-      d += s_v[2][i];
-      e += s_v[3][i];
-    }
-    if (a) atomicOr(c + M_LIVE, (unsigned long long)a);
-    if (b) atomicAdd(c + M_PAIRS, (unsigned long long)b);
-    if (d) atomicAdd(c + M_NZ, (unsigned long long)d);
-    if (e) atomicAdd(c + M_FDEG, (unsigned long long)e);
-  }
-  __syncthreads();
-}
 
 __global__ void __launch_bounds__(kBlock) k_ms_clear(uint64_t* __restrict__ a, uint64_t* __restrict__ b,
                                                      int64_t n) {
@@ -115,24 +73,6 @@ __global__ void k_ms_roots_nz(const uint32_t* __restrict__ old2new, const int64_
   }
 }
 
-// Vertices of the frontier union (set bits of the nonzero bitmap) -> queue,
-// ascending, through a scan of per-word popcounts.
-struct UnionCount {
-  const uint64_t* nz;
-  __device__ uint64_t operator()(size_t w) const { return (uint64_t)__popcll(nz[w]); }
-};
-struct UnionEmit {
-  const uint64_t* nz;
-  uint32_t* queue;
-  __device__ void operator()(size_t w, uint64_t pre) const {
-    uint64_t x = nz[w];
-    while (x) {
-      queue[pre++] = (uint32_t)(w * 64 + __builtin_ctzll(x));
-      x &= x - 1;
-    }
-  }
-};
-
 __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__ queue, int64_t nf,
                                                     const uint64_t* __restrict__ prefix,
                                                     const uint32_t* __restrict__ chunk_rows,
@@ -155,131 +95,11 @@ __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__
   }
 }
 
-// After a push or a sweep: next keeps only lanes new to v, seen gains them,
-// and nz_next gets one bit per vertex whose frontier word is not zero.
-// Blocks bid of nblk take 64-vertex words in a grid-stride loop.
-__device__ __forceinline__ void ms_finish_body(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
-                                               uint64_t* __restrict__ nz_next, const uint64_t* __restrict__ offsets,
-                                               int64_t rows, unsigned long long* __restrict__ c, unsigned bid,
-                                               unsigned nblk) {
-  uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
-  const int64_t words = (rows + 63) / 64;
-  const int lane = threadIdx.x & 63;
-  const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
-  for (int64_t wd = (int64_t)bid * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
-    const int64_t v = wd * 64 + lane;
-    uint64_t nw = 0;
-    if (v < rows) {
-      const uint64_t s = seen[v];
-      nw = next[v] & ~s;
-      next[v] = nw;
-      if (nw) {
-        seen[v] = s | nw;
-        live |= nw;
-        pairs += __popcll(nw);
-        ++nz;
-        fdeg += offsets[v + 1] - offsets[v];
-      }
-    }
-    const uint64_t b = __ballot(nw != 0);
-    if (lane == 0) nz_next[wd] = b;
-  }
-  ms_block_counters(live, pairs, nz, fdeg, c);
-}
-
 __global__ void __launch_bounds__(kBlock) k_ms_finish(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
                                                       uint64_t* __restrict__ nz_next,
                                                       const uint64_t* __restrict__ offsets, int64_t rows,
                                                       unsigned long long* __restrict__ c) {
   ms_finish_body(seen, next, nz_next, offsets, rows, c, blockIdx.x, gridDim.x);
-}
-
-// Pull step: one lane per vertex, the wave helps with long rows.
-// Frontier word of neighbour w: with the filter, read only when w's bit in
-// the nonzero bitmap is set (one more dependent load, far fewer 8-byte reads).
-__device__ __forceinline__ uint64_t front_of(const uint64_t* __restrict__ front,
-                                             const uint64_t* __restrict__ nz_front, bool filter, uint32_t w) {
-  if (filter && !bit_set(nz_front, w)) return 0;
-  return front[w];
-}
-
-__device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
-                                             const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
-                                             uint64_t* __restrict__ nz_next, const uint32_t* __restrict__ first_nbr,
-                                             const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols,
-                                             int64_t rows, uint64_t live_in, bool filter,
-                                             unsigned long long* __restrict__ c, unsigned bid, unsigned nblk) {
-  const int lane = threadIdx.x & 63;
-  const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
-  uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
-  const int64_t words = (rows + 63) / 64;
-  for (int64_t wd = (int64_t)bid * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
-    const int64_t v = wd * 64 + lane;
-    const bool inb = v < rows;
-    const uint64_t s = inb ? seen[v] : ~UINT64_C(0);
-    const uint64_t need = ~s & live_in;
-    uint64_t acc = 0;
-    bool more = false;
-    uint64_t rb = 0, re = 0;
-    if (need) {
-      const uint32_t f = first_nbr[v];
-      if (f != kMsNoNbr) {
-        acc = front_of(front, nz_front, filter, f);
-        if ((acc & need) != need) {
-          rb = offsets[v] + 1;
-          re = offsets[v + 1];
-          more = rb < re;
-        }
-      }
-    }
-    if (more && re - rb <= (uint64_t)kMsShortRow) {
-      // four independent probes per step (entries past the row repeat the first)
-      for (uint64_t k = rb; k < re; k += 4) {
-        const uint32_t w0 = cols[k];
-        const uint32_t w1 = k + 1 < re ? cols[k + 1] : w0;
-        const uint32_t w2 = k + 2 < re ? cols[k + 2] : w0;
-        const uint32_t w3 = k + 3 < re ? cols[k + 3] : w0;
-        acc |= front_of(front, nz_front, filter, w0) | front_of(front, nz_front, filter, w1) |
-               front_of(front, nz_front, filter, w2) | front_of(front, nz_front, filter, w3);
-        if ((acc & need) == need) break;
-      }
-      more = false;
-    }
-    uint64_t lm = __ballot(more);
-    while (lm) {
-      const int src = __ffsll((unsigned long long)lm) - 1;
-      lm &= lm - 1;
-      const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
-      const uint64_t nd = __shfl(need, src, 64);
-      uint64_t a = __shfl(acc, src, 64);
-      // 128 entries per step, two independent probes per lane
-      for (uint64_t base = b; base < e; base += 128) {
-        const uint64_t k = base + lane;
-        const uint32_t w0 = k < e ? cols[k] : 0;
-        const uint32_t w1 = k + 64 < e ? cols[k + 64] : 0;
-        uint64_t x = 0;
-        if (k < e) x = front_of(front, nz_front, filter, w0);
-        if (k + 64 < e) x |= front_of(front, nz_front, filter, w1);
-        a |= wave_or(x);
-        if ((a & nd) == nd) break;
-      }
-      if (lane == src) acc = a;
-    }
-    const uint64_t nw = acc & need;
-    const uint64_t b = __ballot(nw != 0);
-    if (lane == 0) nz_next[wd] = b;
-    if (inb) {
-      next[v] = nw;
-      if (nw) {
-        seen[v] = s | nw;
-        live |= nw;
-        pairs += __popcll(nw);
-        ++nz;
-        fdeg += offsets[v + 1] - offsets[v];
-      }
-    }
-  }
-  ms_block_counters(live, pairs, nz, fdeg, c);
 }
 
 __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
@@ -288,114 +108,11 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
                                                     const uint32_t* __restrict__ first_nbr,
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ cols, int64_t rows,
-                                                    uint64_t live_in, int filter, unsigned long long* __restrict__ c) {
-  ms_pull_body(seen, front, nz_front, next, nz_next, first_nbr, offsets, cols, rows, live_in, filter != 0, c,
-               blockIdx.x, gridDim.x);
-}
-
-// Dense pull step as an edge-parallel sweep over the whole CSR: each lane
-// takes kSweepEpt consecutive column entries (a wave reads 2 KiB of cols in
-// one coalesced pass), ORs the frontier words of the neighbours whose
-// nonzero bit is set into its row's LDS accumulator, and the block writes
-// one word per row: a plain store for rows that lie inside the chunk, an
-// atomicOr for the (at most two) rows that cross its ends, whose words were
-// cleared beforehand by k_ms_sweep_clear.  No early exit: this is the form
-// for levels at which most rows would be scanned to the end anyway.  Rows
-// are those of modes 1 and 2 (every row has at least one entry), so at most
-// kSweepEpb rows overlap a chunk.  next then holds, per vertex, the lanes of
-// its neighbours' frontier words; k_ms_finish keeps the new ones.
-constexpr int kSweepEpt = 4;
-constexpr int kSweepEpb = kSweepEpt * kBlock;
-
-__device__ __forceinline__ void ms_sweep_clear_body(const uint32_t* __restrict__ chunk_rows, uint64_t nchunks,
-                                                    const uint64_t* __restrict__ offsets, uint64_t* __restrict__ next,
-                                                    unsigned bid, unsigned nblk) {
-  for (uint64_t c = (uint64_t)bid * kBlock + threadIdx.x; c < nchunks; c += (uint64_t)nblk * kBlock) {
-    const uint32_t r0 = chunk_rows[2 * c], r1 = chunk_rows[2 * c + 1];
-    const uint64_t e0 = c * kSweepEpb;
-    if (offsets[r0] < e0) next[r0] = 0;                  // first row began in an earlier chunk
-    if (offsets[r1 + 1] > e0 + kSweepEpb) next[r1] = 0;  // last row goes on into a later chunk
-  }
-}
-
-// LDS of one sweep chunk: row starts relative to the chunk's first entry e0
-// (clamped to -1 before it and kSweepEpb + 1 after it, which keeps every
-// comparison below exact) and row accumulators.
-struct SweepLds {
-  int32_t off[kSweepEpb + 2];
-  unsigned long long acc[kSweepEpb + 1];
-};
-
-// Stages the row starts of chunk c; returns the row count (every lane calls
-// it; ends with a barrier).
-__device__ __forceinline__ int sweep_stage(uint64_t chunk, const uint32_t* __restrict__ chunk_rows,
-                                           const uint64_t* __restrict__ offsets, SweepLds& L, bool clear_acc) {
-  const uint32_t r0 = chunk_rows[2 * chunk];
-  const int64_t e0 = (int64_t)(chunk * kSweepEpb);
-  int nr = (int)(chunk_rows[2 * chunk + 1] - r0 + 1);
-  nr = nr > kSweepEpb ? kSweepEpb : nr;  // bound for LDS; rows are non-empty in modes 1 and 2
-  for (int i = threadIdx.x; i <= nr; i += kBlock) {
-    int64_t d = (int64_t)offsets[r0 + i] - e0;
-    d = d < -1 ? -1 : (d > kSweepEpb + 1 ? kSweepEpb + 1 : d);
-    L.off[i] = (int32_t)d;
-    if (clear_acc && i < nr) L.acc[i] = 0;
-  }
-  __syncthreads();
-  return nr;
-}
-
-// Last staged row starting at or before relative entry e (LDS binary search).
-__device__ __forceinline__ int sweep_row_of(const SweepLds& L, int nr, int e) {
-  int lo = 0, hi = nr - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (L.off[mid] <= e) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ void ms_sweep_body(const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
-                                              const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols,
-                                              const uint64_t* __restrict__ front,
-                                              const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
-                                              SweepLds& L, unsigned bid, unsigned nblk) {
-  for (uint64_t chunk = bid; chunk * kSweepEpb < nnz; chunk += nblk) {
-    const uint64_t e0 = chunk * kSweepEpb;
-    const uint64_t e1 = (e0 + kSweepEpb < nnz) ? e0 + kSweepEpb : nnz;
-    const uint32_t r0 = chunk_rows[2 * chunk];
-    const int nr = sweep_stage(chunk, chunk_rows, offsets, L, true);
-    const int len = (int)(e1 - e0);
-    const int my0 = threadIdx.x * kSweepEpt;  // relative to e0
-    if (my0 < len) {
-      int lo = sweep_row_of(L, nr, my0);
-      const int my1 = (my0 + kSweepEpt < len) ? my0 + kSweepEpt : len;
-      uint32_t w[kSweepEpt];
-#pragma unroll
-      for (int j = 0; j < kSweepEpt; ++j) w[j] = (my0 + j < my1) ? cols[e0 + my0 + j] : 0;
-      uint64_t x[kSweepEpt];
-#pragma unroll
-      for (int j = 0; j < kSweepEpt; ++j) x[j] = (my0 + j < my1 && bit_set(nz_front, w[j])) ? front[w[j]] : 0;
-      uint64_t acc = 0;
-      for (int j = 0; j < kSweepEpt && my0 + j < my1; ++j) {
-        while (lo + 1 < nr && L.off[lo + 1] <= my0 + j) {
-          if (acc) atomicOr(&L.acc[lo], (unsigned long long)acc);
-          acc = 0;
-          ++lo;
-        }
-        acc |= x[j];
-      }
-      if (acc) atomicOr(&L.acc[lo], (unsigned long long)acc);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nr; i += kBlock) {
-      const uint64_t a = L.acc[i];
-      if (L.off[i] >= 0 && L.off[i + 1] <= len)
-        next[r0 + i] = a;
-      else if (a)
-        atomicOr((unsigned long long*)(next + r0 + i), (unsigned long long)a);
-    }
-    __syncthreads();
-  }
+                                                    uint64_t live_in, int mode, const uint64_t* __restrict__ pack,
+                                                    const uint64_t* __restrict__ compact,
+                                                    unsigned long long* __restrict__ c) {
+  const FrontView fv{front, nz_front, pack, compact, mode};
+  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_ms_sweep_clear(const uint32_t* __restrict__ chunk_rows,
@@ -412,61 +129,6 @@ __global__ void __launch_bounds__(kBlock) k_ms_sweep(const uint32_t* __restrict_
                                                      uint64_t* __restrict__ next) {
   __shared__ SweepLds L;
   ms_sweep_body(chunk_rows, nnz, offsets, cols, front, nz_front, next, L, blockIdx.x, gridDim.x);
-}
-
-// Push step over the CSR in sweep chunks, for the persistent kernel (no
-// frontier queue, so no scan): a chunk whose rows hold no frontier vertex is
-// skipped after a look at its nonzero-bitmap words; otherwise each lane takes
-// kSweepEpt consecutive entries and, for those whose row u is in the
-// frontier, sends front[u] & ~seen[w] to the target w with a filtered
-// 64-bit atomicOr.  next must be zero beforehand; seen is not written.
-__device__ __forceinline__ void ms_push_csr_body(const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
-                                                 const uint64_t* __restrict__ offsets,
-                                                 const uint32_t* __restrict__ cols,
-                                                 const uint64_t* __restrict__ front,
-                                                 const uint64_t* __restrict__ nz_front,
-                                                 const uint64_t* __restrict__ seen, uint64_t* next, SweepLds& L,
-                                                 unsigned bid, unsigned nblk) {
-  for (uint64_t chunk = bid; chunk * kSweepEpb < nnz; chunk += nblk) {
-    const uint32_t r0 = chunk_rows[2 * chunk], r1 = chunk_rows[2 * chunk + 1];
-    // any frontier row in [r0, r1]?
-    bool any = false;
-    for (uint32_t wd = (r0 >> 6) + threadIdx.x; wd <= (r1 >> 6); wd += kBlock) {
-      uint64_t x = nz_front[wd];
-      if (wd == (r0 >> 6)) x &= ~UINT64_C(0) << (r0 & 63);
-      if (wd == (r1 >> 6) && (r1 & 63) != 63) x &= (UINT64_C(1) << ((r1 & 63) + 1)) - 1;
-      any |= x != 0;
-    }
-    if (!__syncthreads_or(any)) continue;
-    const uint64_t e0 = chunk * kSweepEpb;
-    const uint64_t e1 = (e0 + kSweepEpb < nnz) ? e0 + kSweepEpb : nnz;
-    const int nr = sweep_stage(chunk, chunk_rows, offsets, L, false);
-    const int len = (int)(e1 - e0);
-    const int my0 = threadIdx.x * kSweepEpt;  // relative to e0
-    if (my0 < len) {
-      int lo = sweep_row_of(L, nr, my0);
-      const int my1 = (my0 + kSweepEpt < len) ? my0 + kSweepEpt : len;
-      uint32_t u = r0 + lo;
-      uint64_t fu = bit_set(nz_front, u) ? front[u] : 0;
-      for (int e = my0; e < my1; ++e) {
-        if (lo + 1 < nr && L.off[lo + 1] <= e) {
-          while (lo + 1 < nr && L.off[lo + 1] <= e) ++lo;
-          u = r0 + lo;
-          fu = bit_set(nz_front, u) ? front[u] : 0;
-        }
-        if (!fu) continue;
-        const uint32_t w = cols[e0 + e];
-        const uint64_t m = fu & ~seen[w];
-        if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
-          atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
-      }
-    }
-    __syncthreads();
-  }
-}
-
-__device__ __forceinline__ void ms_zero_body(uint64_t* __restrict__ a, int64_t n, unsigned bid, unsigned nblk) {
-  for (int64_t i = (int64_t)bid * kBlock + threadIdx.x; i < n; i += (int64_t)nblk * kBlock) a[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -531,6 +193,8 @@ struct MsPersistArgs {
   int* dir;                     // [max_levels] level kind (1 push, 0 pull, 2 sweep)
   int* done_levels;             // last level completed
   GridBarrier* bar;
+  uint32_t* chunk_list;         // sweep chunks holding a frontier row (push levels)
+  unsigned* chunk_count;
   double push_limit;            // nnz / alpha
   double sweep_below;           // sweep while reached pairs < sweep_below x roots x rows
   int pull_filter;              // pull reads a frontier word only after its nonzero bit
@@ -569,8 +233,12 @@ k_ms_persistent(MsPersistArgs a) {
     const long long deadline = wall_clock64() + a.timeout_ticks;
     if (push) {
       ms_zero_body(next, a.rows, bid, nblk);
+      if (bid == 0 && threadIdx.x == 0) *a.chunk_count = 0;
       if (!grid_barrier(a.bar, nblk, deadline)) return;
-      ms_push_csr_body(a.sweep_rows, a.nnz, a.offsets, a.cols, front, nz_front, a.seen, next, L, bid, nblk);
+      ms_collect_chunks_body(a.sweep_rows, a.sweep_chunks, nz_front, a.chunk_list, a.chunk_count, bid, nblk);
+      if (!grid_barrier(a.bar, nblk, deadline)) return;
+      ms_push_list_body(a.chunk_list, a.chunk_count, a.sweep_rows, a.nnz, a.offsets, a.cols, front, nz_front,
+                        a.seen, next, L, bid, nblk);
       if (!grid_barrier(a.bar, nblk, deadline)) return;
       ms_finish_body(a.seen, next, nz_next, a.offsets, a.rows, c, bid, nblk);
     } else if (sweep) {
@@ -580,8 +248,8 @@ k_ms_persistent(MsPersistArgs a) {
       if (!grid_barrier(a.bar, nblk, deadline)) return;
       ms_finish_body(a.seen, next, nz_next, a.offsets, a.rows, c, bid, nblk);
     } else {
-      ms_pull_body(a.seen, front, nz_front, next, nz_next, a.first_nbr, a.offsets, a.cols, a.rows, live,
-                   a.pull_filter != 0, c, bid, nblk);
+      const FrontView fv{front, nz_front, nullptr, nullptr, a.pull_filter ? kFrontFilter : kFrontPlain};
+      ms_pull_body(a.seen, fv, next, nz_next, a.first_nbr, a.offsets, a.cols, a.rows, live, c, bid, nblk);
     }
     if (!grid_barrier(a.bar, nblk, deadline)) return;
     live = __hip_atomic_load(&c[M_LIVE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -776,9 +444,18 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       hipLaunchKernelGGL(k_ms_finish, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
                          next, nz_next, g.offsets.p, rows, cnt);
     } else {
+      int mode = m.pull_filter ? kFrontFilter : kFrontPlain;
+      if (m.compact_below > 0 && (double)nz < m.compact_below * (double)rows) {
+        // frontier words of few vertices: pack them densely
+        m.pack.alloc(2 * words);
+        m.compact.alloc(rows);
+        device_scan(CompactCount{nz_front}, CompactEmit{nz_front, front, m.pack.p, m.compact.p}, (size_t)words,
+                    m.scan_tmp.p, s);
+        mode = kFrontCompact;
+      }
       hipLaunchKernelGGL(k_ms_pull, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
                          m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows,
-                         live, m.pull_filter, cnt);
+                         live, mode, m.pack.p, m.compact.p, cnt);
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(m.lev_end[l], s));
@@ -822,7 +499,8 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   m.p_cnt.alloc((size_t)K * M_NUM);
   m.p_ts.alloc(K);
   m.p_dir.alloc(K);
-  m.p_misc.alloc(4);  // [0]: done level, [1..2]: grid barrier
+  m.p_misc.alloc(4);  // [0]: done level, [1..2]: grid barrier, [3]: active chunk count
+  m.chunk_list.alloc(m.sweep_chunks);
   if (!m.h_persist) {
     // counters [levels][M_NUM] | clock [levels] | done level + barrier (4) | roots (64)
     G5_HIP(hipHostMalloc((void**)&m.h_persist, ((size_t)G5_MAX_LEVELS * (M_NUM + 2) + 64) * sizeof(uint64_t),
@@ -863,6 +541,8 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   a.dir = (int*)m.p_dir.p;
   a.done_levels = (int*)m.p_misc.p;
   a.bar = (GridBarrier*)(m.p_misc.p + 1);
+  a.chunk_list = m.chunk_list.p;
+  a.chunk_count = (unsigned*)(m.p_misc.p + 3);
   a.push_limit = (double)g.nnz / m.alpha;
   a.sweep_below = m.sweep_below;
   a.pull_filter = m.pull_filter;

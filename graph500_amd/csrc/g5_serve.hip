@@ -1,0 +1,616 @@
+// g5_serve.hip -- serving loop: (graph, root) BFS queries over graphs resident
+// in HBM, with continuous batching.
+//
+// Each graph has an engine: the 64-lane bit-packed BFS of the batched path
+// (one lane per query), run one level at a time on the graph's own stream, so
+// the levels of different graphs overlap on the device.  At every level
+// boundary the engine
+//   * retires the lanes that discovered nothing in the level just finished
+//     (their answers are complete; their traversed-edge counts are taken from
+//     the seen words before the lane is reused),
+//   * admits waiting queries into free lanes while answer memory allows (one
+//     n_rows-byte level array per query, from a preallocated pool), placing
+//     each root into the current frontier at the current level,
+//   * launches the next level for every active lane, whatever level each
+//     lane's own search has reached.
+// A lane's level values are written into its query's answer array as they
+// are discovered (level = engine level - the level the lane was admitted
+// at), so an answer needs no extraction.  A retired lane "cools" for one
+// level: the next level's sweep clears its bits from the seen words.
+//
+// Reference: there is no serving loop in the reference; each answer equals
+// BfsBase::run_bfs's levels for the same root (mpi/bfs.hpp:2651-2940).
+#include "g5_ctx.hpp"
+#include "g5_ms_device.hpp"
+
+#include <chrono>
+#include <deque>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace g5 {
+
+// Writes the level of every newly reached (lane, vertex) pair into the
+// lane's answer array; a wave writes 64 consecutive bytes per lane.
+struct ServeEpi {
+  uint8_t* ans[64];
+  uint8_t lvl[64];
+  __device__ void operator()(int64_t v, bool inb, uint64_t nw) const {
+    uint64_t u = wave_or(nw);
+    while (u) {
+      const int r = __builtin_ctzll(u);
+      u &= u - 1;
+      if (inb && ((nw >> r) & 1)) ans[r][v] = lvl[r];
+    }
+  }
+};
+
+struct ServeRoots {
+  int64_t root[64];  // original id of the query admitted into each lane, -1 = none
+  uint8_t* ans[64];
+};
+
+__global__ void __launch_bounds__(kBlock) k_sv_pull(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
+                                                    const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
+                                                    uint64_t* __restrict__ nz_next,
+                                                    const uint32_t* __restrict__ first_nbr,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ cols, int64_t rows, uint64_t live_in,
+                                                    int filter, unsigned long long* __restrict__ c, uint64_t keep,
+                                                    ServeEpi epi) {
+  const FrontView fv{front, nz_front, nullptr, nullptr, filter ? kFrontFilter : kFrontPlain};
+  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x, keep, epi);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sv_finish(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
+                                                      uint64_t* __restrict__ nz_next,
+                                                      const uint64_t* __restrict__ offsets, int64_t rows,
+                                                      unsigned long long* __restrict__ c, uint64_t keep,
+                                                      ServeEpi epi) {
+  ms_finish_body(seen, next, nz_next, offsets, rows, c, blockIdx.x, gridDim.x, keep, epi);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sv_collect(const uint32_t* __restrict__ chunk_rows, uint64_t nchunks,
+                                                       const uint64_t* __restrict__ nz_front,
+                                                       uint32_t* __restrict__ list, unsigned* count) {
+  ms_collect_chunks_body(chunk_rows, nchunks, nz_front, list, count, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sv_push(const uint32_t* __restrict__ list, const unsigned* count,
+                                                    const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ cols,
+                                                    const uint64_t* __restrict__ front,
+                                                    const uint64_t* __restrict__ nz_front,
+                                                    const uint64_t* __restrict__ seen, uint64_t* next) {
+  __shared__ SweepLds L;
+  ms_push_list_body(list, count, chunk_rows, nnz, offsets, cols, front, nz_front, seen, next, L, blockIdx.x,
+                    gridDim.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sv_zero(uint64_t* __restrict__ a, int64_t n) {
+  ms_zero_body(a, n, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sv_fill_bytes(uint8_t* __restrict__ p, int64_t n, uint8_t val) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = val;
+}
+
+// seen &= keep over all rows (lanes cooling while the engine is idle).
+__global__ void __launch_bounds__(kBlock) k_sv_clear(uint64_t* __restrict__ seen, int64_t rows, uint64_t keep) {
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < rows; v += (int64_t)gridDim.x * kBlock) {
+    const uint64_t s = seen[v];
+    if (s & ~keep) seen[v] = s & keep;
+  }
+}
+
+// Places the admitted roots into seen / frontier / nonzero bitmap and their
+// answers; one thread per lane (lanes may share a root vertex).
+__global__ void k_sv_admit(const uint32_t* __restrict__ old2new, int64_t rows, ServeRoots rs,
+                           uint64_t* __restrict__ seen, uint64_t* __restrict__ front, uint64_t* __restrict__ nz_front) {
+  const int r = threadIdx.x;
+  if (r >= 64 || rs.root[r] < 0) return;
+  const uint32_t v = old2new[rs.root[r]];
+  if ((int64_t)v >= rows) return;  // a root without edges reaches nothing
+  const unsigned long long bit = 1ull << r;
+  atomicOr((unsigned long long*)(seen + v), bit);
+  atomicOr((unsigned long long*)(front + v), bit);
+  atomicOr((unsigned long long*)(nz_front + (v >> 6)), 1ull << (v & 63));
+  rs.ans[r][v] = 0;
+}
+
+// Per-lane sums of the tuple weights of the reached vertices, lanes in mask.
+__global__ void __launch_bounds__(kBlock) k_sv_teps(const uint64_t* __restrict__ seen,
+                                                    const uint32_t* __restrict__ weight, int64_t rows, uint64_t mask,
+                                                    unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long s_acc[64];
+  if (threadIdx.x < 64) s_acc[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < rows; v += (int64_t)gridDim.x * kBlock) {
+    uint64_t x = seen[v] & mask;
+    if (!x) continue;
+    const uint64_t w = weight[v];
+    while (x) {
+      atomicAdd(&s_acc[__builtin_ctzll(x)], (unsigned long long)w);
+      x &= x - 1;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 64 && s_acc[threadIdx.x]) atomicAdd(acc + threadIdx.x, s_acc[threadIdx.x]);
+}
+
+// Answer (new ids) -> levels by original id.
+__global__ void k_sv_levels(const uint32_t* __restrict__ old2new, int64_t n, int64_t rows,
+                            const uint8_t* __restrict__ ans, int64_t root, uint8_t* __restrict__ out) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t nv = old2new[v];
+    out[v] = (v == root) ? 0 : ((int64_t)nv < rows ? ans[nv] : 0xFF);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host
+// ---------------------------------------------------------------------------
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+enum { SV_WAITING = 0, SV_RUNNING = 1, SV_DONE = 2, SV_RELEASED = 3 };
+
+struct Request {
+  int graph = 0;
+  int64_t root = 0;
+  double arrival = 0, admit = -1, done = -1;
+  int lane = -1;
+  int buf = -1;
+  int64_t edges = -1;
+  int status = SV_WAITING;
+  bool keep = false;
+  int levels = 0;  // levels the query's lane took part in
+  int64_t result_slot = -1;
+};
+
+struct Engine {
+  g5_ctx* ctx = nullptr;
+  hipStream_t s = nullptr;
+  int64_t n = 0, rows = 0, words = 0, nnz = 0;
+  uint64_t nchunks = 0;
+  DevBuf<uint64_t> seen, fr[2], nz[2];
+  DevBuf<uint32_t> chunk_rows, chunk_list;
+  DevBuf<unsigned long long> cnt;    // M_NUM counters + chunk count
+  DevBuf<unsigned long long> tacc;   // per-lane tuple-weight sums of a retirement
+  unsigned long long* h = nullptr;   // pinned: [0, M_NUM) counters, [16, 80) weight sums
+  hipEvent_t ev = nullptr;
+  std::vector<std::unique_ptr<DevBuf<uint8_t>>> bufs;
+  std::vector<int> free_bufs;
+  int64_t lane_req[64];
+  int lane_l0[64];
+  uint64_t active = 0, cooling = 0;
+  int level = 0;        // fr[level & 1] is the current frontier
+  bool busy = false;    // a level is in flight
+  bool teps_pending = false;
+  uint64_t teps_mask = 0;
+  int64_t teps_req[64];
+  uint64_t last_fdeg = 0;
+  std::deque<int64_t> waiting;
+  int64_t levels_run = 0;
+
+  ~Engine() {
+    if (h) (void)hipHostFree(h);
+    if (ev) (void)hipEventDestroy(ev);
+  }
+};
+
+}  // namespace
+
+}  // namespace g5
+
+struct g5_server {
+  std::string err;
+  std::vector<std::unique_ptr<g5::Engine>> eng;
+  std::vector<g5::Request> req;
+  g5_server_config cfg{};
+  int device = 0;
+  int64_t done_count = 0;
+  g5::DevBuf<uint8_t> lv_out;
+};
+
+namespace g5 {
+
+static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
+  Graph& g = ctx->graph;
+  G5_CHECK(g.built, "g5_server_create: a graph context holds no graph (call g5_construct first)");
+  G5_CHECK(g.reorder_mode != 0, "g5_server_create: graphs must be built with reorder mode 1 or 2");
+  e.ctx = ctx;
+  e.s = ctx->stream;
+  e.n = g.n;
+  e.rows = g.n_rows;
+  e.words = (e.rows + 63) / 64;
+  e.nnz = g.nnz;
+  e.seen.alloc(e.rows);
+  for (auto& b : e.fr) b.alloc(e.rows);
+  for (auto& b : e.nz) b.alloc(e.words);
+  e.nchunks = ((uint64_t)e.nnz + kSweepEpb - 1) / kSweepEpb;
+  e.chunk_rows.alloc(2 * e.nchunks + 2);
+  e.chunk_list.alloc(e.nchunks + 1);
+  e.cnt.alloc(M_NUM + 1);
+  e.tacc.alloc(64);
+  G5_HIP(hipHostMalloc((void**)&e.h, 96 * sizeof(unsigned long long), hipHostMallocDefault));
+  G5_HIP(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
+  hipStream_t s = e.s;
+  if (e.nchunks)
+    hipLaunchKernelGGL(k_chunk_rows<kSweepEpb>, dim3(grid_of((int64_t)(2 * e.nchunks), kBlock, 65536)), dim3(kBlock),
+                       0, s, g.offsets.p, e.rows, e.nchunks, e.chunk_rows.p);
+  hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, s, e.seen.p, e.rows);
+  for (auto& b : e.fr) hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, s, b.p, e.rows);
+  for (auto& b : e.nz) hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.words, kBlock, 4096)), dim3(kBlock), 0, s, b.p, e.words);
+  G5_HIP(hipGetLastError());
+  const int nb = srv->cfg.answer_buffers;
+  for (int i = 0; i < nb; ++i) {
+    e.bufs.emplace_back(new DevBuf<uint8_t>());
+    e.bufs.back()->alloc(e.rows ? e.rows : 1);
+    e.free_bufs.push_back(nb - 1 - i);
+  }
+  for (int r = 0; r < 64; ++r) {
+    e.lane_req[r] = -1;
+    e.lane_l0[r] = 0;
+    e.teps_req[r] = -1;
+  }
+  G5_HIP(hipStreamSynchronize(s));
+}
+
+// Consumes the traversed-edge sums of the last retirement (stream order puts
+// them before the level whose completion is being processed).
+static void sv_take_teps(g5_server* srv, Engine& e) {
+  if (!e.teps_pending) return;
+  uint64_t m = e.teps_mask;
+  while (m) {
+    const int r = __builtin_ctzll(m);
+    m &= m - 1;
+    Request& q = srv->req[e.teps_req[r]];
+    q.edges = (int64_t)(e.h[16 + r] / 2);
+  }
+  e.teps_pending = false;
+  e.teps_mask = 0;
+}
+
+// Processes a finished level; returns true if anything changed.
+static void sv_complete_level(g5_server* srv, Engine& e, double t) {
+  sv_take_teps(srv, e);
+  const uint64_t live = e.h[M_LIVE];
+  e.last_fdeg = e.h[M_FDEG];
+  e.cooling = 0;  // cleared from seen by the level just finished
+  const uint64_t finished = e.active & ~live;
+  if (finished) {
+    // traversed edges of the finished lanes, before their bits are cleared
+    G5_HIP(hipMemsetAsync(e.tacc.p, 0, 64 * 8, e.s));
+    hipLaunchKernelGGL(k_sv_teps, dim3(grid_of(e.rows, kBlock, 2048)), dim3(kBlock), 0, e.s, e.seen.p,
+                       e.ctx->graph.weight.p, e.rows, finished, e.tacc.p);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpyAsync(e.h + 16, e.tacc.p, 64 * 8, hipMemcpyDeviceToHost, e.s));
+    e.teps_pending = true;
+    e.teps_mask = finished;
+    uint64_t m = finished;
+    while (m) {
+      const int r = __builtin_ctzll(m);
+      m &= m - 1;
+      const int64_t id = e.lane_req[r];
+      Request& q = srv->req[id];
+      q.status = SV_DONE;
+      q.done = t;
+      ++srv->done_count;
+      q.levels = e.level - e.lane_l0[r];
+      e.teps_req[r] = id;
+      e.lane_req[r] = -1;
+      if (!q.keep) {
+        e.free_bufs.push_back(q.buf);
+        q.buf = -1;
+      }
+    }
+    e.active &= ~finished;
+    e.cooling = finished;
+  }
+}
+
+// Admits waiting queries into free lanes; returns the lanes admitted.
+static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
+  const uint64_t busy_lanes = e.active | e.cooling;
+  if (e.waiting.empty() || busy_lanes == ~UINT64_C(0) || e.free_bufs.empty()) return 0;
+  ServeRoots rs;
+  for (int r = 0; r < 64; ++r) {
+    rs.root[r] = -1;
+    rs.ans[r] = nullptr;
+  }
+  uint64_t admitted = 0;
+  for (int r = 0; r < 64 && !e.waiting.empty() && !e.free_bufs.empty(); ++r) {
+    if ((busy_lanes >> r) & 1) continue;
+    const int64_t id = e.waiting.front();
+    e.waiting.pop_front();
+    Request& q = srv->req[id];
+    q.lane = r;
+    q.buf = e.free_bufs.back();
+    e.free_bufs.pop_back();
+    q.admit = t;
+    q.status = SV_RUNNING;
+    e.lane_req[r] = id;
+    e.lane_l0[r] = e.level;
+    uint8_t* ans = e.bufs[q.buf]->p;
+    hipLaunchKernelGGL(k_sv_fill_bytes, dim3(grid_of(e.rows, kBlock, 2048)), dim3(kBlock), 0, e.s, ans, e.rows,
+                       (uint8_t)0xFF);
+    rs.root[r] = q.root;
+    rs.ans[r] = ans;
+    admitted |= UINT64_C(1) << r;
+  }
+  if (admitted) {
+    hipLaunchKernelGGL(k_sv_admit, dim3(1), dim3(64), 0, e.s, e.ctx->graph.old2new.p, e.rows, rs, e.seen.p,
+                       e.fr[e.level & 1].p, e.nz[e.level & 1].p);
+    G5_HIP(hipGetLastError());
+    e.active |= admitted;
+  }
+  return admitted;
+}
+
+// Launches the next level for the active lanes.
+static void sv_launch_level(g5_server* srv, Engine& e) {
+  Graph& g = e.ctx->graph;
+  hipStream_t s = e.s;
+  const int cur = e.level & 1;
+  const int l = e.level + 1;
+  G5_CHECK(l < (1 << 30), "serving: level counter overflow");
+  uint64_t* front = e.fr[cur].p;
+  uint64_t* next = e.fr[cur ^ 1].p;
+  uint64_t* nz_front = e.nz[cur].p;
+  uint64_t* nz_next = e.nz[cur ^ 1].p;
+  ServeEpi epi;
+  uint64_t m = e.active;
+  for (int r = 0; r < 64; ++r) {
+    epi.ans[r] = nullptr;
+    epi.lvl[r] = 0;
+  }
+  while (m) {
+    const int r = __builtin_ctzll(m);
+    m &= m - 1;
+    const int rel = l - e.lane_l0[r];
+    G5_CHECK(rel <= 254, "serving: a query needs more than 254 levels (uint8 level arrays)");
+    epi.ans[r] = e.bufs[srv->req[e.lane_req[r]].buf]->p;
+    epi.lvl[r] = (uint8_t)rel;
+  }
+  const uint64_t keep = ~e.cooling;
+  const bool push = (double)e.last_fdeg < (double)e.nnz / srv->cfg.alpha;
+  unsigned long long* c = e.cnt.p;
+  G5_HIP(hipMemsetAsync(c, 0, (M_NUM + 1) * 8, s));
+  if (push) {
+    hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, s, next, e.rows);
+    hipLaunchKernelGGL(k_sv_collect, dim3(grid_of((int64_t)e.nchunks, kBlock, 4096)), dim3(kBlock), 0, s,
+                       e.chunk_rows.p, e.nchunks, nz_front, e.chunk_list.p, (unsigned*)(c + M_NUM));
+    hipLaunchKernelGGL(k_sv_push, dim3(grid_of((int64_t)e.nchunks, 1, 4096)), dim3(kBlock), 0, s, e.chunk_list.p,
+                       (const unsigned*)(c + M_NUM), e.chunk_rows.p, (uint64_t)e.nnz, g.offsets.p, g.cols.p, front,
+                       nz_front, e.seen.p, next);
+    hipLaunchKernelGGL(k_sv_finish, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, e.seen.p, next,
+                       nz_next, g.offsets.p, e.rows, c, keep, epi);
+  } else {
+    hipLaunchKernelGGL(k_sv_pull, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, e.seen.p, front,
+                       nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, e.rows, e.active,
+                       srv->cfg.pull_filter, c, keep, epi);
+  }
+  G5_HIP(hipGetLastError());
+  G5_HIP(hipMemcpyAsync(e.h, c, M_NUM * 8, hipMemcpyDeviceToHost, s));
+  G5_HIP(hipEventRecord(e.ev, s));
+  e.level = l;
+  e.busy = true;
+  ++e.levels_run;
+}
+
+// One non-blocking step of an engine; returns true if it made progress.
+static bool sv_step(g5_server* srv, Engine& e, double t) {
+  if (e.busy) {
+    const hipError_t q = hipEventQuery(e.ev);
+    if (q == hipErrorNotReady) return false;
+    G5_HIP(q);
+    e.busy = false;
+    sv_complete_level(srv, e, t);
+  }
+  if (!e.active && e.cooling && !e.waiting.empty()) {
+    // idle engine: clear the cooling lanes directly before reusing them
+    hipLaunchKernelGGL(k_sv_clear, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, e.s, e.seen.p, e.rows,
+                       ~e.cooling);
+    G5_HIP(hipGetLastError());
+    e.cooling = 0;
+  }
+  const uint64_t adm = sv_admit(srv, e, t);
+  if (adm && !(e.active & ~adm)) e.last_fdeg = 0;  // only new roots: a tiny frontier
+  if (e.active) {
+    sv_launch_level(srv, e);
+    return true;
+  }
+  return adm != 0;
+}
+
+static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeout_s, g5_query_result* res,
+                   g5_server_report* rep) {
+  const int ng = (int)srv->eng.size();
+  for (int64_t i = 0; i < nq; ++i) {
+    G5_CHECK(qs[i].graph >= 0 && qs[i].graph < ng, "g5_server_run: query graph index out of range");
+    G5_CHECK(qs[i].root >= 0 && qs[i].root < srv->eng[qs[i].graph]->n, "g5_server_run: query root out of range");
+    G5_CHECK(i == 0 || qs[i].arrival_s >= qs[i - 1].arrival_s, "g5_server_run: arrivals must be non-decreasing");
+  }
+  const int64_t base = (int64_t)srv->req.size();
+  srv->req.resize(base + nq);
+  for (int64_t i = 0; i < nq; ++i) {
+    Request& q = srv->req[base + i];
+    q.graph = qs[i].graph;
+    q.root = qs[i].root;
+    q.arrival = qs[i].arrival_s;
+    q.keep = qs[i].keep != 0;
+  }
+  int64_t next_arrival = 0, done = 0;
+  const int64_t done0 = srv->done_count;
+  const double t0 = now_s();
+  int64_t levels0 = 0;
+  for (auto& e : srv->eng) levels0 += e->levels_run;
+  int max_active = 0;
+  while (done < nq) {
+    const double t = now_s() - t0;
+    G5_CHECK_CODE(t <= timeout_s, G5_ERR_LIMIT, "g5_server_run: timed out before every query completed");
+    while (next_arrival < nq && srv->req[base + next_arrival].arrival <= t) {
+      Request& q = srv->req[base + next_arrival];
+      srv->eng[q.graph]->waiting.push_back(base + next_arrival);
+      ++next_arrival;
+    }
+    bool progress = false;
+    for (auto& e : srv->eng) {
+      progress |= sv_step(srv, *e, t);
+      const int a = __builtin_popcountll(e->active);
+      if (a > max_active) max_active = a;
+    }
+    done = srv->done_count - done0;
+    if (!progress) {
+      bool any_busy = false;
+      for (auto& e : srv->eng) any_busy |= e->busy;
+      if (!any_busy && next_arrival < nq) {
+        const double wait = srv->req[base + next_arrival].arrival - (now_s() - t0);
+        if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
+      }
+    }
+  }
+  // drain in-flight work (levels launched for lanes that were all retired) and pending edge counts
+  for (auto& e : srv->eng) {
+    G5_HIP(hipStreamSynchronize(e->s));
+    if (e->busy) {
+      e->busy = false;
+      sv_complete_level(srv, *e, now_s() - t0);
+      G5_HIP(hipStreamSynchronize(e->s));
+    }
+    sv_take_teps(srv, *e);
+  }
+  const double wall = now_s() - t0;
+  int64_t edges = 0, levels1 = 0;
+  for (auto& e : srv->eng) levels1 += e->levels_run;
+  for (int64_t i = 0; i < nq; ++i) {
+    const Request& q = srv->req[base + i];
+    g5_query_result& r = res[i];
+    r.request = base + i;
+    r.admit_s = q.admit;
+    r.done_s = q.done;
+    r.edges = q.edges;
+    r.levels = q.levels;
+    r.status = q.status;
+    edges += q.edges > 0 ? q.edges : 0;
+  }
+  if (rep) {
+    rep->wall_s = wall;
+    rep->completed = nq;
+    rep->edges = edges;
+    rep->levels_launched = levels1 - levels0;
+    rep->max_active_lanes = max_active;
+  }
+}
+
+}  // namespace g5
+
+using namespace g5;
+
+namespace {
+template <class F>
+int sv_guarded(g5_server* srv, F&& f) {
+  if (!srv) return G5_ERR_ARG;
+  try {
+    srv->err.clear();
+    G5_HIP(hipSetDevice(srv->device));
+    f();
+    return G5_OK;
+  } catch (const Error& e) {
+    srv->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    srv->err = "host allocation failed";
+    return G5_ERR_MEM;
+  } catch (...) {
+    srv->err = "unexpected exception";
+    return G5_ERR_STATE;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_config* cfg, g5_server** out) {
+  if (!out || !graphs || num_graphs < 1 || num_graphs > 64 || !cfg) return G5_ERR_ARG;
+  *out = nullptr;
+  if (cfg->answer_buffers < 1 || cfg->answer_buffers > 4096 || !(cfg->alpha > 0) ||
+      (cfg->pull_filter != 0 && cfg->pull_filter != 1))
+    return G5_ERR_ARG;
+  for (int i = 0; i < num_graphs; ++i) {
+    if (!graphs[i] || graphs[i]->device != graphs[0]->device) return G5_ERR_ARG;
+    for (int j = 0; j < i; ++j)
+      if (graphs[j] == graphs[i]) return G5_ERR_ARG;
+  }
+  g5_server* srv = new (std::nothrow) g5_server();
+  if (!srv) return G5_ERR_MEM;
+  srv->cfg = *cfg;
+  srv->device = graphs[0]->device;
+  const int rc = sv_guarded(srv, [&] {
+    for (int i = 0; i < num_graphs; ++i) {
+      srv->eng.emplace_back(new Engine());
+      sv_setup_engine(srv, *srv->eng.back(), graphs[i]);
+    }
+  });
+  if (rc != G5_OK) {
+    // keep the message reachable through a failed create: print nothing, return the code
+    delete srv;
+    return rc;
+  }
+  *out = srv;
+  return G5_OK;
+}
+
+void g5_server_destroy(g5_server* srv) {
+  if (!srv) return;
+  (void)hipSetDevice(srv->device);
+  for (auto& e : srv->eng) (void)hipStreamSynchronize(e->s);
+  delete srv;
+}
+
+const char* g5_server_last_error(const g5_server* srv) { return srv ? srv->err.c_str() : "null server"; }
+
+int g5_server_run(g5_server* srv, const g5_query* queries, int64_t num_queries, double timeout_s,
+                  g5_query_result* results, g5_server_report* report) {
+  return sv_guarded(srv, [&] {
+    G5_CHECK(num_queries >= 0 && (num_queries == 0 || (queries && results)), "g5_server_run: bad arguments");
+    G5_CHECK(timeout_s > 0, "g5_server_run: timeout must be positive");
+    sv_run(srv, queries, num_queries, timeout_s, results, report);
+  });
+}
+
+int g5_server_get_levels(g5_server* srv, int64_t request, uint8_t* levels) {
+  return sv_guarded(srv, [&] {
+    G5_CHECK(levels, "g5_server_get_levels: null output");
+    G5_CHECK(request >= 0 && request < (int64_t)srv->req.size(), "g5_server_get_levels: unknown request");
+    const Request& q = srv->req[request];
+    G5_CHECK(q.status == SV_DONE && q.buf >= 0,
+             "g5_server_get_levels: request not complete, not kept (keep = 0) or released");
+    Engine& e = *srv->eng[q.graph];
+    srv->lv_out.alloc(e.n);
+    hipLaunchKernelGGL(k_sv_levels, dim3(grid_of(e.n, kBlock, 65536)), dim3(kBlock), 0, e.s,
+                       e.ctx->graph.old2new.p, e.n, e.rows, e.bufs[q.buf]->p, q.root, srv->lv_out.p);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpyAsync(levels, srv->lv_out.p, e.n, hipMemcpyDeviceToHost, e.s));
+    G5_HIP(hipStreamSynchronize(e.s));
+  });
+}
+
+int g5_server_release(g5_server* srv, int64_t request) {
+  return sv_guarded(srv, [&] {
+    G5_CHECK(request >= 0 && request < (int64_t)srv->req.size(), "g5_server_release: unknown request");
+    Request& q = srv->req[request];
+    G5_CHECK(q.status == SV_DONE, "g5_server_release: request not complete");
+    if (q.buf >= 0) srv->eng[q.graph]->free_bufs.push_back(q.buf);
+    q.buf = -1;
+    q.status = SV_RELEASED;
+  });
+}
+
+}  // extern "C"

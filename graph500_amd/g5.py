@@ -39,7 +39,8 @@ EXPORTS = [
     "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_validate", "g5_validate_pred", "g5_end_bfs",
     "g5_batch_prepare", "g5_batch_set_tuning", "g5_batch_get_tuning", "g5_run_bfs_batch",
     "g5_batch_get_levels", "g5_batch_teps_edges", "g5_batch_stats_get", "g5_batch_model_bytes",
-    "g5_batch_end",
+    "g5_batch_end", "g5_server_create", "g5_server_destroy", "g5_server_last_error", "g5_server_run",
+    "g5_server_get_levels", "g5_server_release",
 ]
 
 
@@ -93,11 +94,36 @@ class BatchStats(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("answer_buffers", ctypes.c_int32), ("pull_filter", ctypes.c_int32), ("alpha", ctypes.c_double)]
+
+
+class Query(ctypes.Structure):
+    _fields_ = [("graph", ctypes.c_int32), ("keep", ctypes.c_int32), ("root", ctypes.c_int64),
+                ("arrival_s", ctypes.c_double)]
+
+
+class QueryResult(ctypes.Structure):
+    _fields_ = [("request", ctypes.c_int64), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("edges", ctypes.c_int64), ("levels", ctypes.c_int32), ("status", ctypes.c_int32)]
+
+
+class ServerReport(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("completed", ctypes.c_int64), ("edges", ctypes.c_int64),
+                ("levels_launched", ctypes.c_int64), ("max_active_lanes", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+QUERY_DTYPE = np.dtype([("graph", np.int32), ("keep", np.int32), ("root", np.int64), ("arrival_s", np.float64)])
+RESULT_DTYPE = np.dtype([("request", np.int64), ("admit_s", np.float64), ("done_s", np.float64),
+                         ("edges", np.int64), ("levels", np.int32), ("status", np.int32)])
+
+
 class BatchTuning(ctypes.Structure):
     _fields_ = [("alpha", ctypes.c_double), ("pull_grid", ctypes.c_int64), ("sweep_below", ctypes.c_double),
                 ("sweep_grid", ctypes.c_int64), ("persistent", ctypes.c_int32), ("persist_levels", ctypes.c_int32),
                 ("persist_grid", ctypes.c_int64), ("persist_waves", ctypes.c_int32), ("pull_filter", ctypes.c_int32),
-                ("beta", ctypes.c_double)]
+                ("beta", ctypes.c_double), ("compact_below", ctypes.c_double)]
 
 
 _lib = None
@@ -150,6 +176,12 @@ def load_library(path=LIB_PATH):
         "g5_batch_stats_get": ([vp, ctypes.POINTER(BatchStats)], i32),
         "g5_batch_model_bytes": ([vp, vp, ctypes.POINTER(ctypes.c_double)], i32),
         "g5_batch_end": ([vp], i32),
+        "g5_server_create": ([vp, i32, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], i32),
+        "g5_server_destroy": ([vp], None),
+        "g5_server_last_error": ([vp], ctypes.c_char_p),
+        "g5_server_run": ([vp, vp, i64, ctypes.c_double, vp, ctypes.POINTER(ServerReport)], i32),
+        "g5_server_get_levels": ([vp, i64, vp], i32),
+        "g5_server_release": ([vp, i64], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -333,7 +365,7 @@ class G5Context:
         self._check(self._lib.g5_batch_prepare(self._h))
 
     TUNING_FIELDS = ("alpha", "pull_grid", "sweep_below", "sweep_grid", "persistent", "persist_levels",
-                     "persist_grid", "persist_waves", "pull_filter", "beta")
+                     "persist_grid", "persist_waves", "pull_filter", "beta", "compact_below")
 
     def batch_set_tuning(self, **kw):
         """Sets any of TUNING_FIELDS; the others keep their values."""
@@ -399,3 +431,58 @@ class G5Context:
 
     def synchronize(self):
         self._check(self._lib.g5_synchronize(self._h))
+
+
+class G5Server:
+    """Serving loop over graphs resident in HBM (g5_server_*): continuous
+    batching of (graph, root) BFS queries, up to 64 in flight per graph."""
+
+    def __init__(self, contexts, answer_buffers=128, alpha=16.0, pull_filter=0):
+        self._lib = load_library()
+        self._ctxs = list(contexts)  # the server uses their graphs and streams
+        arr = (ctypes.c_void_p * len(self._ctxs))(*[c._h for c in self._ctxs])
+        cfg = ServerConfig(answer_buffers, pull_filter, alpha)
+        h = ctypes.c_void_p()
+        rc = self._lib.g5_server_create(arr, len(self._ctxs), ctypes.byref(cfg), ctypes.byref(h))
+        if rc != G5_OK:
+            raise G5Error(rc, "g5_server_create failed (graphs built with reorder mode 1 or 2 on one device?)")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.g5_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise G5Error(rc, self._lib.g5_server_last_error(self._h).decode())
+        return rc
+
+    def run(self, queries, timeout_s=600.0):
+        """queries: array of QUERY_DTYPE (graph, keep, root, arrival_s),
+        arrivals non-decreasing.  Returns (results array of RESULT_DTYPE,
+        report dict)."""
+        q = np.ascontiguousarray(queries, dtype=QUERY_DTYPE)
+        res = np.zeros(len(q), RESULT_DTYPE)
+        rep = ServerReport()
+        self._check(self._lib.g5_server_run(self._h, _ptr(q), len(q), float(timeout_s), _ptr(res),
+                                            ctypes.byref(rep)))
+        return res, {"wall_s": rep.wall_s, "completed": rep.completed, "edges": rep.edges,
+                     "levels_launched": rep.levels_launched, "max_active_lanes": rep.max_active_lanes}
+
+    def levels(self, request, n):
+        out = np.zeros(n, np.uint8)
+        self._check(self._lib.g5_server_get_levels(self._h, int(request), _ptr(out)))
+        return out
+
+    def release(self, request):
+        self._check(self._lib.g5_server_release(self._h, int(request)))

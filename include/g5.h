@@ -99,6 +99,9 @@ typedef struct g5_batch_tuning {
                               1-bit-per-vertex "frontier word is not zero" map is set */
   double beta;             /* pull (never push) once fewer than 1/beta of the (root, vertex) pairs
                               are unreached; 0 disables */
+  double compact_below;    /* per-level pulls (persistent = 0) read frontier words from a compact
+                              copy holding only the frontier's words when the frontier spans fewer
+                              than compact_below x rows vertices; 0 disables */
 } g5_batch_tuning;
 
 /* ---- context ---- */
@@ -215,6 +218,61 @@ int g5_batch_stats_get(g5_ctx* ctx, g5_batch_stats* out);
  * until every lacking root is found; plus 1 byte per (root, reached vertex). */
 int g5_batch_model_bytes(g5_ctx* ctx, double* per_level, double* total);
 int g5_batch_end(g5_ctx* ctx);
+
+/* ---- serving: continuous batching over resident graphs ----
+ * (no counterpart in the reference; every answer equals run_bfs's levels
+ *  for the same root, mpi/bfs.hpp:2651-2940)
+ * A server drives up to 64 graph contexts (each built with reorder mode 1 or
+ * 2, all on one device); each graph runs the 64-lane batched BFS one level at
+ * a time on its own stream.  At every level boundary finished queries leave
+ * their lane and waiting queries take free lanes, while the other lanes go on
+ * at whatever level they have reached.  A query holds one answer array
+ * (num_rows bytes of levels) from a pool of answer_buffers per graph; a query
+ * waits while its graph has no free lane or no free answer array.  The
+ * graph contexts must not be used for anything else while a server runs. */
+typedef struct g5_server g5_server;
+
+typedef struct g5_server_config {
+  int32_t answer_buffers;  /* answer arrays preallocated per graph (HBM admission budget) */
+  int32_t pull_filter;     /* as g5_batch_tuning.pull_filter */
+  double alpha;            /* push while the frontier union has fewer than nnz/alpha edges */
+} g5_server_config;
+
+typedef struct g5_query {
+  int32_t graph;     /* index into the server's graphs */
+  int32_t keep;      /* 1: keep the answer resident until g5_server_release */
+  int64_t root;      /* original vertex id */
+  double arrival_s;  /* arrival time, seconds after g5_server_run starts (non-decreasing) */
+} g5_query;
+
+typedef struct g5_query_result {
+  int64_t request;   /* id for g5_server_get_levels / g5_server_release */
+  double admit_s;    /* when the query took a lane */
+  double done_s;     /* when its answer was complete (latency = done_s - arrival_s) */
+  int64_t edges;     /* traversed edges, as the validator counts them */
+  int32_t levels;    /* levels its lane took part in */
+  int32_t status;    /* 2 = done */
+} g5_query_result;
+
+typedef struct g5_server_report {
+  double wall_s;            /* from the start of the run to the last answer */
+  int64_t completed;
+  int64_t edges;            /* sum of traversed edges */
+  int64_t levels_launched;  /* engine levels over all graphs */
+  int32_t max_active_lanes; /* most lanes active at once in one graph */
+  int32_t reserved;
+} g5_server_report;
+
+int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_config* config, g5_server** out);
+void g5_server_destroy(g5_server* server);
+const char* g5_server_last_error(const g5_server* server);
+/* Open-loop run: query i is submitted at arrival_s; returns when every query
+ * has its answer (or fails with G5_ERR_LIMIT after timeout_s). */
+int g5_server_run(g5_server* server, const g5_query* queries, int64_t num_queries, double timeout_s,
+                  g5_query_result* results, g5_server_report* report);
+/* Levels of a kept answer by original id (0xFF unreached), pred_size() entries. */
+int g5_server_get_levels(g5_server* server, int64_t request, uint8_t* levels);
+int g5_server_release(g5_server* server, int64_t request);
 
 #ifdef __cplusplus
 }

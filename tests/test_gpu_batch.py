@@ -64,6 +64,22 @@ def test_batch_64_roots_match_reference_s16(s16ctx, alpha, sweep, filt, beta, pe
     s16ctx.batch_set_tuning(alpha=16.0, sweep_below=0.25, persistent=0, pull_filter=0, beta=50.0)
 
 
+@pytest.mark.parametrize("compact", [1.01, 0.3])
+def test_batch_compact_frontier_matches_reference_s16(s16ctx, compact):
+    """Pull levels reading the compact frontier copy (always, or when the
+    frontier spans under 30 % of the rows)."""
+    g = gold("ref_s16_64roots.json")
+    roots = [p["root"] for p in g["per_root"]]
+    s16ctx.batch_set_tuning(alpha=1e12 if compact > 1 else 16.0, persistent=0, compact_below=compact, beta=0.0)
+    say("batch s16 compact", compact)
+    s16ctx.run_bfs_batch(roots)
+    teps = s16ctx.batch_teps_edges()
+    for i, pr in enumerate(g["per_root"]):
+        assert sha(s16ctx.batch_levels(i)) == pr["levels_sha256"], (compact, pr["root"])
+        assert teps[i] == pr["nedge"]
+    s16ctx.batch_set_tuning(alpha=16.0, compact_below=0.0, beta=50.0)
+
+
 def test_batch_model_bytes_match_oracle(s16ctx, oracle_mod):
     O = oracle_mod
     oc = O.Csr(O.generate_edges(16))
