@@ -148,12 +148,17 @@ __device__ __forceinline__ bool first_marked_nbr(bool active, uint32_t v, const 
     }
   }
   if (need && re - rb <= (uint64_t)kShortRow) {
-    for (uint64_t k = rb; k < re; ++k) {
-      const uint32_t c = cols[k];
-      if (bit_set(bm, c)) {
+    // four independent probes per step (entries past the row repeat the
+    // first); any marked neighbour is a valid parent, the first is taken
+    for (uint64_t k = rb; k < re && !found; k += 4) {
+      const uint32_t c0 = cols[k];
+      const uint32_t c1 = k + 1 < re ? cols[k + 1] : c0;
+      const uint32_t c2 = k + 2 < re ? cols[k + 2] : c0;
+      const uint32_t c3 = k + 3 < re ? cols[k + 3] : c0;
+      const bool h0 = bit_set(bm, c0), h1 = bit_set(bm, c1), h2 = bit_set(bm, c2), h3 = bit_set(bm, c3);
+      if (h0 | h1 | h2 | h3) {
         found = true;
-        p = c;
-        break;
+        p = h0 ? c0 : (h1 ? c1 : (h2 ? c2 : c3));
       }
     }
     need = false;
@@ -163,17 +168,19 @@ __device__ __forceinline__ bool first_marked_nbr(bool active, uint32_t v, const 
     const int src = __ffsll((unsigned long long)lm) - 1;
     lm &= lm - 1;
     const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
-    for (uint64_t base = b; base < e; base += 64) {
+    // 128 entries per step, two independent probes per lane
+    for (uint64_t base = b; base < e; base += 128) {
       const uint64_t k = base + lane;
-      uint32_t c = 0;
-      bool hit = false;
-      if (k < e) {
-        c = cols[k];
-        hit = bit_set(bm, c);
-      }
-      const uint64_t hm = __ballot(hit);
-      if (hm) {
-        const uint32_t hp = __shfl(c, __ffsll((unsigned long long)hm) - 1, 64);
+      uint32_t c0 = 0, c1 = 0;
+      bool h0 = false, h1 = false;
+      if (k < e) c0 = cols[k];
+      if (k + 64 < e) c1 = cols[k + 64];
+      if (k < e) h0 = bit_set(bm, c0);
+      if (k + 64 < e) h1 = bit_set(bm, c1);
+      const uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
+      if (m0 | m1) {
+        const uint32_t hp = m0 ? __shfl(c0, __ffsll((unsigned long long)m0) - 1, 64)
+                               : __shfl(c1, __ffsll((unsigned long long)m1) - 1, 64);
         if (lane == src) {
           found = true;
           p = hp;

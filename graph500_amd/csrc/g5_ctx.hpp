@@ -96,7 +96,7 @@ struct MsState {
   std::vector<int64_t> host_roots;
   double alpha = 16.0;       // push while the frontier union has < nnz/alpha edges
   int64_t pull_grid = 8192;  // workgroups of the vertex-parallel pull
-  double sweep_below = 0.25; // edge-parallel sweep while reached pairs < this x roots x rows
+  double sweep_below = 0.0;  // edge-parallel sweep while reached pairs < this x roots x rows
   int64_t sweep_grid = 4096; // workgroups of the edge-parallel sweep
   uint64_t sweep_chunks = 0;
   DevBuf<uint32_t> sweep_rows;  // first/last row of every sweep chunk of the CSR

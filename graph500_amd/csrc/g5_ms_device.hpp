@@ -73,9 +73,12 @@ struct UnionEmit {
 
 // Per-level hooks of the finish and pull steps.  keep: lanes of seen that
 // stay (the others belong to finished searches and are cleared on the way);
-// epi(v, inb, nw): called by every lane of a wave with its vertex's new lanes.
+// epi(v, inb, nw): called by every lane of a wave with its vertex's new lanes;
+// epi.begin() / epi.end() by every thread of the block around the sweep.
 struct NoEpilogue {
+  __device__ void begin() const {}
   __device__ void operator()(int64_t, bool, uint64_t) const {}
+  __device__ void end() const {}
 };
 
 // After a push or a sweep: next keeps only lanes new to v, seen gains them,
@@ -90,6 +93,7 @@ __device__ __forceinline__ void ms_finish_body(uint64_t* __restrict__ seen, uint
   const int64_t words = (rows + 63) / 64;
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
+  epi.begin();
   for (int64_t wd = (int64_t)bid * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
     const int64_t v = wd * 64 + lane;
     const bool inb = v < rows;
@@ -111,6 +115,7 @@ __device__ __forceinline__ void ms_finish_body(uint64_t* __restrict__ seen, uint
     const uint64_t b = __ballot(nw != 0);
     if (lane == 0) nz_next[wd] = b;
   }
+  epi.end();
   ms_block_counters(live, pairs, nz, fdeg, c);
 }
 
@@ -175,17 +180,20 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
                                              const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols,
                                              int64_t rows, uint64_t live_in, unsigned long long* __restrict__ c,
                                              unsigned bid, unsigned nblk, uint64_t keep = ~UINT64_C(0),
-                                             const Epi& epi = Epi()) {
+                                             const Epi& epi = Epi(), bool merge = false) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
   uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
   const int64_t words = (rows + 63) / 64;
+  epi.begin();
   for (int64_t wd = (int64_t)bid * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
     const int64_t v = wd * 64 + lane;
     const bool inb = v < rows;
     const uint64_t s0 = inb ? seen[v] : ~UINT64_C(0);
     const uint64_t s = inb ? (s0 & keep) : s0;
     const uint64_t need = ~s & live_in;
+    // lanes pushed into next earlier in this level (merge): new ones join nw
+    const uint64_t pushed = (merge && inb) ? next[v] & ~s : 0;
     uint64_t acc = 0;
     bool more = false;
     uint64_t rb = 0, re = 0;
@@ -232,7 +240,7 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
       }
       if (lane == src) acc = a;
     }
-    const uint64_t nw = acc & need;
+    const uint64_t nw = (acc & need) | pushed;
     epi(v, inb, nw);
     const uint64_t b = __ballot(nw != 0);
     if (lane == 0) nz_next[wd] = b;
@@ -247,6 +255,7 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
       }
     }
   }
+  epi.end();
   ms_block_counters(live, pairs, nz, fdeg, c);
 }
 
@@ -379,7 +388,8 @@ __device__ __forceinline__ void push_chunk(uint64_t chunk, const uint32_t* __res
                                            const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols,
                                            const uint64_t* __restrict__ front,
                                            const uint64_t* __restrict__ nz_front,
-                                           const uint64_t* __restrict__ seen, uint64_t* next, SweepLds& L) {
+                                           const uint64_t* __restrict__ seen, uint64_t* next, SweepLds& L,
+                                           uint64_t lanes = ~UINT64_C(0)) {
   const uint32_t r0 = chunk_rows[2 * chunk];
   const uint64_t e0 = chunk * kSweepEpb;
   const uint64_t e1 = (e0 + kSweepEpb < nnz) ? e0 + kSweepEpb : nnz;
@@ -390,12 +400,12 @@ __device__ __forceinline__ void push_chunk(uint64_t chunk, const uint32_t* __res
     int lo = sweep_row_of(L, nr, my0);
     const int my1 = (my0 + kSweepEpt < len) ? my0 + kSweepEpt : len;
     uint32_t u = r0 + lo;
-    uint64_t fu = bit_set(nz_front, u) ? front[u] : 0;
+    uint64_t fu = bit_set(nz_front, u) ? front[u] & lanes : 0;
     for (int e = my0; e < my1; ++e) {
       if (lo + 1 < nr && L.off[lo + 1] <= e) {
         while (lo + 1 < nr && L.off[lo + 1] <= e) ++lo;
         u = r0 + lo;
-        fu = bit_set(nz_front, u) ? front[u] : 0;
+        fu = bit_set(nz_front, u) ? front[u] & lanes : 0;
       }
       if (!fu) continue;
       const uint32_t w = cols[e0 + e];
@@ -436,10 +446,10 @@ __device__ __forceinline__ void ms_push_list_body(const uint32_t* __restrict__ l
                                                   const uint64_t* __restrict__ front,
                                                   const uint64_t* __restrict__ nz_front,
                                                   const uint64_t* __restrict__ seen, uint64_t* next, SweepLds& L,
-                                                  unsigned bid, unsigned nblk) {
+                                                  unsigned bid, unsigned nblk, uint64_t lanes = ~UINT64_C(0)) {
   const unsigned n = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (unsigned i = bid; i < n; i += nblk)
-    push_chunk(list[i], chunk_rows, nnz, offsets, cols, front, nz_front, seen, next, L);
+    push_chunk(list[i], chunk_rows, nnz, offsets, cols, front, nz_front, seen, next, L, lanes);
 }
 
 __device__ __forceinline__ void ms_zero_body(uint64_t* __restrict__ a, int64_t n, unsigned bid, unsigned nblk) {

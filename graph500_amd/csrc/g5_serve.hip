@@ -32,17 +32,35 @@
 namespace g5 {
 
 // Writes the level of every newly reached (lane, vertex) pair into the
-// lane's answer array; a wave writes 64 consecutive bytes per lane.
+// lane's answer array (a wave writes 64 consecutive bytes per lane) and
+// counts each lane's newly reached vertices (block sums in LDS, one atomic
+// per lane and block into lane_nq).
 struct ServeEpi {
   uint8_t* ans[64];
   uint8_t lvl[64];
+  unsigned long long* lane_nq;
+  __device__ unsigned long long* lds() const {
+    __shared__ unsigned long long s_nq[64];
+    return s_nq;
+  }
+  __device__ void begin() const {
+    if (threadIdx.x < 64) lds()[threadIdx.x] = 0;
+    __syncthreads();
+  }
   __device__ void operator()(int64_t v, bool inb, uint64_t nw) const {
     uint64_t u = wave_or(nw);
     while (u) {
       const int r = __builtin_ctzll(u);
       u &= u - 1;
-      if (inb && ((nw >> r) & 1)) ans[r][v] = lvl[r];
+      const bool mine = inb && ((nw >> r) & 1);
+      if (mine) ans[r][v] = lvl[r];
+      const uint64_t b = __ballot(mine);
+      if ((threadIdx.x & 63) == 0) atomicAdd(&lds()[r], (unsigned long long)__popcll(b));
     }
+  }
+  __device__ void end() const {
+    __syncthreads();
+    if (threadIdx.x < 64 && lds()[threadIdx.x]) atomicAdd(lane_nq + threadIdx.x, lds()[threadIdx.x]);
   }
 };
 
@@ -58,9 +76,10 @@ __global__ void __launch_bounds__(kBlock) k_sv_pull(uint64_t* __restrict__ seen,
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ cols, int64_t rows, uint64_t live_in,
                                                     int filter, unsigned long long* __restrict__ c, uint64_t keep,
-                                                    ServeEpi epi) {
+                                                    int merge, ServeEpi epi) {
   const FrontView fv{front, nz_front, nullptr, nullptr, filter ? kFrontFilter : kFrontPlain};
-  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x, keep, epi);
+  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x, keep, epi,
+               merge != 0);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sv_finish(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
@@ -83,10 +102,11 @@ __global__ void __launch_bounds__(kBlock) k_sv_push(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ cols,
                                                     const uint64_t* __restrict__ front,
                                                     const uint64_t* __restrict__ nz_front,
-                                                    const uint64_t* __restrict__ seen, uint64_t* next) {
+                                                    const uint64_t* __restrict__ seen, uint64_t* next,
+                                                    uint64_t lanes) {
   __shared__ SweepLds L;
   ms_push_list_body(list, count, chunk_rows, nnz, offsets, cols, front, nz_front, seen, next, L, blockIdx.x,
-                    gridDim.x);
+                    gridDim.x, lanes);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sv_zero(uint64_t* __restrict__ a, int64_t n) {
@@ -180,14 +200,19 @@ struct Engine {
   uint64_t nchunks = 0;
   DevBuf<uint64_t> seen, fr[2], nz[2];
   DevBuf<uint32_t> chunk_rows, chunk_list;
-  DevBuf<unsigned long long> cnt;    // M_NUM counters + chunk count
+  DevBuf<unsigned long long> cnt;    // M_NUM counters, chunk count, 64 per-lane new-vertex counts
   DevBuf<unsigned long long> tacc;   // per-lane tuple-weight sums of a retirement
-  unsigned long long* h = nullptr;   // pinned: [0, M_NUM) counters, [16, 80) weight sums
+  unsigned long long* h = nullptr;   // pinned: [0, M_NUM) counters, [16, 80) weight sums, [80, 144) lane counts
   hipEvent_t ev = nullptr;
   std::vector<std::unique_ptr<DevBuf<uint8_t>>> bufs;
   std::vector<int> free_bufs;
   int64_t lane_req[64];
   int lane_l0[64];
+  // per-lane direction state, the reference rule per search (mpi/bfs.hpp:2799-2829)
+  bool lane_fwd[64];
+  bool lane_growing[64];
+  int64_t lane_prev_nq[64];
+  int64_t lane_visited[64];
   uint64_t active = 0, cooling = 0;
   int level = 0;        // fr[level & 1] is the current frontier
   bool busy = false;    // a level is in flight
@@ -236,9 +261,9 @@ static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
   e.nchunks = ((uint64_t)e.nnz + kSweepEpb - 1) / kSweepEpb;
   e.chunk_rows.alloc(2 * e.nchunks + 2);
   e.chunk_list.alloc(e.nchunks + 1);
-  e.cnt.alloc(M_NUM + 1);
+  e.cnt.alloc(M_NUM + 1 + 64);
   e.tacc.alloc(64);
-  G5_HIP(hipHostMalloc((void**)&e.h, 96 * sizeof(unsigned long long), hipHostMallocDefault));
+  G5_HIP(hipHostMalloc((void**)&e.h, 160 * sizeof(unsigned long long), hipHostMallocDefault));
   G5_HIP(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
   hipStream_t s = e.s;
   if (e.nchunks)
@@ -284,6 +309,23 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
   e.last_fdeg = e.h[M_FDEG];
   e.cooling = 0;  // cleared from seen by the level just finished
   const uint64_t finished = e.active & ~live;
+  // each search's own direction for its next level
+  const double na = (double)e.ctx->graph.n_active;
+  uint64_t m = e.active & live;
+  while (m) {
+    const int r = __builtin_ctzll(m);
+    m &= m - 1;
+    const int64_t nq = (int64_t)e.h[80 + r];
+    e.lane_visited[r] += nq;
+    const int64_t unvisited = (int64_t)na - e.lane_visited[r];
+    if (e.lane_growing[r] && nq > e.lane_prev_nq[r]) {
+      if (e.lane_fwd[r] && (double)nq > na / 2000.0) e.lane_fwd[r] = false;
+    } else if (!e.lane_fwd[r] && unvisited < (int64_t)(na / 8.0)) {
+      e.lane_fwd[r] = true;
+      e.lane_growing[r] = false;
+    }
+    e.lane_prev_nq[r] = nq;
+  }
   if (finished) {
     // traversed edges of the finished lanes, before their bits are cleared
     G5_HIP(hipMemsetAsync(e.tacc.p, 0, 64 * 8, e.s));
@@ -337,6 +379,10 @@ static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
     q.status = SV_RUNNING;
     e.lane_req[r] = id;
     e.lane_l0[r] = e.level;
+    e.lane_fwd[r] = true;
+    e.lane_growing[r] = true;
+    e.lane_prev_nq[r] = 0;
+    e.lane_visited[r] = 1;
     uint8_t* ans = e.bufs[q.buf]->p;
     hipLaunchKernelGGL(k_sv_fill_bytes, dim3(grid_of(e.rows, kBlock, 2048)), dim3(kBlock), 0, e.s, ans, e.rows,
                        (uint8_t)0xFF);
@@ -379,25 +425,33 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
     epi.lvl[r] = (uint8_t)rel;
   }
   const uint64_t keep = ~e.cooling;
-  const bool push = (double)e.last_fdeg < (double)e.nnz / srv->cfg.alpha;
+  // top-down lanes push, bottom-up lanes pull, in the same level
+  uint64_t fwd = 0;
+  for (int r = 0; r < 64; ++r)
+    if (e.lane_fwd[r]) fwd |= UINT64_C(1) << r;
+  const uint64_t td = e.active & fwd, bu = e.active & ~fwd;
   unsigned long long* c = e.cnt.p;
-  G5_HIP(hipMemsetAsync(c, 0, (M_NUM + 1) * 8, s));
-  if (push) {
+  epi.lane_nq = c + M_NUM + 1;
+  G5_HIP(hipMemsetAsync(c, 0, (M_NUM + 1 + 64) * 8, s));
+  if (td) {
     hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, s, next, e.rows);
     hipLaunchKernelGGL(k_sv_collect, dim3(grid_of((int64_t)e.nchunks, kBlock, 4096)), dim3(kBlock), 0, s,
                        e.chunk_rows.p, e.nchunks, nz_front, e.chunk_list.p, (unsigned*)(c + M_NUM));
     hipLaunchKernelGGL(k_sv_push, dim3(grid_of((int64_t)e.nchunks, 1, 4096)), dim3(kBlock), 0, s, e.chunk_list.p,
                        (const unsigned*)(c + M_NUM), e.chunk_rows.p, (uint64_t)e.nnz, g.offsets.p, g.cols.p, front,
-                       nz_front, e.seen.p, next);
+                       nz_front, e.seen.p, next, td);
+  }
+  if (bu) {
+    hipLaunchKernelGGL(k_sv_pull, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, e.seen.p, front,
+                       nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, e.rows, bu,
+                       srv->cfg.pull_filter, c, keep, td ? 1 : 0, epi);
+  } else {
     hipLaunchKernelGGL(k_sv_finish, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, e.seen.p, next,
                        nz_next, g.offsets.p, e.rows, c, keep, epi);
-  } else {
-    hipLaunchKernelGGL(k_sv_pull, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, e.seen.p, front,
-                       nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, e.rows, e.active,
-                       srv->cfg.pull_filter, c, keep, epi);
   }
   G5_HIP(hipGetLastError());
   G5_HIP(hipMemcpyAsync(e.h, c, M_NUM * 8, hipMemcpyDeviceToHost, s));
+  G5_HIP(hipMemcpyAsync(e.h + 80, c + M_NUM + 1, 64 * 8, hipMemcpyDeviceToHost, s));
   G5_HIP(hipEventRecord(e.ev, s));
   e.level = l;
   e.busy = true;
@@ -420,13 +474,12 @@ static bool sv_step(g5_server* srv, Engine& e, double t) {
     G5_HIP(hipGetLastError());
     e.cooling = 0;
   }
-  const uint64_t adm = sv_admit(srv, e, t);
-  if (adm && !(e.active & ~adm)) e.last_fdeg = 0;  // only new roots: a tiny frontier
+  sv_admit(srv, e, t);
   if (e.active) {
     sv_launch_level(srv, e);
     return true;
   }
-  return adm != 0;
+  return false;
 }
 
 static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeout_s, g5_query_result* res,
