@@ -58,6 +58,7 @@ sys.path.insert(0, HERE)
 
 METRIC = "GTEPS hmean SCALE 28 ef16: 64-root batch vs single-root; serve p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+STATED_LOAD_QPS = 1000.0  # the serving load at which p50/p99 are quoted
 # batch level kinds (g5_batch_stats.direction) -> (profile name, kernel description)
 BATCH_KERNELS = {
     0: ("k_ms_pull", "k_ms_pull (batched vertex-parallel pull)"),
@@ -234,7 +235,8 @@ def serve_leg(scales, rates, max_queries, edgefactor, seed=1):
         cand = rng.integers(0, n, size=min(n, 200000))
         has.append(np.array([v for v in cand[:20000] if c.has_edge(int(v))], np.int64))
     runs = []
-    with graph500_amd.G5Server(ctxs, answer_buffers=128) as srv:
+    policy = {"answer_buffers": 128, "min_admit": 64, "max_wait_s": 0.3, "lane_rule": 1, "alpha": 64.0}
+    with graph500_amd.G5Server(ctxs, **policy) as srv:
         for rate in rates:
             nq = int(min(max_queries, max(64, rate * 1.0)))
             q = np.zeros(nq, graph500_amd.QUERY_DTYPE)
@@ -261,7 +263,10 @@ def serve_leg(scales, rates, max_queries, edgefactor, seed=1):
             runs.append(run)
     for c in ctxs:
         c.close()
-    return {"graphs_scale": list(scales), "answer_buffers_per_graph": 128, "runs": runs}
+    stated = next((r for r in runs if r["rate_qps"] == STATED_LOAD_QPS), None)
+    return {"graphs_scale": list(scales), "policy": policy, "stated_load_qps": STATED_LOAD_QPS,
+            "p50_ms_at_stated_load": stated["p50_ms"] if stated else None,
+            "p99_ms_at_stated_load": stated["p99_ms"] if stated else None, "runs": runs}
 
 
 def main():
@@ -279,7 +284,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-scales", type=lambda x: [int(v) for v in x.split(",")], default=[18, 22, 25, 27])
     ap.add_argument("--serve-rates", type=lambda x: [float(v) for v in x.split(",")],
-                    default=[100.0, 1000.0, 5000.0, 20000.0])
+                    default=[100.0, 300.0, 1000.0, 3000.0, 20000.0])
     ap.add_argument("--serve-max-queries", type=int, default=3000)
     ap.add_argument("--autotune-budget", type=float, default=30.0)
     ap.add_argument("--copy-calibration", action="store_true")

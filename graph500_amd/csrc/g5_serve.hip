@@ -23,6 +23,7 @@
 #include "g5_ctx.hpp"
 #include "g5_ms_device.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <deque>
 #include <string>
@@ -38,29 +39,38 @@ namespace g5 {
 struct ServeEpi {
   uint8_t* ans[64];
   uint8_t lvl[64];
-  unsigned long long* lane_nq;
+  unsigned long long* lane_nq;    // [0, 64): new vertices per lane, [64, 128): their degree sums
+  const uint64_t* offsets;
+  uint64_t* nz_sel;  // per vertex: new lanes among sel (the top-down lanes), one bit
+  uint64_t sel;
   __device__ unsigned long long* lds() const {
-    __shared__ unsigned long long s_nq[64];
+    __shared__ unsigned long long s_nq[128];
     return s_nq;
   }
   __device__ void begin() const {
-    if (threadIdx.x < 64) lds()[threadIdx.x] = 0;
+    if (threadIdx.x < 128) lds()[threadIdx.x] = 0;
     __syncthreads();
   }
   __device__ void operator()(int64_t v, bool inb, uint64_t nw) const {
     uint64_t u = wave_or(nw);
+    const uint64_t deg = nw ? offsets[v + 1] - offsets[v] : 0;
     while (u) {
       const int r = __builtin_ctzll(u);
       u &= u - 1;
       const bool mine = inb && ((nw >> r) & 1);
-      if (mine) ans[r][v] = lvl[r];
+      if (mine) {
+        ans[r][v] = lvl[r];
+        atomicAdd(&lds()[64 + r], (unsigned long long)deg);
+      }
       const uint64_t b = __ballot(mine);
       if ((threadIdx.x & 63) == 0) atomicAdd(&lds()[r], (unsigned long long)__popcll(b));
     }
+    const uint64_t bs = __ballot((nw & sel) != 0);
+    if ((threadIdx.x & 63) == 0) nz_sel[v >> 6] = bs;
   }
   __device__ void end() const {
     __syncthreads();
-    if (threadIdx.x < 64 && lds()[threadIdx.x]) atomicAdd(lane_nq + threadIdx.x, lds()[threadIdx.x]);
+    if (threadIdx.x < 128 && lds()[threadIdx.x]) atomicAdd(lane_nq + threadIdx.x, lds()[threadIdx.x]);
   }
 };
 
@@ -128,7 +138,8 @@ __global__ void __launch_bounds__(kBlock) k_sv_clear(uint64_t* __restrict__ seen
 // Places the admitted roots into seen / frontier / nonzero bitmap and their
 // answers; one thread per lane (lanes may share a root vertex).
 __global__ void k_sv_admit(const uint32_t* __restrict__ old2new, int64_t rows, ServeRoots rs,
-                           uint64_t* __restrict__ seen, uint64_t* __restrict__ front, uint64_t* __restrict__ nz_front) {
+                           uint64_t* __restrict__ seen, uint64_t* __restrict__ front, uint64_t* __restrict__ nz_front,
+                           uint64_t* __restrict__ nz_sel_front) {
   const int r = threadIdx.x;
   if (r >= 64 || rs.root[r] < 0) return;
   const uint32_t v = old2new[rs.root[r]];
@@ -137,6 +148,7 @@ __global__ void k_sv_admit(const uint32_t* __restrict__ old2new, int64_t rows, S
   atomicOr((unsigned long long*)(seen + v), bit);
   atomicOr((unsigned long long*)(front + v), bit);
   atomicOr((unsigned long long*)(nz_front + (v >> 6)), 1ull << (v & 63));
+  atomicOr((unsigned long long*)(nz_sel_front + (v >> 6)), 1ull << (v & 63));
   rs.ans[r][v] = 0;
 }
 
@@ -199,11 +211,15 @@ struct Engine {
   int64_t n = 0, rows = 0, words = 0, nnz = 0;
   uint64_t nchunks = 0;
   DevBuf<uint64_t> seen, fr[2], nz[2];
+  DevBuf<uint64_t> nzs[2];  // nonzero bitmap restricted to the lanes that were top-down
+  bool nzs_valid = true;    // false after a lane turned back to top-down
   DevBuf<uint32_t> chunk_rows, chunk_list;
-  DevBuf<unsigned long long> cnt;    // M_NUM counters, chunk count, 64 per-lane new-vertex counts
+  DevBuf<unsigned long long> cnt;    // M_NUM counters, chunk count, 64 + 64 per-lane counts and degree sums
   DevBuf<unsigned long long> tacc;   // per-lane tuple-weight sums of a retirement
-  unsigned long long* h = nullptr;   // pinned: [0, M_NUM) counters, [16, 80) weight sums, [80, 144) lane counts
+  unsigned long long* h = nullptr;   // pinned: [0, M_NUM) counters, [16, 80) weight sums, [80, 208) lane counts
   hipEvent_t ev = nullptr;
+  hipEvent_t ev_start = nullptr, ev_end = nullptr;  // timing of the level in flight
+  g5_server_graph_stats st{};
   std::vector<std::unique_ptr<DevBuf<uint8_t>>> bufs;
   std::vector<int> free_bufs;
   int64_t lane_req[64];
@@ -226,6 +242,8 @@ struct Engine {
   ~Engine() {
     if (h) (void)hipHostFree(h);
     if (ev) (void)hipEventDestroy(ev);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_end) (void)hipEventDestroy(ev_end);
   }
 };
 
@@ -258,13 +276,16 @@ static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
   e.seen.alloc(e.rows);
   for (auto& b : e.fr) b.alloc(e.rows);
   for (auto& b : e.nz) b.alloc(e.words);
+  for (auto& b : e.nzs) b.alloc(e.words);
   e.nchunks = ((uint64_t)e.nnz + kSweepEpb - 1) / kSweepEpb;
   e.chunk_rows.alloc(2 * e.nchunks + 2);
   e.chunk_list.alloc(e.nchunks + 1);
-  e.cnt.alloc(M_NUM + 1 + 64);
+  e.cnt.alloc(M_NUM + 1 + 128);
   e.tacc.alloc(64);
-  G5_HIP(hipHostMalloc((void**)&e.h, 160 * sizeof(unsigned long long), hipHostMallocDefault));
+  G5_HIP(hipHostMalloc((void**)&e.h, 224 * sizeof(unsigned long long), hipHostMallocDefault));
   G5_HIP(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
+  G5_HIP(hipEventCreate(&e.ev_start));
+  G5_HIP(hipEventCreate(&e.ev_end));
   hipStream_t s = e.s;
   if (e.nchunks)
     hipLaunchKernelGGL(k_chunk_rows<kSweepEpb>, dim3(grid_of((int64_t)(2 * e.nchunks), kBlock, 65536)), dim3(kBlock),
@@ -272,6 +293,7 @@ static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
   hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, s, e.seen.p, e.rows);
   for (auto& b : e.fr) hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, s, b.p, e.rows);
   for (auto& b : e.nz) hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.words, kBlock, 4096)), dim3(kBlock), 0, s, b.p, e.words);
+  for (auto& b : e.nzs) hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.words, kBlock, 4096)), dim3(kBlock), 0, s, b.p, e.words);
   G5_HIP(hipGetLastError());
   const int nb = srv->cfg.answer_buffers;
   for (int i = 0; i < nb; ++i) {
@@ -316,14 +338,23 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
     const int r = __builtin_ctzll(m);
     m &= m - 1;
     const int64_t nq = (int64_t)e.h[80 + r];
+    const double fdeg = (double)e.h[144 + r];
     e.lane_visited[r] += nq;
     const int64_t unvisited = (int64_t)na - e.lane_visited[r];
-    if (e.lane_growing[r] && nq > e.lane_prev_nq[r]) {
+    const bool was_fwd = e.lane_fwd[r];
+    if (srv->cfg.lane_rule == 2) {
+      // one direction for all searches, from the frontier union's edges
+      e.lane_fwd[r] = (double)e.last_fdeg < (double)e.nnz / srv->cfg.alpha;
+    } else if (srv->cfg.lane_rule == 1) {
+      // edge rule: pull the searches whose frontier has many edges
+      e.lane_fwd[r] = fdeg < (double)e.nnz / srv->cfg.alpha;
+    } else if (e.lane_growing[r] && nq > e.lane_prev_nq[r]) {
       if (e.lane_fwd[r] && (double)nq > na / 2000.0) e.lane_fwd[r] = false;
     } else if (!e.lane_fwd[r] && unvisited < (int64_t)(na / 8.0)) {
       e.lane_fwd[r] = true;
       e.lane_growing[r] = false;
     }
+    if (e.lane_fwd[r] && !was_fwd) e.nzs_valid = false;  // its frontier is not in the top-down bitmap
     e.lane_prev_nq[r] = nq;
   }
   if (finished) {
@@ -344,6 +375,7 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
       q.status = SV_DONE;
       q.done = t;
       ++srv->done_count;
+      ++e.st.completed;
       q.levels = e.level - e.lane_l0[r];
       e.teps_req[r] = id;
       e.lane_req[r] = -1;
@@ -361,6 +393,13 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
 static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
   const uint64_t busy_lanes = e.active | e.cooling;
   if (e.waiting.empty() || busy_lanes == ~UINT64_C(0) || e.free_bufs.empty()) return 0;
+  if (e.active) {
+    // a busy graph waits for a cohort (or for the oldest query's patience)
+    const int64_t can = std::min<int64_t>(
+        {(int64_t)e.waiting.size(), (int64_t)(64 - __builtin_popcountll(busy_lanes)), (int64_t)e.free_bufs.size()});
+    const double waited = t - srv->req[e.waiting.front()].arrival;
+    if (can < srv->cfg.min_admit && waited < srv->cfg.max_wait_s) return 0;
+  }
   ServeRoots rs;
   for (int r = 0; r < 64; ++r) {
     rs.root[r] = -1;
@@ -392,7 +431,7 @@ static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
   }
   if (admitted) {
     hipLaunchKernelGGL(k_sv_admit, dim3(1), dim3(64), 0, e.s, e.ctx->graph.old2new.p, e.rows, rs, e.seen.p,
-                       e.fr[e.level & 1].p, e.nz[e.level & 1].p);
+                       e.fr[e.level & 1].p, e.nz[e.level & 1].p, e.nzs[e.level & 1].p);
     G5_HIP(hipGetLastError());
     e.active |= admitted;
   }
@@ -432,11 +471,21 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
   const uint64_t td = e.active & fwd, bu = e.active & ~fwd;
   unsigned long long* c = e.cnt.p;
   epi.lane_nq = c + M_NUM + 1;
-  G5_HIP(hipMemsetAsync(c, 0, (M_NUM + 1 + 64) * 8, s));
+  epi.offsets = g.offsets.p;
+  epi.nz_sel = e.nzs[cur ^ 1].p;
+  epi.sel = td;
+  // chunks to push: those holding a top-down lane's frontier vertex
+  const uint64_t* nz_push = e.nzs_valid ? e.nzs[cur].p : nz_front;
+  e.nzs_valid = true;
+  G5_HIP(hipEventRecord(e.ev_start, s));
+  G5_HIP(hipMemsetAsync(c, 0, (M_NUM + 1 + 128) * 8, s));
+  ++e.st.levels;
+  if (td && bu) ++e.st.mixed_levels; else if (td) ++e.st.push_levels; else ++e.st.pull_levels;
+  e.st.lane_levels += __builtin_popcountll(e.active);
   if (td) {
     hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, s, next, e.rows);
     hipLaunchKernelGGL(k_sv_collect, dim3(grid_of((int64_t)e.nchunks, kBlock, 4096)), dim3(kBlock), 0, s,
-                       e.chunk_rows.p, e.nchunks, nz_front, e.chunk_list.p, (unsigned*)(c + M_NUM));
+                       e.chunk_rows.p, e.nchunks, nz_push, e.chunk_list.p, (unsigned*)(c + M_NUM));
     hipLaunchKernelGGL(k_sv_push, dim3(grid_of((int64_t)e.nchunks, 1, 4096)), dim3(kBlock), 0, s, e.chunk_list.p,
                        (const unsigned*)(c + M_NUM), e.chunk_rows.p, (uint64_t)e.nnz, g.offsets.p, g.cols.p, front,
                        nz_front, e.seen.p, next, td);
@@ -451,7 +500,8 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
   }
   G5_HIP(hipGetLastError());
   G5_HIP(hipMemcpyAsync(e.h, c, M_NUM * 8, hipMemcpyDeviceToHost, s));
-  G5_HIP(hipMemcpyAsync(e.h + 80, c + M_NUM + 1, 64 * 8, hipMemcpyDeviceToHost, s));
+  G5_HIP(hipMemcpyAsync(e.h + 80, c + M_NUM + 1, 128 * 8, hipMemcpyDeviceToHost, s));
+  G5_HIP(hipEventRecord(e.ev_end, s));
   G5_HIP(hipEventRecord(e.ev, s));
   e.level = l;
   e.busy = true;
@@ -465,6 +515,8 @@ static bool sv_step(g5_server* srv, Engine& e, double t) {
     if (q == hipErrorNotReady) return false;
     G5_HIP(q);
     e.busy = false;
+    float lms = 0;
+    if (hipEventElapsedTime(&lms, e.ev_start, e.ev_end) == hipSuccess) e.st.busy_ms += lms;
     sv_complete_level(srv, e, t);
   }
   if (!e.active && e.cooling && !e.waiting.empty()) {
@@ -594,7 +646,9 @@ int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_conf
   if (!out || !graphs || num_graphs < 1 || num_graphs > 64 || !cfg) return G5_ERR_ARG;
   *out = nullptr;
   if (cfg->answer_buffers < 1 || cfg->answer_buffers > 4096 || !(cfg->alpha > 0) ||
-      (cfg->pull_filter != 0 && cfg->pull_filter != 1))
+      (cfg->pull_filter != 0 && cfg->pull_filter != 1) || cfg->min_admit < 1 || cfg->min_admit > 64 ||
+      cfg->lane_rule < 0 || cfg->lane_rule > 2 ||
+      !(cfg->max_wait_s >= 0))
     return G5_ERR_ARG;
   for (int i = 0; i < num_graphs; ++i) {
     if (!graphs[i] || graphs[i]->device != graphs[0]->device) return G5_ERR_ARG;
@@ -652,6 +706,13 @@ int g5_server_get_levels(g5_server* srv, int64_t request, uint8_t* levels) {
     G5_HIP(hipGetLastError());
     G5_HIP(hipMemcpyAsync(levels, srv->lv_out.p, e.n, hipMemcpyDeviceToHost, e.s));
     G5_HIP(hipStreamSynchronize(e.s));
+  });
+}
+
+int g5_server_graph_stats_get(g5_server* srv, int graph, g5_server_graph_stats* out) {
+  return sv_guarded(srv, [&] {
+    G5_CHECK(out && graph >= 0 && graph < (int)srv->eng.size(), "g5_server_graph_stats_get: bad arguments");
+    *out = srv->eng[graph]->st;
   });
 }
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "g5_batch_prepare", "g5_batch_set_tuning", "g5_batch_get_tuning", "g5_run_bfs_batch",
     "g5_batch_get_levels", "g5_batch_teps_edges", "g5_batch_stats_get", "g5_batch_model_bytes",
     "g5_batch_end", "g5_server_create", "g5_server_destroy", "g5_server_last_error", "g5_server_run",
-    "g5_server_get_levels", "g5_server_release",
+    "g5_server_get_levels", "g5_server_release", "g5_server_graph_stats_get",
 ]
 
 
@@ -95,7 +95,8 @@ class BatchStats(ctypes.Structure):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("answer_buffers", ctypes.c_int32), ("pull_filter", ctypes.c_int32), ("alpha", ctypes.c_double)]
+    _fields_ = [("answer_buffers", ctypes.c_int32), ("pull_filter", ctypes.c_int32), ("alpha", ctypes.c_double),
+                ("min_admit", ctypes.c_int32), ("lane_rule", ctypes.c_int32), ("max_wait_s", ctypes.c_double)]
 
 
 class Query(ctypes.Structure):
@@ -106,6 +107,12 @@ class Query(ctypes.Structure):
 class QueryResult(ctypes.Structure):
     _fields_ = [("request", ctypes.c_int64), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
                 ("edges", ctypes.c_int64), ("levels", ctypes.c_int32), ("status", ctypes.c_int32)]
+
+
+class ServerGraphStats(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int64), ("push_levels", ctypes.c_int64), ("pull_levels", ctypes.c_int64),
+                ("mixed_levels", ctypes.c_int64), ("lane_levels", ctypes.c_int64), ("busy_ms", ctypes.c_double),
+                ("completed", ctypes.c_int64)]
 
 
 class ServerReport(ctypes.Structure):
@@ -182,6 +189,7 @@ def load_library(path=LIB_PATH):
         "g5_server_run": ([vp, vp, i64, ctypes.c_double, vp, ctypes.POINTER(ServerReport)], i32),
         "g5_server_get_levels": ([vp, i64, vp], i32),
         "g5_server_release": ([vp, i64], i32),
+        "g5_server_graph_stats_get": ([vp, i32, ctypes.POINTER(ServerGraphStats)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -437,11 +445,15 @@ class G5Server:
     """Serving loop over graphs resident in HBM (g5_server_*): continuous
     batching of (graph, root) BFS queries, up to 64 in flight per graph."""
 
-    def __init__(self, contexts, answer_buffers=128, alpha=16.0, pull_filter=0):
+    def __init__(self, contexts, answer_buffers=128, alpha=64.0, pull_filter=0, min_admit=64, max_wait_s=0.3,
+                 lane_rule=1):
+        """Defaults: a busy graph admits new queries as a cohort of 64 or once
+        the oldest has waited 0.3 s; each search pulls while its frontier has
+        more than nnz/64 edges and pushes otherwise."""
         self._lib = load_library()
         self._ctxs = list(contexts)  # the server uses their graphs and streams
         arr = (ctypes.c_void_p * len(self._ctxs))(*[c._h for c in self._ctxs])
-        cfg = ServerConfig(answer_buffers, pull_filter, alpha)
+        cfg = ServerConfig(answer_buffers, pull_filter, alpha, min_admit, lane_rule, max_wait_s)
         h = ctypes.c_void_p()
         rc = self._lib.g5_server_create(arr, len(self._ctxs), ctypes.byref(cfg), ctypes.byref(h))
         if rc != G5_OK:
@@ -486,3 +498,8 @@ class G5Server:
 
     def release(self, request):
         self._check(self._lib.g5_server_release(self._h, int(request)))
+
+    def graph_stats(self, graph):
+        st = ServerGraphStats()
+        self._check(self._lib.g5_server_graph_stats_get(self._h, int(graph), ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerGraphStats._fields_}

@@ -235,7 +235,14 @@ typedef struct g5_server g5_server;
 typedef struct g5_server_config {
   int32_t answer_buffers;  /* answer arrays preallocated per graph (HBM admission budget) */
   int32_t pull_filter;     /* as g5_batch_tuning.pull_filter */
-  double alpha;            /* push while the frontier union has fewer than nnz/alpha edges */
+  double alpha;            /* lane_rule 1: a search pulls when its frontier has more than nnz/alpha
+                              edges, else pushes */
+  int32_t min_admit;       /* a busy graph admits only when at least this many queries can start
+                              together (searches started together share their sweeps) ... */
+  int32_t lane_rule;       /* 0: each search switches direction by the reference rule on its own
+                              frontier size (mpi/bfs.hpp:2799-2829); 1: by its frontier's edges;
+                              2: all searches together, by the frontier union's edges */
+  double max_wait_s;       /* ... or when its oldest waiting query has waited this long */
 } g5_server_config;
 
 typedef struct g5_query {
@@ -263,7 +270,19 @@ typedef struct g5_server_report {
   int32_t reserved;
 } g5_server_report;
 
+/* Per-graph engine counters since the server was created. */
+typedef struct g5_server_graph_stats {
+  int64_t levels;         /* levels launched */
+  int64_t push_levels;    /* levels with top-down lanes only */
+  int64_t pull_levels;    /* levels with bottom-up lanes only */
+  int64_t mixed_levels;   /* levels with both */
+  int64_t lane_levels;    /* sum over levels of the active lanes */
+  double busy_ms;         /* device time of the levels (HIP events) */
+  int64_t completed;      /* queries answered */
+} g5_server_graph_stats;
+
 int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_config* config, g5_server** out);
+int g5_server_graph_stats_get(g5_server* server, int graph, g5_server_graph_stats* out);
 void g5_server_destroy(g5_server* server);
 const char* g5_server_last_error(const g5_server* server);
 /* Open-loop run: query i is submitted at arrival_s; returns when every query

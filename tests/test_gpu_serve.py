@@ -55,7 +55,8 @@ def _queries(specs):
     return q
 
 
-def test_serve_answers_equal_single_root(graphs):
+@pytest.mark.parametrize("lane_rule", [0, 1, 2])
+def test_serve_answers_equal_single_root(graphs, lane_rule):
     import graph500_amd
     rng = np.random.default_rng(7)
     specs, t = [], 0.0
@@ -63,7 +64,7 @@ def test_serve_answers_equal_single_root(graphs):
         g = int(rng.integers(0, 3))
         t += float(rng.exponential(2e-4))
         specs.append((g, _roots_with_edges(graphs[g], 1, rng)[0], t, 1))
-    with graph500_amd.G5Server(graphs) as srv:
+    with graph500_amd.G5Server(graphs, lane_rule=lane_rule) as srv:
         say("serve 160 queries")
         res, rep = srv.run(_queries(specs), timeout_s=120)
         assert rep["completed"] == 160
@@ -106,6 +107,27 @@ def test_alone_and_in_the_mix_are_identical(graphs):
             lv = srv.levels(res["request"][i], graphs[g].pred_size())
             assert np.array_equal(lv, alone[k][0]), (k, g, r)
             assert res["edges"][i] == alone[k][1]
+            srv.release(res["request"][i])
+
+
+def test_cohort_admission_keeps_answers(graphs):
+    """With cohort admission (a busy graph waits for 16 free lanes or 2 ms)
+    answers stay exact."""
+    import graph500_amd
+    rng = np.random.default_rng(5)
+    specs, t = [], 0.0
+    for i in range(200):
+        t += float(rng.exponential(5e-5))
+        g = int(rng.integers(0, 3))
+        specs.append((g, _roots_with_edges(graphs[g], 1, rng)[0], t, 1 if i % 10 == 0 else 0))
+    with graph500_amd.G5Server(graphs, min_admit=16, max_wait_s=0.002) as srv:
+        res, _ = srv.run(_queries(specs), timeout_s=120)
+        assert (res["status"] >= 2).all()
+        for i in range(0, 200, 10):
+            g, r = specs[i][0], specs[i][1]
+            lv, e = _single(graphs[g], r)
+            assert np.array_equal(srv.levels(res["request"][i], graphs[g].pred_size()), lv), i
+            assert res["edges"][i] == e
             srv.release(res["request"][i])
 
 
