@@ -26,9 +26,10 @@ DEFAULT_CACHE = os.environ.get("G5_TUNING_CACHE",
 SPACE = [
     ("persistent", [0, 1]),
     ("sweep_below", [0.0, 0.25]),
-    ("alpha", [4.0, 8.0, 16.0, 32.0, 64.0]),
+    ("alpha", [1.0, 2.0, 4.0, 8.0, 16.0, 64.0]),
     ("beta", [0.0, 20.0, 50.0, 200.0]),
-    ("pull_filter", [0, 1]),
+    ("pull_filter", [0, 1, 2]),
+    ("filter_from", [0.001, 0.01, 0.05]),
     ("compact_below", [0.0, 0.25, 0.5]),
     ("pull_grid", [4096, 8192, 16384]),
     ("persist_waves", [4, 6, 8]),

@@ -384,7 +384,8 @@ int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* t) {
                  t->persist_levels >= 2 && t->persist_levels <= G5_MAX_LEVELS - 2 && t->persist_grid >= 0 &&
                  t->persist_grid <= (1 << 20) &&
                  (t->persist_waves == 4 || t->persist_waves == 6 || t->persist_waves == 8) &&
-                 (t->pull_filter == 0 || t->pull_filter == 1) && t->beta >= 0 && t->compact_below >= 0,
+                 t->pull_filter >= 0 && t->pull_filter <= 2 && t->beta >= 0 && t->compact_below >= 0 &&
+                 t->filter_from >= 0 && t->filter_from <= 1,
              "g5_batch_set_tuning: bad tuning");
     ctx->ms.persistent = t->persistent;
     ctx->ms.persist_levels = t->persist_levels;
@@ -393,6 +394,7 @@ int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* t) {
     ctx->ms.pull_filter = t->pull_filter;
     ctx->ms.beta = t->beta;
     ctx->ms.compact_below = t->compact_below;
+    ctx->ms.filter_from = t->filter_from;
     ctx->ms.alpha = t->alpha;
     ctx->ms.pull_grid = t->pull_grid;
     ctx->ms.sweep_below = t->sweep_below;
@@ -414,6 +416,7 @@ int g5_batch_get_tuning(g5_ctx* ctx, g5_batch_tuning* t) {
     t->pull_filter = ctx->ms.pull_filter;
     t->beta = ctx->ms.beta;
     t->compact_below = ctx->ms.compact_below;
+    t->filter_from = ctx->ms.filter_from;
   });
 }
 

@@ -94,7 +94,7 @@ struct MsState {
   int nroots = 0;
   int nlevels = 0;
   std::vector<int64_t> host_roots;
-  double alpha = 16.0;       // push while the frontier union has < nnz/alpha edges
+  double alpha = 4.0;        // push while the frontier union has < nnz/alpha edges
   int64_t pull_grid = 8192;  // workgroups of the vertex-parallel pull
   double sweep_below = 0.0;  // edge-parallel sweep while reached pairs < this x roots x rows
   int64_t sweep_grid = 4096; // workgroups of the edge-parallel sweep
@@ -105,8 +105,10 @@ struct MsState {
   int persistent = 0;
   int persist_levels = 16;     // frontier-word arrays preallocated for it
   int64_t persist_grid = 0;    // 0: every co-resident workgroup
-  int persist_waves = 8;       // register budget of the persistent kernel: 4, 6 or 8 waves per SIMD
-  int pull_filter = 0;         // pull tests the nonzero bitmap before a frontier word
+  int persist_waves = 4;       // register budget of the persistent kernel: 4, 6 or 8 waves per SIMD
+  int pull_filter = 0;         // pull tests the nonzero bitmap before a frontier word: 0 never, 1 always,
+                               // 2 for vertices from filter_from x rows on
+  double filter_from = 0.01;
   double beta = 50.0;          // pull whenever fewer than 1/beta of the (root, vertex) pairs are unreached
   double compact_below = 0.0;  // pull from a compact frontier when it spans fewer than this x rows vertices
   DevBuf<uint64_t> pack, compact;
@@ -119,7 +121,7 @@ struct MsState {
   unsigned p_grid = 0;
   int clock_khz = 100000;
   DevBuf<uint64_t*> p_fr;
-  DevBuf<uint64_t> p_cnt, p_misc;
+  DevBuf<uint64_t> p_cnt, p_misc, p_bar;
   DevBuf<int64_t> p_ts;
   DevBuf<int> p_dir;
   uint64_t* h_persist = nullptr;
@@ -160,6 +162,7 @@ struct MsState {
     p_fr.release();
     p_cnt.release();
     p_misc.release();
+    p_bar.release();
     p_ts.release();
     p_dir.release();
     if (h_persist) (void)hipHostFree(h_persist);

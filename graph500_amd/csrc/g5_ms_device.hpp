@@ -131,7 +131,9 @@ __device__ __forceinline__ void ms_finish_body(uint64_t* __restrict__ seen, uint
 //                  compact[rank + popcount(bits below w)]; the compact array
 //                  holds only the frontier's words, small enough at the
 //                  sparse levels to stay in the last-level cache
-enum { kFrontPlain = 0, kFrontFilter = 1, kFrontCompact = 2 };
+//   kFrontHybrid   plain for w < filter_from (the high-degree vertices, whose
+//                  words are often in the frontier and in cache), filtered above
+enum { kFrontPlain = 0, kFrontFilter = 1, kFrontCompact = 2, kFrontHybrid = 3 };
 
 struct FrontView {
   const uint64_t* front;
@@ -139,6 +141,7 @@ struct FrontView {
   const uint64_t* pack;
   const uint64_t* compact;
   int mode;
+  uint32_t filter_from;
 };
 
 __device__ __forceinline__ uint64_t front_of(const FrontView& f, uint32_t w) {
@@ -149,6 +152,7 @@ __device__ __forceinline__ uint64_t front_of(const FrontView& f, uint32_t w) {
     return f.compact[f.pack[2 * (w >> 6) + 1] + (uint64_t)__popcll(below)];
   }
   if (f.mode == kFrontFilter && !bit_set(f.nz, w)) return 0;
+  if (f.mode == kFrontHybrid && w >= f.filter_from && !bit_set(f.nz, w)) return 0;
   return f.front[w];
 }
 

@@ -109,9 +109,9 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ cols, int64_t rows,
                                                     uint64_t live_in, int mode, const uint64_t* __restrict__ pack,
-                                                    const uint64_t* __restrict__ compact,
+                                                    const uint64_t* __restrict__ compact, uint32_t filter_from,
                                                     unsigned long long* __restrict__ c) {
-  const FrontView fv{front, nz_front, pack, compact, mode};
+  const FrontView fv{front, nz_front, pack, compact, mode, filter_from};
   ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x);
 }
 
@@ -139,11 +139,19 @@ __global__ void __launch_bounds__(kBlock) k_ms_sweep(const uint32_t* __restrict_
 // arrays run out (the host loop continues from there), or when a barrier wait
 // exceeds its deadline (then every block leaves and the host reports it).
 // ---------------------------------------------------------------------------
+// Two-level arrival: workgroups count in on one of up to kBarMaxGroups group
+// counters (kBarGroup each, one 128-byte line apiece), the last of a group
+// counts in on the top counter, and the last group flips the generation that
+// everyone waits on; this keeps the atomics on any one address to a few tens
+// per barrier instead of one per workgroup.
+constexpr unsigned kBarGroup = 16;
+constexpr unsigned kBarMaxGroups = 1024;
 struct GridBarrier {
   unsigned count;
   unsigned gen;
   unsigned abort;
-  unsigned pad;
+  unsigned pad[29];
+  unsigned group[kBarMaxGroups][32];
 };
 
 // Returns false (in every thread of the block) once any block has given up.
@@ -154,21 +162,31 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned nblk, long
     int ok = 1;
     const unsigned g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __threadfence();
-    if (__hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nblk - 1) {
+    const unsigned grp = blockIdx.x / kBarGroup;
+    const unsigned ngroups = (nblk + kBarGroup - 1) / kBarGroup;
+    const unsigned gsize = (grp + 1 < ngroups) ? kBarGroup : nblk - grp * kBarGroup;
+    bool last = false;
+    if (__hip_atomic_fetch_add(&b->group[grp][0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1) {
+      __hip_atomic_store(&b->group[grp][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == ngroups - 1;
+    }
+    if (last) {
       __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+      // relaxed polls (an acquire per poll would invalidate the caches the
+      // still-working workgroups depend on); one acquire fence after
+      while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
         if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
             wall_clock64() > deadline) {
           __hip_atomic_store(&b->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ok = 0;
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(8);
       }
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
     s_ok = ok;
   }
@@ -248,7 +266,7 @@ k_ms_persistent(MsPersistArgs a) {
       if (!grid_barrier(a.bar, nblk, deadline)) return;
       ms_finish_body(a.seen, next, nz_next, a.offsets, a.rows, c, bid, nblk);
     } else {
-      const FrontView fv{front, nz_front, nullptr, nullptr, a.pull_filter ? kFrontFilter : kFrontPlain};
+      const FrontView fv{front, nz_front, nullptr, nullptr, a.pull_filter == 1 ? kFrontFilter : kFrontPlain, 0};
       ms_pull_body(a.seen, fv, next, nz_next, a.first_nbr, a.offsets, a.cols, a.rows, live, c, bid, nblk);
     }
     if (!grid_barrier(a.bar, nblk, deadline)) return;
@@ -444,7 +462,8 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       hipLaunchKernelGGL(k_ms_finish, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
                          next, nz_next, g.offsets.p, rows, cnt);
     } else {
-      int mode = m.pull_filter ? kFrontFilter : kFrontPlain;
+      int mode = m.pull_filter == 1 ? kFrontFilter : (m.pull_filter == 2 ? kFrontHybrid : kFrontPlain);
+      const uint32_t filter_from = (uint32_t)std::min<double>((double)rows, m.filter_from * (double)rows);
       if (m.compact_below > 0 && (double)nz < m.compact_below * (double)rows) {
         // frontier words of few vertices: pack them densely
         m.pack.alloc(2 * words);
@@ -455,7 +474,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       }
       hipLaunchKernelGGL(k_ms_pull, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
                          m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows,
-                         live, mode, m.pack.p, m.compact.p, cnt);
+                         live, mode, m.pack.p, m.compact.p, filter_from, cnt);
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(m.lev_end[l], s));
@@ -499,7 +518,8 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   m.p_cnt.alloc((size_t)K * M_NUM);
   m.p_ts.alloc(K);
   m.p_dir.alloc(K);
-  m.p_misc.alloc(4);  // [0]: done level, [1..2]: grid barrier, [3]: active chunk count
+  m.p_misc.alloc(4);  // [0]: done level, [3]: active chunk count
+  m.p_bar.alloc(sizeof(GridBarrier) / 8);
   m.chunk_list.alloc(m.sweep_chunks);
   if (!m.h_persist) {
     // counters [levels][M_NUM] | clock [levels] | done level + barrier (4) | roots (64)
@@ -518,6 +538,7 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   G5_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
   G5_CHECK(per_cu >= 1, "g5_run_bfs_batch: persistent kernel does not fit a compute unit");
   int64_t grid = (int64_t)per_cu * cus;
+  G5_CHECK(grid <= (int64_t)kBarGroup * kBarMaxGroups, "g5_run_bfs_batch: persistent grid beyond the barrier's groups");
   if (m.persist_grid > 0 && m.persist_grid < grid) grid = m.persist_grid;
   m.p_grid = (unsigned)grid;
   int rate_khz = 0;
@@ -540,7 +561,7 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   a.ts = (long long*)m.p_ts.p;
   a.dir = (int*)m.p_dir.p;
   a.done_levels = (int*)m.p_misc.p;
-  a.bar = (GridBarrier*)(m.p_misc.p + 1);
+  a.bar = (GridBarrier*)m.p_bar.p;
   a.chunk_list = m.chunk_list.p;
   a.chunk_count = (unsigned*)(m.p_misc.p + 3);
   a.push_limit = (double)g.nnz / m.alpha;
@@ -558,6 +579,8 @@ static void ms_persistent_setup(g5_ctx* ctx) {
   hipLaunchKernelGGL(k_ms_fill, dim3(grid_of((int64_t)K * M_NUM, kBlock, 64)), dim3(kBlock), 0, s, m.p_cnt.p,
                      (int64_t)K * M_NUM);
   hipLaunchKernelGGL(k_ms_fill, dim3(1), dim3(kBlock), 0, s, m.p_misc.p, (int64_t)4);
+  hipLaunchKernelGGL(k_ms_fill, dim3(grid_of((int64_t)(sizeof(GridBarrier) / 8), kBlock, 64)), dim3(kBlock), 0, s,
+                     m.p_bar.p, (int64_t)(sizeof(GridBarrier) / 8));
   hipLaunchKernelGGL(k_ms_clear, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, m.seen.p, lev[0], rows);
   hipLaunchKernelGGL(k_ms_fill, dim3(grid_of(words, kBlock, 1024)), dim3(kBlock), 0, s, m.nz[0].p, words);
   hipLaunchKernelGGL(k_ms_roots, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, 64, rows, g.offsets.p, m.seen.p,
@@ -625,10 +648,12 @@ static void ms_run_persistent(g5_ctx* ctx, const int64_t* roots, int nroots) {
   G5_HIP(hipMemcpyAsync(h_cnt, m.p_cnt.p, (size_t)K * M_NUM * 8, hipMemcpyDeviceToHost, s));
   G5_HIP(hipMemcpyAsync(h_ts, m.p_ts.p, (size_t)K * 8, hipMemcpyDeviceToHost, s));
   G5_HIP(hipMemcpyAsync(h_misc, m.p_misc.p, 4 * 8, hipMemcpyDeviceToHost, s));
+  GridBarrier* hb = (GridBarrier*)(h_misc + 4);
+  G5_HIP(hipMemcpyAsync(hb, m.p_bar.p, 16, hipMemcpyDeviceToHost, s));
   std::vector<int> h_dir(K);
   G5_HIP(hipMemcpyAsync(h_dir.data(), m.p_dir.p, (size_t)K * sizeof(int), hipMemcpyDeviceToHost, s));
   G5_HIP(hipStreamSynchronize(s));
-  const GridBarrier* hb = (const GridBarrier*)(h_misc + 1);
+
   G5_CHECK_CODE(!hb->abort, G5_ERR_HIP,
                 "g5_run_bfs_batch: a grid barrier of the persistent kernel timed out (blocks not co-resident?)");
   const int done = (int)(int32_t)(h_misc[0] & 0xFFFFFFFFu);

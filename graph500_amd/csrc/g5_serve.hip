@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(kBlock) k_sv_pull(uint64_t* __restrict__ seen,
                                                     const uint32_t* __restrict__ cols, int64_t rows, uint64_t live_in,
                                                     int filter, unsigned long long* __restrict__ c, uint64_t keep,
                                                     int merge, ServeEpi epi) {
-  const FrontView fv{front, nz_front, nullptr, nullptr, filter ? kFrontFilter : kFrontPlain};
+  const FrontView fv{front, nz_front, nullptr, nullptr, filter ? kFrontFilter : kFrontPlain, 0};
   ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x, keep, epi,
                merge != 0);
 }

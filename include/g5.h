@@ -96,9 +96,11 @@ typedef struct g5_batch_tuning {
   int64_t persist_grid;    /* workgroups of the persistent launch; 0 = all co-resident ones */
   int32_t persist_waves;   /* register budget of the persistent kernel: 4, 6 or 8 waves per SIMD */
   int32_t pull_filter;     /* 1: the pull reads a neighbour's frontier word only when its bit in a
-                              1-bit-per-vertex "frontier word is not zero" map is set */
+                              1-bit-per-vertex "frontier word is not zero" map is set; 2: only
+                              for neighbours from filter_from x rows on (the lower-degree ones) */
   double beta;             /* pull (never push) once fewer than 1/beta of the (root, vertex) pairs
                               are unreached; 0 disables */
+  double filter_from;      /* pull_filter 2 threshold, a fraction of the rows */
   double compact_below;    /* per-level pulls (persistent = 0) read frontier words from a compact
                               copy holding only the frontier's words when the frontier spans fewer
                               than compact_below x rows vertices; 0 disables */

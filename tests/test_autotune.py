@@ -14,7 +14,7 @@ class FakeCtx:
     def __init__(self, bad=None):
         self.t = {"alpha": 16.0, "pull_grid": 8192, "sweep_below": 0.25, "sweep_grid": 4096, "persistent": 0,
                   "persist_levels": 16, "persist_grid": 0, "persist_waves": 8, "pull_filter": 0, "beta": 50.0,
-                  "compact_below": 0.0}
+                  "compact_below": 0.0, "filter_from": 0.01}
         self.bad = bad
         self.runs = 0
 
@@ -35,7 +35,7 @@ class FakeCtx:
 
     def batch_stats(self):
         t = self.t
-        ms = 10.0 + abs(t["alpha"] - 32.0) / 8 + (0 if t["beta"] == 20.0 else 1) + t["pull_filter"] \
+        ms = 10.0 + abs(t["alpha"] - 2.0) / 8 + (0 if t["beta"] == 20.0 else 1) + t["pull_filter"] \
             + (2 if t["persistent"] else 0) + (0 if t["pull_grid"] == 16384 else 0.5)
         return {"device_ms": ms}
 
@@ -49,7 +49,7 @@ class FakeCtx:
 def test_descent_finds_the_best_per_knob(tmp_path):
     c = FakeCtx()
     res = autotune.autotune(c, [1, 2], reps=1, cache=str(tmp_path / "t.json"))
-    assert res["tuning"]["alpha"] == 32.0
+    assert res["tuning"]["alpha"] == 2.0
     assert res["tuning"]["beta"] == 20.0
     assert res["tuning"]["pull_filter"] == 0
     assert res["tuning"]["persistent"] == 0

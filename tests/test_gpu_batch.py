@@ -43,7 +43,8 @@ def s16ctx():
 # graph-replayed kernel and through per-level launches
 @pytest.mark.parametrize("persistent", [1, 0])
 @pytest.mark.parametrize("alpha,sweep,filt,beta", [(16.0, 0.25, 0, 50.0), (16.0, 0.25, 1, 0.0), (1e12, 0.0, 0, 0.0),
-                                                   (1e12, 0.0, 1, 0.0), (1e12, 2.0, 0, 0.0), (1e-9, 0.25, 0, 0.0)])
+                                                   (1e12, 0.0, 1, 0.0), (1e12, 0.0, 2, 0.0), (1e12, 2.0, 0, 0.0),
+                                                   (1e-9, 0.25, 0, 0.0)])
 def test_batch_64_roots_match_reference_s16(s16ctx, alpha, sweep, filt, beta, persistent):
     g = gold("ref_s16_64roots.json")
     roots = [p["root"] for p in g["per_root"]]
