@@ -32,6 +32,7 @@ SPACE = [
     ("filter_from", [0.001, 0.01, 0.05]),
     ("compact_below", [0.0, 0.25, 0.5]),
     ("pull_grid", [4096, 8192, 16384]),
+    ("pull_unroll", [0, 1, 2, 3, 4]),
     ("persist_waves", [4, 6, 8]),
 ]
 

@@ -109,6 +109,7 @@ struct MsState {
   int pull_filter = 0;         // pull tests the nonzero bitmap before a frontier word: 0 never, 1 always,
                                // 2 for vertices from filter_from x rows on
   double filter_from = 0.01;
+  int pull_unroll = 0;         // probe unrolling of the per-level pull (0..4, see g5_msbfs.hip)
   double beta = 50.0;          // pull whenever fewer than 1/beta of the (root, vertex) pairs are unreached
   double compact_below = 0.0;  // pull from a compact frontier when it spans fewer than this x rows vertices
   DevBuf<uint64_t> pack, compact;

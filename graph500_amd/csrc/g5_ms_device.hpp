@@ -177,7 +177,9 @@ struct CompactEmit {
   }
 };
 
-template <class Epi = NoEpilogue>
+// kU: independent probes per step of a lane's short row; kW: entries per lane
+// and step of the wave's scan of a long row (kW x 64 per step).
+template <class Epi = NoEpilogue, int kU = 4, int kW = 2>
 __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const FrontView fv,
                                              uint64_t* __restrict__ next, uint64_t* __restrict__ nz_next,
                                              const uint32_t* __restrict__ first_nbr,
@@ -213,13 +215,16 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
       }
     }
     if (more && re - rb <= (uint64_t)kMsShortRow) {
-      // four independent probes per step (entries past the row repeat the first)
-      for (uint64_t k = rb; k < re; k += 4) {
-        const uint32_t w0 = cols[k];
-        const uint32_t w1 = k + 1 < re ? cols[k + 1] : w0;
-        const uint32_t w2 = k + 2 < re ? cols[k + 2] : w0;
-        const uint32_t w3 = k + 3 < re ? cols[k + 3] : w0;
-        acc |= front_of(fv, w0) | front_of(fv, w1) | front_of(fv, w2) | front_of(fv, w3);
+      // kU independent probes per step (entries past the row repeat the first)
+      for (uint64_t k = rb; k < re; k += kU) {
+        uint32_t w[kU];
+        w[0] = cols[k];
+#pragma unroll
+        for (int j = 1; j < kU; ++j) w[j] = k + j < re ? cols[k + j] : w[0];
+        uint64_t x = 0;
+#pragma unroll
+        for (int j = 0; j < kU; ++j) x |= front_of(fv, w[j]);
+        acc |= x;
         if ((acc & need) == need) break;
       }
       more = false;
@@ -231,14 +236,15 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
       const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
       const uint64_t nd = __shfl(need, src, 64);
       uint64_t a = __shfl(acc, src, 64);
-      // 128 entries per step, two independent probes per lane
-      for (uint64_t base = b; base < e; base += 128) {
-        const uint64_t k = base + lane;
-        const uint32_t w0 = k < e ? cols[k] : 0;
-        const uint32_t w1 = k + 64 < e ? cols[k + 64] : 0;
+      // 64 kW entries per step, kW independent probes per lane
+      for (uint64_t base = b; base < e; base += 64 * kW) {
+        uint32_t w[kW];
+#pragma unroll
+        for (int j = 0; j < kW; ++j) w[j] = base + lane + 64 * j < e ? cols[base + lane + 64 * j] : 0;
         uint64_t x = 0;
-        if (k < e) x = front_of(fv, w0);
-        if (k + 64 < e) x |= front_of(fv, w1);
+#pragma unroll
+        for (int j = 0; j < kW; ++j)
+          if (base + lane + 64 * j < e) x |= front_of(fv, w[j]);
         a |= wave_or(x);
         if ((a & nd) == nd) break;
       }

@@ -102,6 +102,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_finish(uint64_t* __restrict__ see
   ms_finish_body(seen, next, nz_next, offsets, rows, c, blockIdx.x, gridDim.x);
 }
 
+template <int kU, int kW>
 __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
                                                     const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
                                                     uint64_t* __restrict__ nz_next,
@@ -112,7 +113,8 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
                                                     const uint64_t* __restrict__ compact, uint32_t filter_from,
                                                     unsigned long long* __restrict__ c) {
   const FrontView fv{front, nz_front, pack, compact, mode, filter_from};
-  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x);
+  ms_pull_body<NoEpilogue, kU, kW>(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x,
+                                   gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_ms_sweep_clear(const uint32_t* __restrict__ chunk_rows,
@@ -472,9 +474,18 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
                     m.scan_tmp.p, s);
         mode = kFrontCompact;
       }
-      hipLaunchKernelGGL(k_ms_pull, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
-                         m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows,
-                         live, mode, m.pack.p, m.compact.p, filter_from, cnt);
+      // probe unrolling: (short-row probes per step, long-row probes per lane and step)
+      decltype(&k_ms_pull<4, 2>) kp = k_ms_pull<4, 2>;
+      switch (m.pull_unroll) {
+        case 1: kp = k_ms_pull<8, 2>; break;
+        case 2: kp = k_ms_pull<8, 4>; break;
+        case 3: kp = k_ms_pull<4, 4>; break;
+        case 4: kp = k_ms_pull<2, 1>; break;
+        default: break;
+      }
+      hipLaunchKernelGGL(kp, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s, m.seen.p, front,
+                         nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, live, mode, m.pack.p,
+                         m.compact.p, filter_from, cnt);
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(m.lev_end[l], s));

@@ -130,7 +130,8 @@ class BatchTuning(ctypes.Structure):
     _fields_ = [("alpha", ctypes.c_double), ("pull_grid", ctypes.c_int64), ("sweep_below", ctypes.c_double),
                 ("sweep_grid", ctypes.c_int64), ("persistent", ctypes.c_int32), ("persist_levels", ctypes.c_int32),
                 ("persist_grid", ctypes.c_int64), ("persist_waves", ctypes.c_int32), ("pull_filter", ctypes.c_int32),
-                ("beta", ctypes.c_double), ("filter_from", ctypes.c_double), ("compact_below", ctypes.c_double)]
+                ("beta", ctypes.c_double), ("filter_from", ctypes.c_double), ("pull_unroll", ctypes.c_int32),
+                ("reserved2", ctypes.c_int32), ("compact_below", ctypes.c_double)]
 
 
 _lib = None
@@ -373,7 +374,8 @@ class G5Context:
         self._check(self._lib.g5_batch_prepare(self._h))
 
     TUNING_FIELDS = ("alpha", "pull_grid", "sweep_below", "sweep_grid", "persistent", "persist_levels",
-                     "persist_grid", "persist_waves", "pull_filter", "beta", "filter_from", "compact_below")
+                     "persist_grid", "persist_waves", "pull_filter", "beta", "filter_from", "pull_unroll",
+                     "compact_below")
 
     def batch_set_tuning(self, **kw):
         """Sets any of TUNING_FIELDS; the others keep their values."""

@@ -101,6 +101,9 @@ typedef struct g5_batch_tuning {
   double beta;             /* pull (never push) once fewer than 1/beta of the (root, vertex) pairs
                               are unreached; 0 disables */
   double filter_from;      /* pull_filter 2 threshold, a fraction of the rows */
+  int32_t pull_unroll;     /* probes in flight per lane in the per-level pull: 0 = 4 per short-row step
+                              and 2 per long-row step, 1 = 8/2, 2 = 8/4, 3 = 4/4, 4 = 2/1 */
+  int32_t reserved2;
   double compact_below;    /* per-level pulls (persistent = 0) read frontier words from a compact
                               copy holding only the frontier's words when the frontier spans fewer
                               than compact_below x rows vertices; 0 disables */

@@ -106,3 +106,33 @@ def test_driver_reports_missing_gpu():
         pytest.skip("a GPU is present")
     r = subprocess.run([build.DRIVER, "10"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "g5_create" in r.stderr
+
+
+STRUCTS = {"GraphInfo": "g5_graph_info", "BfsStats": "g5_bfs_stats", "BatchStats": "g5_batch_stats",
+           "BatchTuning": "g5_batch_tuning", "ServerConfig": "g5_server_config", "Query": "g5_query",
+           "QueryResult": "g5_query_result", "ServerReport": "g5_server_report",
+           "ServerGraphStats": "g5_server_graph_stats"}
+
+
+def test_binding_struct_layouts_match_header(tmp_path):
+    """Every ctypes structure has the field offsets and size of its C struct."""
+    import subprocess
+    from graph500_amd import g5
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "g5.h"', "int main(void) {"]
+    expect = []
+    for py, c in STRUCTS.items():
+        S = getattr(g5, py)
+        for name, _ in S._fields_:
+            lines.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (c, name, c, name))
+            expect.append("%s.%s %d" % (c, name, getattr(S, name).offset))
+        lines.append('  printf("%s.size %%zu\\n", sizeof(%s));' % (c, c))
+        expect.append("%s.size %d" % (c, ctypes.sizeof(S)))
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n")
+    assert [l for l in got if l] == expect
