@@ -23,8 +23,8 @@ around run_bfs; GTEPS = 1 / mean(t_i / e_i) with e_i the validator's
 traversed-edge count.
 
 `roofline` is for the dominant kernel of the step (the kind of batch level
-with the most device time: push, vertex-parallel pull or edge-parallel
-sweep): algorithmic bytes of its launches (byte model evaluated on the
+with the most device time: push, vertex-parallel pull, edge-parallel sweep,
+or a level where some roots push and the others pull): algorithmic bytes of its launches (byte model evaluated on the
 GPU from the final levels, outside timing; equal to the CPU oracle's count,
 see tests) over their HIP-event durations on the library stream.  `traffic`
 is the HBM bytes per launch measured by rocprof (FETCH_SIZE + WRITE_SIZE,
@@ -64,6 +64,8 @@ BATCH_KERNELS = {
     0: ("k_ms_pull", "k_ms_pull (batched vertex-parallel pull)"),
     1: ("k_ms_push", "k_ms_push + k_ms_finish (batched push)"),
     2: ("k_ms_sweep", "k_ms_sweep + k_ms_finish (batched edge-parallel pull sweep)"),
+    3: ("k_ms_mixed", "k_ms_select + k_ms_collect + k_ms_push_list + k_ms_pull_lanes (batched level with "
+                      "per-root directions: the pushing roots, then the pulling roots)"),
 }
 
 
@@ -400,7 +402,7 @@ def main():
             "traversed_edges_sum": total_edges,
             "bytes_per_edge": round(tot / total_edges, 4),
             "roofline_frac_whole_batch": round((tot / (sum(dev) / len(dev))) / (HBM_PEAK_GBS * 1e9), 4),
-            "levels": "".join("UPS"[d] for d in st["direction"]),
+            "levels": "".join("UPSM"[d] for d in st["direction"]),
             "level_ms": [round(x, 3) for x in st["level_ms"]],
             "tuning": ctx.batch_tuning(),
             "autotune": ({"trials": tuned["trials"], "seconds": tuned["seconds"]} if tuned else None),

@@ -22,11 +22,14 @@ import time
 DEFAULT_CACHE = os.environ.get("G5_TUNING_CACHE",
                                os.path.join(os.path.expanduser("~"), ".cache", "graph500_amd", "tuning.json"))
 
-# knob -> candidate values, in the order the descent visits them
+# knob -> candidate values, in the order the descent visits them.  A knob
+# "a,b" takes pairs: the direction rule and its alpha are tried together, as
+# alpha means a different threshold under each rule.
 SPACE = [
+    ("lane_rule,alpha", [(0, 1.0), (0, 2.0), (0, 4.0), (0, 8.0), (0, 16.0), (0, 64.0),
+                         (1, 32.0), (1, 64.0), (1, 128.0), (1, 256.0)]),
     ("persistent", [0, 1]),
     ("sweep_below", [0.0, 0.25]),
-    ("alpha", [1.0, 2.0, 4.0, 8.0, 16.0, 64.0]),
     ("beta", [0.0, 20.0, 50.0, 200.0]),
     ("pull_filter", [0, 1, 2]),
     ("filter_from", [0.001, 0.01, 0.05]),
@@ -90,10 +93,12 @@ def autotune(ctx, roots, reps=2, budget_s=60.0, cache=DEFAULT_CACHE, use_cache=T
     for knob, values in SPACE:
         if knob == "persist_waves" and not cur["persistent"]:
             continue
+        names = knob.split(",")
         for v in values:
-            if v == cur[knob] or time.time() - t0 > budget_s:
+            setting = dict(zip(names, v if len(names) > 1 else (v,)))
+            if all(cur[k] == x for k, x in setting.items()) or time.time() - t0 > budget_s:
                 continue
-            trial = dict(cur, **{knob: v})
+            trial = dict(cur, **setting)
             ctx.batch_set_tuning(**trial)
             ms = _measure(ctx, roots, reps)
             trials += 1

@@ -362,6 +362,21 @@ int g5_validate_pred(g5_ctx* ctx, int64_t root, const int64_t* pred, int64_t* er
   });
 }
 
+int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* t) {
+  return guarded(ctx, [&] {
+    G5_CHECK(t && t->bin_edges >= 0, "g5_bfs_set_tuning: bad tuning");
+    ctx->bfs.bin_edges = t->bin_edges;
+  });
+}
+
+int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* t) {
+  return guarded(ctx, [&] {
+    G5_CHECK(t, "g5_bfs_get_tuning: null output");
+    memset(t, 0, sizeof(*t));
+    t->bin_edges = ctx->bfs.bin_edges;
+  });
+}
+
 int g5_end_bfs(g5_ctx* ctx) {
   return guarded(ctx, [&] {
     set_device(ctx);
@@ -385,7 +400,8 @@ int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* t) {
                  t->persist_grid <= (1 << 20) &&
                  (t->persist_waves == 4 || t->persist_waves == 6 || t->persist_waves == 8) &&
                  t->pull_filter >= 0 && t->pull_filter <= 2 && t->beta >= 0 && t->compact_below >= 0 &&
-                 t->filter_from >= 0 && t->filter_from <= 1 && t->pull_unroll >= 0 && t->pull_unroll <= 4,
+                 t->filter_from >= 0 && t->filter_from <= 1 && t->pull_unroll >= 0 && t->pull_unroll <= 4 &&
+                 (t->lane_rule == 0 || t->lane_rule == 1),
              "g5_batch_set_tuning: bad tuning");
     ctx->ms.persistent = t->persistent;
     ctx->ms.persist_levels = t->persist_levels;
@@ -396,6 +412,7 @@ int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* t) {
     ctx->ms.compact_below = t->compact_below;
     ctx->ms.filter_from = t->filter_from;
     ctx->ms.pull_unroll = t->pull_unroll;
+    ctx->ms.lane_rule = t->lane_rule;
     ctx->ms.alpha = t->alpha;
     ctx->ms.pull_grid = t->pull_grid;
     ctx->ms.sweep_below = t->sweep_below;
@@ -419,7 +436,7 @@ int g5_batch_get_tuning(g5_ctx* ctx, g5_batch_tuning* t) {
     t->compact_below = ctx->ms.compact_below;
     t->filter_from = ctx->ms.filter_from;
     t->pull_unroll = ctx->ms.pull_unroll;
-    t->reserved2 = 0;
+    t->lane_rule = ctx->ms.lane_rule;
   });
 }
 

@@ -35,6 +35,7 @@
 namespace g5 {
 
 constexpr uint64_t kTdSmall = 1 << 20;  // top-down levels up to this many edges use the atomic path
+constexpr int kBinShift = 19;           // binned top-down: at least 2^19 targets per bucket
 constexpr int kShortRow = 32;    // bottom-up: lane-sequential scan up to this length
 constexpr uint32_t kNoNbr = 0xFFFFFFFFu;
 
@@ -190,6 +191,94 @@ __device__ __forceinline__ bool first_marked_nbr(bool active, uint32_t v, const 
     }
   }
   return found;
+}
+
+// Binned top-down for hub levels (hundreds of millions of frontier edges):
+// the edges' (target, source) pairs are first written out grouped by
+// target range (bucket = target >> shift, 2^shift targets, whose level and
+// parent bytes fit in L2), then applied bucket after bucket, so the
+// random level probes of the plain bulk path become L2-local.  Two passes
+// over the frontier's rows (count, then scatter with per-block ranges
+// reserved by one atomic per bucket) and one over the pairs.
+constexpr int kBinMaxBuckets = 2048;
+
+__global__ void __launch_bounds__(kBlock) k_td_bin_count(const uint32_t* __restrict__ queue, int64_t nf,
+                                                         const uint64_t* __restrict__ prefix,
+                                                         const uint32_t* __restrict__ chunk_rows,
+                                                         const uint64_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ cols, int shift, int nb,
+                                                         unsigned long long* __restrict__ bucket_cnt) {
+  __shared__ ChunkRows cr;
+  __shared__ unsigned s_hist[kBinMaxBuckets];
+  for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0;
+  __syncthreads();
+  const uint64_t E = prefix[nf];
+  for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
+    const uint64_t e0 = chunk * kEpb;
+    const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
+    const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
+    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t) { atomicAdd(&s_hist[w >> shift], 1u); });
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < nb; i += kBlock)
+    if (s_hist[i]) atomicAdd(bucket_cnt + i, (unsigned long long)s_hist[i]);
+}
+
+// bucket_cnt -> exclusive offsets in bucket_cur (one block; nb <= kBinMaxBuckets)
+__global__ void __launch_bounds__(kBlock) k_td_bin_offsets(const unsigned long long* __restrict__ bucket_cnt,
+                                                           int nb, unsigned long long* __restrict__ bucket_cur) {
+  if (threadIdx.x == 0) {
+    unsigned long long run = 0;
+    for (int i = 0; i < nb; ++i) {
+      bucket_cur[i] = run;
+      run += bucket_cnt[i];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_td_bin_scatter(const uint32_t* __restrict__ queue, int64_t nf,
+                                                           const uint64_t* __restrict__ prefix,
+                                                           const uint32_t* __restrict__ chunk_rows,
+                                                           const uint64_t* __restrict__ offsets,
+                                                           const uint32_t* __restrict__ cols, int shift, int nb,
+                                                           unsigned long long* __restrict__ bucket_cur,
+                                                           uint64_t* __restrict__ pairs) {
+  __shared__ ChunkRows cr;
+  __shared__ unsigned long long s_cur[kBinMaxBuckets];
+  __shared__ unsigned s_hist[kBinMaxBuckets];
+  const uint64_t E = prefix[nf];
+  for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
+    const uint64_t e0 = chunk * kEpb;
+    const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
+    for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0;
+    const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);  // ends with a barrier
+    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t) { atomicAdd(&s_hist[w >> shift], 1u); });
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += kBlock)
+      s_cur[i] = s_hist[i] ? atomicAdd(bucket_cur + i, (unsigned long long)s_hist[i]) : 0;
+    __syncthreads();
+    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
+      const unsigned long long pos = atomicAdd(&s_cur[w >> shift], 1ull);
+      pairs[pos] = ((uint64_t)w << 32) | u;
+    });
+    __syncthreads();
+  }
+}
+
+// Applies the pairs in order.  The grid strides over them together, so the
+// pairs in flight at any moment span a window of grid * kBlock pairs, i.e.
+// one or two target ranges, whose level and parent lines stay in L2.
+__global__ void __launch_bounds__(kBlock) k_td_bin_apply(const uint64_t* __restrict__ pairs, uint64_t E,
+                                                         uint8_t* level, uint32_t* __restrict__ parent,
+                                                         uint32_t lvl) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < E; i += stride) {
+    const uint64_t x = pairs[i];
+    const uint32_t w = (uint32_t)(x >> 32);
+    if (level[w] != 0xFF) continue;
+    level[w] = (uint8_t)lvl;
+    parent[w] = (uint32_t)x;
+  }
 }
 
 // After a bulk top-down level: the vertices whose level became 'lvl' join the
@@ -450,9 +539,27 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
                            b.prefix.p, b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p,
                            b.parent.p, b.queue[qi ^ 1].p, cnt, (uint32_t)lvl);
       } else {
-        hipLaunchKernelGGL(k_td_expand<true>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf,
-                           b.prefix.p, b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p,
-                           b.parent.p, (uint32_t*)nullptr, cnt, (uint32_t)lvl);
+        if (b.bin_edges > 0 && E >= (uint64_t)b.bin_edges) {
+          // binned by target range (see k_td_bin_count)
+          int shift = kBinShift;
+          while ((rows >> shift) >= kBinMaxBuckets) ++shift;
+          const int nb = (int)(((rows - 1) >> shift) + 1);
+          b.bin_pairs.alloc(E);
+          b.bin_cnt.alloc(2 * kBinMaxBuckets);
+          unsigned long long* bc = (unsigned long long*)b.bin_cnt.p;
+          G5_HIP(hipMemsetAsync(bc, 0, nb * 8, s));
+          hipLaunchKernelGGL(k_td_bin_count, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
+                             b.chunk_rows.p, g.offsets.p, g.cols.p, shift, nb, bc);
+          hipLaunchKernelGGL(k_td_bin_offsets, dim3(1), dim3(kBlock), 0, s, bc, nb, bc + kBinMaxBuckets);
+          hipLaunchKernelGGL(k_td_bin_scatter, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
+                             b.chunk_rows.p, g.offsets.p, g.cols.p, shift, nb, bc + kBinMaxBuckets, b.bin_pairs.p);
+          hipLaunchKernelGGL(k_td_bin_apply, dim3(grid_of((int64_t)E, kBlock, 2048)), dim3(kBlock), 0, s,
+                             b.bin_pairs.p, E, b.level.p, b.parent.p, (uint32_t)lvl);
+        } else {
+          hipLaunchKernelGGL(k_td_expand<true>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf,
+                             b.prefix.p, b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p,
+                             b.parent.p, (uint32_t*)nullptr, cnt, (uint32_t)lvl);
+        }
         G5_HIP(hipGetLastError());
         // k_td_finish makes vis[cur^1] exactly this level's vertices and adds
         // them to vis[cur]

@@ -59,6 +59,9 @@ struct BfsState {
   uint64_t* h_counters = nullptr; // pinned host mirror
   DevBuf<int64_t> pred_out;       // n, built on demand by get_pred
   DevBuf<uint8_t> level_out;      // n, built on demand by get_levels
+  int64_t bin_edges = 0;  // top-down levels with at least this many edges are binned (0: never)
+  DevBuf<uint64_t> bin_pairs;     // (target << 32 | source) of a binned level
+  DevBuf<uint64_t> bin_cnt;       // bucket counts and cursors
   g5_bfs_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> lev_begin, lev_end;  // per level
@@ -73,6 +76,8 @@ struct BfsState {
     counters.release();
     pred_out.release();
     level_out.release();
+    bin_pairs.release();
+    bin_cnt.release();
     if (h_counters) (void)hipHostFree(h_counters);
     h_counters = nullptr;
     if (ev0) (void)hipEventDestroy(ev0);
@@ -110,9 +115,14 @@ struct MsState {
                                // 2 for vertices from filter_from x rows on
   double filter_from = 0.01;
   int pull_unroll = 0;         // probe unrolling of the per-level pull (0..4, see g5_msbfs.hip)
+  int lane_rule = 0;           // 0: one direction per level for every root; 1: per root (per-level launches)
+  uint64_t level_push[G5_MAX_LEVELS] = {};  // per level: roots pushed, roots pulled (per-root rule)
+  uint64_t level_pull[G5_MAX_LEVELS] = {};
+  bool level_masks = false;    // the last run used the per-root rule
   double beta = 50.0;          // pull whenever fewer than 1/beta of the (root, vertex) pairs are unreached
   double compact_below = 0.0;  // pull from a compact frontier when it spans fewer than this x rows vertices
   DevBuf<uint64_t> pack, compact;
+  DevBuf<uint64_t> nz_sel;     // per-root rule: vertices holding a pushing root's frontier
   hipGraphExec_t graph_exec = nullptr;
   int graph_levels = 0;
   double graph_alpha = 0, graph_sweep = 0;
@@ -157,6 +167,7 @@ struct MsState {
     chunk_list.release();
     pack.release();
     compact.release();
+    nz_sel.release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
     graph_levels = 0;

@@ -81,6 +81,53 @@ struct NoEpilogue {
   __device__ void end() const {}
 };
 
+// Per-lane statistics of a level: for every lane the new vertices and the sum
+// of their row lengths (the lane's next frontier edges).  Thread i of a wave
+// keeps lane i's sums in registers: per 64-vertex word it walks the vertices
+// with new lanes, reading each one's word and row length by readlane, so no
+// two threads add to one place until the block's LDS reduction at the end;
+// totals go to lane_nq[0, 64) and lane_nq[64, 128).
+struct LaneEpi {
+  unsigned long long* lane_nq;
+  const uint64_t* offsets;
+  mutable uint64_t n_acc = 0, d_acc = 0;
+  __device__ unsigned long long* lds() const {
+    __shared__ unsigned long long s_lane[128];
+    return s_lane;
+  }
+  __device__ void begin() const {
+    if (threadIdx.x < 128) lds()[threadIdx.x] = 0;
+    __syncthreads();
+  }
+  __device__ void operator()(int64_t v, bool inb, uint64_t nw) const {
+    const uint64_t deg = (inb && nw) ? offsets[v + 1] - offsets[v] : 0;
+    const int me = threadIdx.x & 63;
+    uint64_t m = __ballot(inb && nw != 0);
+    while (m) {
+      const int t = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const uint32_t wlo = __builtin_amdgcn_readlane((uint32_t)nw, t);
+      const uint32_t whi = __builtin_amdgcn_readlane((uint32_t)(nw >> 32), t);
+      const uint32_t dlo = __builtin_amdgcn_readlane((uint32_t)deg, t);
+      const uint32_t dhi = __builtin_amdgcn_readlane((uint32_t)(deg >> 32), t);
+      const uint32_t bit = me < 32 ? (wlo >> me) & 1 : (whi >> (me - 32)) & 1;
+      if (bit) {
+        ++n_acc;
+        d_acc += ((uint64_t)dhi << 32) | dlo;
+      }
+    }
+  }
+  __device__ void end() const {
+    const int me = threadIdx.x & 63;
+    if (n_acc) {
+      atomicAdd(&lds()[me], (unsigned long long)n_acc);
+      atomicAdd(&lds()[64 + me], (unsigned long long)d_acc);
+    }
+    __syncthreads();
+    if (threadIdx.x < 128 && lds()[threadIdx.x]) atomicAdd(lane_nq + threadIdx.x, lds()[threadIdx.x]);
+  }
+};
+
 // After a push or a sweep: next keeps only lanes new to v, seen gains them,
 // and nz_next gets one bit per vertex whose frontier word is not zero.
 // Blocks bid of nblk take 64-vertex words in a grid-stride loop.

@@ -117,6 +117,63 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
                                    gridDim.x);
 }
 
+// --- per-root directions (lane_rule 1) ---
+__global__ void __launch_bounds__(kBlock) k_ms_finish_lanes(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
+                                                            uint64_t* __restrict__ nz_next,
+                                                            const uint64_t* __restrict__ offsets, int64_t rows,
+                                                            unsigned long long* __restrict__ c, LaneEpi epi_in) {
+  const LaneEpi epi = epi_in;  // a copy in registers: the epilogue accumulates per thread
+  ms_finish_body(seen, next, nz_next, offsets, rows, c, blockIdx.x, gridDim.x, ~UINT64_C(0), epi);
+}
+
+// Pull for the lanes in live_in; with merge, next already holds this level's
+// pushed lanes, which are folded in.
+__global__ void __launch_bounds__(kBlock) k_ms_pull_lanes(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
+                                                          const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
+                                                          uint64_t* __restrict__ nz_next,
+                                                          const uint32_t* __restrict__ first_nbr,
+                                                          const uint64_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ cols, int64_t rows,
+                                                          uint64_t live_in, int filter, int merge,
+                                                          unsigned long long* __restrict__ c, LaneEpi epi_in) {
+  const LaneEpi epi = epi_in;  // a copy in registers: the epilogue accumulates per thread
+  const FrontView fv{front, nz_front, nullptr, nullptr, filter ? kFrontFilter : kFrontPlain, 0};
+  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x,
+               ~UINT64_C(0), epi, merge != 0);
+}
+
+// One bit per vertex whose frontier word has one of `lanes`.
+__global__ void __launch_bounds__(kBlock) k_ms_select(const uint64_t* __restrict__ front, int64_t rows, uint64_t lanes,
+                                                      uint64_t* __restrict__ nz_sel) {
+  const int lane = threadIdx.x & 63;
+  const int64_t words = (rows + 63) / 64;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t wd = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
+    const int64_t v = wd * 64 + lane;
+    const uint64_t b = __ballot(v < rows && (front[v] & lanes) != 0);
+    if (lane == 0) nz_sel[wd] = b;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ms_collect(const uint32_t* __restrict__ chunk_rows, uint64_t nchunks,
+                                                       const uint64_t* __restrict__ nz_sel, uint32_t* __restrict__ list,
+                                                       unsigned* count) {
+  ms_collect_chunks_body(chunk_rows, nchunks, nz_sel, list, count, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ms_push_list(const uint32_t* __restrict__ list, const unsigned* count,
+                                                         const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
+                                                         const uint64_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ cols,
+                                                         const uint64_t* __restrict__ front,
+                                                         const uint64_t* __restrict__ nz_sel,
+                                                         const uint64_t* __restrict__ seen, uint64_t* next,
+                                                         uint64_t lanes) {
+  __shared__ SweepLds L;
+  ms_push_list_body(list, count, chunk_rows, nnz, offsets, cols, front, nz_sel, seen, next, L, blockIdx.x, gridDim.x,
+                    lanes);
+}
+
 __global__ void __launch_bounds__(kBlock) k_ms_sweep_clear(const uint32_t* __restrict__ chunk_rows,
                                                            uint64_t nchunks, const uint64_t* __restrict__ offsets,
                                                            uint64_t* __restrict__ next) {
@@ -342,10 +399,14 @@ struct PopCount {
 };
 
 // --- batched byte model (evaluated after the run from the per-level words) ---
-// pull level l: 24 n + sum over vertices with a needed lane of (8 + 12 k),
-// k = probes until every needed lane is found (the row length if not).
+// level l: 24 n + sum over vertices with a needed pulled lane of (8 + 12 k),
+// k = probes until every needed lane is found (the row length if not), +
+// sum over vertices whose frontier word holds a pushed lane of (8 + 12 deg):
+// the word, then per entry the column and the target's seen word.  With one
+// direction for all roots every level is costed as a pull of all lanes.
 __global__ void __launch_bounds__(kBlock) k_ms_model(const uint64_t* const* __restrict__ fr, int l,
-                                                     uint64_t lanes, const uint64_t* __restrict__ offsets,
+                                                     uint64_t lanes, uint64_t push_lanes,
+                                                     const uint64_t* __restrict__ offsets,
                                                      const uint32_t* __restrict__ cols, int64_t rows,
                                                      unsigned long long* __restrict__ acc) {
   uint64_t b = 0;
@@ -354,6 +415,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_model(const uint64_t* const* __re
     for (int i = 0; i < l; ++i) seen_before |= fr[i][v];
     const uint64_t need = ~seen_before & lanes;
     const uint64_t s = offsets[v], e = offsets[v + 1];
+    if (fr[l - 1][v] & push_lanes) b += 8 + 12 * (e - s);
     if (!need || s == e) continue;
     uint64_t a = 0, k = 0;
     for (uint64_t j = s; j < e; ++j) {
@@ -381,7 +443,7 @@ void ms_prepare(g5_ctx* ctx) {
   m.prefix.alloc(rows + 1);
   m.chunk_rows.alloc(2 * ((uint64_t)g.nnz / kEpb + 2));
   m.scan_tmp.alloc(scan_tmp_elems(rows + 1));
-  m.counters.alloc(M_NUM);
+  m.counters.alloc(M_NUM + 1 + 128);  // level counters, a list count, per-lane counts and degree sums
   m.roots.alloc(64);
   m.frp.alloc(G5_MAX_LEVELS);
   if (g.reorder_mode != 0 && g.nnz > 0) {
@@ -392,7 +454,7 @@ void ms_prepare(g5_ctx* ctx) {
                        dim3(kBlock), 0, ctx->stream, g.offsets.p, rows, m.sweep_chunks, m.sweep_rows.p);
     G5_HIP(hipGetLastError());
   }
-  G5_HIP(hipHostMalloc((void**)&m.h_counters, M_NUM * sizeof(uint64_t), hipHostMallocDefault));
+  G5_HIP(hipHostMalloc((void**)&m.h_counters, (M_NUM + 1 + 128) * sizeof(uint64_t), hipHostMallocDefault));
   G5_HIP(hipEventCreate(&m.ev0));
   G5_HIP(hipEventCreate(&m.ev1));
   m.lev_begin.assign(G5_MAX_LEVELS, nullptr);
@@ -427,6 +489,13 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
   const double sweep_limit = m.sweep_below * roots_live * (double)rows;
   const double total_pairs = roots_live * (double)rows;
   const double unseen_frac = m.beta > 0 ? 1.0 / m.beta : 0.0;
+  // per-root rule (from the first level, reorder modes 1 and 2): each root's
+  // frontier edges and the edges of the vertices it has reached
+  const bool lanes_mode = m.lane_rule == 1 && m.sweep_chunks && l0 == 1;
+  m.level_masks = lanes_mode;
+  double lane_fd[64] = {}, lane_vd[64] = {};
+  const uint64_t* h_lane = m.h_counters + M_NUM + 1;
+  unsigned* d_list_count = (unsigned*)(cnt + M_NUM);
   int l = l0 - 1;
   while (live) {
     ++l;
@@ -435,11 +504,50 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     const uint64_t* front = m.fr[l - 1]->p;  // after ms_level: the vector may have grown
     const uint64_t* nz_front = m.nz[(l - 1) & 1].p;
     uint64_t* nz_next = m.nz[l & 1].p;
-    const bool push = (double)fdeg < push_limit && total_pairs - reached >= unseen_frac * total_pairs;
-    const bool sweep = !push && m.sweep_chunks && reached < sweep_limit;
-    G5_HIP(hipMemsetAsync(m.counters.p, 0, M_NUM * 8, s));
+    bool push = (double)fdeg < push_limit && total_pairs - reached >= unseen_frac * total_pairs;
+    bool sweep = !push && m.sweep_chunks && reached < sweep_limit;
+    uint64_t td = 0, bu = 0;
+    if (lanes_mode) {
+      // pull when the frontier's edges exceed 1/alpha of the edges not yet
+      // reached (the roots' own level pushes; all pull in the beta tail)
+      const bool tail = total_pairs - reached < unseen_frac * total_pairs;
+      for (int r = 0; r < 64; ++r) {
+        const uint64_t bit = UINT64_C(1) << r;
+        if (!(live & bit)) continue;
+        const double unreached = (double)g.nnz - lane_vd[r];
+        const bool pull_r = tail || (l > 1 && lane_fd[r] * m.alpha > unreached);
+        (pull_r ? bu : td) |= bit;
+      }
+      push = bu == 0;
+      sweep = false;
+    }
+    m.level_push[l] = push ? live : td;
+    m.level_pull[l] = push ? 0 : (lanes_mode ? bu : live);
+    G5_HIP(hipMemsetAsync(m.counters.p, 0, (M_NUM + 1 + 128) * 8, s));
     G5_HIP(hipEventRecord(m.lev_begin[l], s));
-    if (push) {
+    const LaneEpi lepi{cnt + M_NUM + 1, g.offsets.p};
+    if (lanes_mode && td && bu) {
+      // the pushing roots over the CSR chunks holding one of their frontier
+      // vertices, then the pull of the others, merging the pushed lanes
+      m.chunk_list.alloc(m.sweep_chunks);
+      m.nz_sel.alloc(words);
+      hipLaunchKernelGGL(k_ms_select, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, front, rows, td,
+                         m.nz_sel.p);
+      G5_HIP(hipMemsetAsync(next, 0, rows * 8, s));
+      hipLaunchKernelGGL(k_ms_collect, dim3(grid_of((int64_t)m.sweep_chunks, kBlock, 4096)), dim3(kBlock), 0, s,
+                         m.sweep_rows.p, m.sweep_chunks, m.nz_sel.p, m.chunk_list.p, d_list_count);
+      hipLaunchKernelGGL(k_ms_push_list, dim3(grid_of((int64_t)m.sweep_chunks, 1, 4096)), dim3(kBlock), 0, s,
+                         m.chunk_list.p, (const unsigned*)d_list_count, m.sweep_rows.p, (uint64_t)g.nnz, g.offsets.p,
+                         g.cols.p, front, m.nz_sel.p, m.seen.p, next, td);
+      G5_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_ms_pull_lanes, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
+                         m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, bu,
+                         m.pull_filter == 1 ? 1 : 0, 1, cnt, lepi);
+    } else if (lanes_mode && !push) {
+      hipLaunchKernelGGL(k_ms_pull_lanes, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
+                         m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, bu,
+                         m.pull_filter == 1 ? 1 : 0, 0, cnt, lepi);
+    } else if (push) {
       device_scan(UnionCount{nz_front}, UnionEmit{nz_front, m.queue.p}, (size_t)words, m.scan_tmp.p, s);
       const int64_t nf = (int64_t)nz;
       device_scan(FrontierDegree{m.queue.p, g.offsets.p, nf}, StoreU64{m.prefix.p}, (size_t)nf + 1,
@@ -453,8 +561,12 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)fdeg, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p,
                          nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, m.seen.p, next);
       G5_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_ms_finish, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
-                         next, nz_next, g.offsets.p, rows, cnt);
+      if (lanes_mode)
+        hipLaunchKernelGGL(k_ms_finish_lanes, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
+                           next, nz_next, g.offsets.p, rows, cnt, lepi);
+      else
+        hipLaunchKernelGGL(k_ms_finish, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
+                           next, nz_next, g.offsets.p, rows, cnt);
     } else if (sweep) {
       hipLaunchKernelGGL(k_ms_sweep_clear, dim3(grid_of((int64_t)m.sweep_chunks, kBlock, 4096)), dim3(kBlock), 0,
                          s, m.sweep_rows.p, m.sweep_chunks, g.offsets.p, next);
@@ -489,12 +601,18 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(m.lev_end[l], s));
-    G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, M_NUM * 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, (lanes_mode ? M_NUM + 1 + 128 : M_NUM) * 8,
+                          hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     float lms = 0;
     G5_HIP(hipEventElapsedTime(&lms, m.lev_begin[l], m.lev_end[l]));
     st.level_ms[l] = lms;
-    st.direction[l] = push ? 1 : (sweep ? 2 : 0);
+    st.direction[l] = (lanes_mode && td && bu) ? 3 : (push ? 1 : (sweep ? 2 : 0));
+    if (lanes_mode)
+      for (int r = 0; r < 64; ++r) {
+        lane_fd[r] = (double)h_lane[64 + r];
+        lane_vd[r] += (double)h_lane[64 + r];
+      }
     st.pairs[l] = (int64_t)m.h_counters[M_PAIRS];
     reached += (double)m.h_counters[M_PAIRS];
     st.num_levels = l + 1;
@@ -700,6 +818,7 @@ void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots) {
   const int64_t rows = g.n_rows;
   unsigned long long* cnt = (unsigned long long*)m.counters.p;
   m.have_result = false;
+  m.level_masks = false;
   m.nroots = nroots;
   m.host_roots.assign(roots, roots + nroots);
   g5_batch_stats& st = m.stats;
@@ -792,8 +911,10 @@ void ms_model_bytes(g5_ctx* ctx, double* per_level, double* total) {
     if (per_level) per_level[l] = 0;
   for (int l = 1; l < nlev; ++l) {
     G5_HIP(hipMemsetAsync(ctx->acc.p, 0, 8, s));
+    const uint64_t pull = m.level_masks ? m.level_pull[l] & lanes : lanes;
+    const uint64_t push = m.level_masks ? m.level_push[l] & lanes : 0;
     hipLaunchKernelGGL(k_ms_model, dim3(grid_of(g.n_rows, kBlock, 8192)), dim3(kBlock), 0, s,
-                       (const uint64_t* const*)m.frp.p, l, lanes, g.offsets.p, g.cols.p, g.n_rows,
+                       (const uint64_t* const*)m.frp.p, l, pull, push, g.offsets.p, g.cols.p, g.n_rows,
                        (unsigned long long*)ctx->acc.p);
     G5_HIP(hipGetLastError());
     uint64_t h = 0;

@@ -36,7 +36,7 @@ EXPORTS = [
     "g5_edges_download", "g5_edges_release", "g5_generate_kronecker", "g5_construct",
     "g5_graph_info_get", "g5_has_edge", "g5_pred_size", "g5_find_roots", "g5_graph_download",
     "g5_prepare_bfs", "g5_run_bfs", "g5_get_pred", "g5_get_levels", "g5_teps_edges",
-    "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_validate", "g5_validate_pred", "g5_end_bfs",
+    "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_bfs_set_tuning", "g5_bfs_get_tuning", "g5_validate", "g5_validate_pred", "g5_end_bfs",
     "g5_batch_prepare", "g5_batch_set_tuning", "g5_batch_get_tuning", "g5_run_bfs_batch",
     "g5_batch_get_levels", "g5_batch_teps_edges", "g5_batch_stats_get", "g5_batch_model_bytes",
     "g5_batch_end", "g5_server_create", "g5_server_destroy", "g5_server_last_error", "g5_server_run",
@@ -131,7 +131,7 @@ class BatchTuning(ctypes.Structure):
                 ("sweep_grid", ctypes.c_int64), ("persistent", ctypes.c_int32), ("persist_levels", ctypes.c_int32),
                 ("persist_grid", ctypes.c_int64), ("persist_waves", ctypes.c_int32), ("pull_filter", ctypes.c_int32),
                 ("beta", ctypes.c_double), ("filter_from", ctypes.c_double), ("pull_unroll", ctypes.c_int32),
-                ("reserved2", ctypes.c_int32), ("compact_below", ctypes.c_double)]
+                ("lane_rule", ctypes.c_int32), ("compact_below", ctypes.c_double)]
 
 
 _lib = None
@@ -173,6 +173,8 @@ def load_library(path=LIB_PATH):
         "g5_bfs_stats_get": ([vp, ctypes.POINTER(BfsStats)], i32),
         "g5_bfs_model_bytes": ([vp, vp, ctypes.POINTER(ctypes.c_double)], i32),
         "g5_validate": ([vp, vp, ctypes.POINTER(i32)], i32),
+        "g5_bfs_set_tuning": ([vp, vp], i32),
+        "g5_bfs_get_tuning": ([vp, vp], i32),
         "g5_validate_pred": ([vp, i64, vp, vp, ctypes.POINTER(i32), ctypes.POINTER(i64)], i32),
         "g5_end_bfs": ([vp], i32),
         "g5_batch_prepare": ([vp], i32),
@@ -343,6 +345,18 @@ class G5Context:
         L = self.bfs_stats()["num_levels"]
         return per[1:L].tolist(), tot.value
 
+    def bfs_set_tuning(self, bin_edges):
+        """Top-down levels with at least bin_edges frontier edges are binned by
+        target range (0 = never)."""
+        t = np.zeros(4, np.int64)
+        t[0] = bin_edges
+        self._check(self._lib.g5_bfs_set_tuning(self._h, _ptr(t)))
+
+    def bfs_tuning(self):
+        t = np.zeros(4, np.int64)
+        self._check(self._lib.g5_bfs_get_tuning(self._h, _ptr(t)))
+        return {"bin_edges": int(t[0])}
+
     VALIDATION_RULES = ["root", "self_parent", "depth", "edge", "parent_edge", "range"]
 
     def validate(self):
@@ -375,7 +389,7 @@ class G5Context:
 
     TUNING_FIELDS = ("alpha", "pull_grid", "sweep_below", "sweep_grid", "persistent", "persist_levels",
                      "persist_grid", "persist_waves", "pull_filter", "beta", "filter_from", "pull_unroll",
-                     "compact_below")
+                     "lane_rule", "compact_below")
 
     def batch_set_tuning(self, **kw):
         """Sets any of TUNING_FIELDS; the others keep their values."""

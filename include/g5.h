@@ -73,7 +73,8 @@ typedef struct g5_batch_stats {
   int32_t num_roots;
   int32_t num_levels;               /* levels executed + 1 (last level finds nothing) */
   int32_t direction[G5_MAX_LEVELS]; /* 1 = push (top-down), 0 = vertex-parallel pull (bottom-up),
-                                       2 = edge-parallel pull sweep */
+                                       2 = edge-parallel pull sweep, 3 = some roots push and
+                                       the others pull (lane_rule 1) */
   int64_t pairs[G5_MAX_LEVELS];     /* (root, vertex) pairs discovered at that level */
   double level_ms[G5_MAX_LEVELS];   /* device time per level (HIP events) */
   double device_ms;                 /* device time of the whole batch */
@@ -85,7 +86,7 @@ typedef struct g5_batch_stats {
 typedef struct g5_batch_tuning {
   double alpha;        /* push while the frontier union has fewer than nnz/alpha edges */
   int64_t pull_grid;   /* workgroups of the vertex-parallel pull (256 threads each) */
-  double sweep_below;  /* pull levels use the edge-parallel sweep while the (root, vertex)
+  double sweep_below;  /* (lane_rule 0) pull levels use the edge-parallel sweep while the (root, vertex)
                           pairs reached so far are below sweep_below x roots x rows
                           (reorder modes 1 and 2; 0 disables the sweep) */
   int64_t sweep_grid;  /* workgroups of the edge-parallel sweep */
@@ -103,7 +104,11 @@ typedef struct g5_batch_tuning {
   double filter_from;      /* pull_filter 2 threshold, a fraction of the rows */
   int32_t pull_unroll;     /* probes in flight per lane in the per-level pull: 0 = 4 per short-row step
                               and 2 per long-row step, 1 = 8/2, 2 = 8/4, 3 = 4/4, 4 = 2/1 */
-  int32_t reserved2;
+  int32_t lane_rule;       /* 0: one direction per level for all roots (the rule on alpha above);
+                              1: a direction per root and level (per-level launches, reorder modes
+                              1 and 2): a root pulls when its frontier's edges exceed 1/alpha of the
+                              edges of the vertices it has not reached, else pushes; a level with
+                              both kinds pushes first, then pulls and merges */
   double compact_below;    /* per-level pulls (persistent = 0) read frontier words from a compact
                               copy holding only the frontier's words when the frontier spans fewer
                               than compact_below x rows vertices; 0 disables */
@@ -188,6 +193,18 @@ int g5_bfs_stats_get(g5_ctx* ctx, g5_bfs_stats* out);
  * from the final levels, outside any timed region). per_level has
  * G5_MAX_LEVELS entries, index = level. */
 int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
+/* Single-root tuning (no counterpart in the reference; results do not depend
+ * on it).  bin_edges: top-down levels with at least this many frontier edges
+ * first group their (target, source) pairs by target range so that the level
+ * probes stay in L2 (0 = never, the default: up to SCALE 28 the level bytes
+ * fit the 256 MB last-level cache and the plain probes are twice as fast). */
+typedef struct g5_bfs_tuning {
+  int64_t bin_edges;
+  int64_t reserved[3];
+} g5_bfs_tuning;
+int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* tuning);
+int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* tuning);
+
 /* Validates the last run on the GPU with the rules of BfsValidation::validate
  * (mpi/validate.hpp:506-802), checking every edge of the CSR (all non-loop
  * input tuples, both directions).  errors[6] receives the number of

@@ -14,7 +14,7 @@ class FakeCtx:
     def __init__(self, bad=None):
         self.t = {"alpha": 16.0, "pull_grid": 8192, "sweep_below": 0.25, "sweep_grid": 4096, "persistent": 0,
                   "persist_levels": 16, "persist_grid": 0, "persist_waves": 8, "pull_filter": 0, "beta": 50.0,
-                  "compact_below": 0.0, "filter_from": 0.01, "pull_unroll": 0}
+                  "compact_below": 0.0, "filter_from": 0.01, "pull_unroll": 0, "lane_rule": 0}
         self.bad = bad
         self.runs = 0
 
@@ -35,7 +35,9 @@ class FakeCtx:
 
     def batch_stats(self):
         t = self.t
-        ms = 10.0 + abs(t["alpha"] - 2.0) / 8 + (0 if t["beta"] == 20.0 else 1) + t["pull_filter"] \
+        # the per-root rule wins only with its own alpha scale
+        a = abs(t["alpha"] - 128.0) / 64 - 1 if t["lane_rule"] else abs(t["alpha"] - 2.0) / 8
+        ms = 10.0 + a + (0 if t["beta"] == 20.0 else 1) + t["pull_filter"] \
             + (2 if t["persistent"] else 0) + (0 if t["pull_grid"] == 16384 else 0.5)
         return {"device_ms": ms}
 
@@ -49,7 +51,7 @@ class FakeCtx:
 def test_descent_finds_the_best_per_knob(tmp_path):
     c = FakeCtx()
     res = autotune.autotune(c, [1, 2], reps=1, cache=str(tmp_path / "t.json"))
-    assert res["tuning"]["alpha"] == 2.0
+    assert res["tuning"]["lane_rule"] == 1 and res["tuning"]["alpha"] == 128.0
     assert res["tuning"]["beta"] == 20.0
     assert res["tuning"]["pull_filter"] == 0
     assert res["tuning"]["persistent"] == 0
@@ -69,3 +71,12 @@ def test_cache_is_reused(tmp_path):
 def test_changed_results_are_refused(tmp_path):
     with pytest.raises(RuntimeError):
         autotune.autotune(FakeCtx(bad=("alpha", 8.0)), [1], reps=1, cache=str(tmp_path / "t.json"))
+
+
+def test_union_rule_alpha_is_kept_when_it_wins(tmp_path):
+    class UnionWins(FakeCtx):
+        def batch_stats(self):
+            ms = super().batch_stats()["device_ms"]
+            return {"device_ms": ms + (5.0 if self.t["lane_rule"] else 0.0)}
+    res = autotune.autotune(UnionWins(), [1], reps=1, cache=str(tmp_path / "t.json"))
+    assert res["tuning"]["lane_rule"] == 0 and res["tuning"]["alpha"] == 2.0

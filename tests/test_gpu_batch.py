@@ -81,6 +81,43 @@ def test_batch_compact_frontier_matches_reference_s16(s16ctx, compact):
     s16ctx.batch_set_tuning(alpha=16.0, compact_below=0.0, beta=50.0)
 
 
+def _per_root_run(s16ctx, alpha, filt):
+    g = gold("ref_s16_64roots.json")
+    roots = [p["root"] for p in g["per_root"]]
+    s16ctx.batch_set_tuning(alpha=alpha, persistent=0, lane_rule=1, pull_filter=filt, beta=0.0, sweep_below=0.0)
+    say("batch s16 per-root rule alpha", alpha, "filter", filt)
+    try:
+        s16ctx.run_bfs_batch(roots)
+        st = s16ctx.batch_stats()
+        if alpha < 1e-6:
+            assert set(st["direction"]) == {1}, st["direction"]
+        if alpha > 1e6:
+            assert st["direction"][0] == 1 and set(st["direction"][1:]) <= {0, 3}, st["direction"]
+        teps = s16ctx.batch_teps_edges()
+        for i, pr in enumerate(g["per_root"]):
+            assert sha(s16ctx.batch_levels(i)) == pr["levels_sha256"], (alpha, pr["root"])
+            assert teps[i] == pr["nedge"]
+        assert sum(st["pairs"]) + 64 == sum(sum(p["level_histogram"]) for p in g["per_root"])
+        return st["direction"]
+    finally:
+        s16ctx.batch_set_tuning(alpha=16.0, sweep_below=0.25, persistent=0, pull_filter=0, beta=50.0, lane_rule=0)
+
+
+@pytest.mark.parametrize("alpha,filt", [(4.0, 0), (16.0, 1), (1e-9, 0), (1e12, 0), (0.05, 0)])
+def test_batch_per_root_directions_match_reference_s16(s16ctx, alpha, filt):
+    """lane_rule 1: each root pulls or pushes by its own frontier; a level
+    with both runs the push of some roots and the pull of the others.  1e-9
+    pushes every root at every level, 1e12 pulls every root after its first
+    level."""
+    _per_root_run(s16ctx, alpha, filt)
+
+
+def test_batch_per_root_levels_mix_directions_s16(s16ctx):
+    """Some level of the S16 batch runs pushes and pulls together at alpha 4."""
+    d = _per_root_run(s16ctx, 4.0, 0)
+    assert 3 in d, d
+
+
 def test_batch_model_bytes_match_oracle(s16ctx, oracle_mod):
     O = oracle_mod
     oc = O.Csr(O.generate_edges(16))
@@ -119,8 +156,8 @@ def test_batch_partial_and_odd_roots(oracle_mod):
                 assert np.array_equal(c.batch_levels(i), lv), (i, r)
 
 
-@pytest.mark.parametrize("scale,sweep", [(20, 0.25), (20, 2.0)])
-def test_batch_matches_single_root_gpu(scale, sweep):
+@pytest.mark.parametrize("scale,sweep,rule", [(20, 0.25, 0), (20, 2.0, 0), (20, 0.0, 1)])
+def test_batch_matches_single_root_gpu(scale, sweep, rule):
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
     with graph500_amd.G5Context(0) as c:
@@ -128,8 +165,8 @@ def test_batch_matches_single_root_gpu(scale, sweep):
         c.construct(2, release_edges=True)
         roots = c.find_roots(64)
         c.batch_prepare()
-        c.batch_set_tuning(sweep_below=sweep)
-        say("batch s%d sweep %g" % (scale, sweep))
+        c.batch_set_tuning(sweep_below=sweep, lane_rule=rule, persistent=0)
+        say("batch s%d sweep %g lane_rule %d" % (scale, sweep, rule))
         c.run_bfs_batch(roots)
         teps = c.batch_teps_edges()
         for i, pr in enumerate(g["per_root"]):

@@ -123,12 +123,16 @@ def test_bfs_64_roots_bit_exact_s16(ctx, s16, oracle_mod):
         assert cnt == pr["nedge"]
 
 
-@pytest.mark.parametrize("scale", [18, 20, 22])
-def test_bfs_reference_fixtures_larger_scales(scale, oracle_mod):
+@pytest.mark.parametrize("scale,bin_edges", [(18, None), (20, None), (22, None), (20, 1), (22, 1 << 22)])
+def test_bfs_reference_fixtures_larger_scales(scale, bin_edges, oracle_mod):
+    """bin_edges None keeps the default; 1 bins every bulk top-down level."""
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
-    say("scale", scale)
+    say("scale", scale, "bin_edges", bin_edges)
     with graph500_amd.G5Context(0) as c:
+        if bin_edges is not None:
+            c.bfs_set_tuning(bin_edges)
+            assert c.bfs_tuning() == {"bin_edges": bin_edges}
         c.generate_kronecker(scale, 16, 2, 3)
         t = c.edges_download().view(np.uint32).reshape(-1, 3)
         assert sha(t) == g["tuples_sha256"]
@@ -233,6 +237,8 @@ def test_error_paths():
             c.construct(2)  # empty edge list
         with pytest.raises(E):
             c.generate_kronecker(0)  # scale out of range
+        with pytest.raises(E):
+            c.bfs_set_tuning(-1)  # negative threshold
         c.generate_kronecker(8)
         with pytest.raises(E):
             c.construct(7)  # unknown reorder mode
