@@ -99,7 +99,7 @@ struct MsState {
   int nroots = 0;
   int nlevels = 0;
   std::vector<int64_t> host_roots;
-  double alpha = 4.0;        // push while the frontier union has < nnz/alpha edges
+  double alpha = 128.0;      // direction threshold (its meaning depends on lane_rule)
   int64_t pull_grid = 8192;  // workgroups of the vertex-parallel pull
   double sweep_below = 0.0;  // edge-parallel sweep while reached pairs < this x roots x rows
   int64_t sweep_grid = 4096; // workgroups of the edge-parallel sweep
@@ -115,7 +115,7 @@ struct MsState {
                                // 2 for vertices from filter_from x rows on
   double filter_from = 0.01;
   int pull_unroll = 0;         // probe unrolling of the per-level pull (0..4, see g5_msbfs.hip)
-  int lane_rule = 0;           // 0: one direction per level for every root; 1: per root (per-level launches)
+  int lane_rule = 1;           // 0: one direction per level for every root; 1: per root (per-level launches)
   uint64_t level_push[G5_MAX_LEVELS] = {};  // per level: roots pushed, roots pulled (per-root rule)
   uint64_t level_pull[G5_MAX_LEVELS] = {};
   bool level_masks = false;    // the last run used the per-root rule

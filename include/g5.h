@@ -82,7 +82,10 @@ typedef struct g5_batch_stats {
   int32_t persistent_levels;        /* levels run inside the persistent launch */
 } g5_batch_stats;
 
-/* Tuning of the batched BFS (set by the autotuner, graph500_amd/autotune.py). */
+/* Tuning of the batched BFS (set by the autotuner, graph500_amd/autotune.py;
+ * defaults: lane_rule 1, alpha 128, pull_grid 8192, sweep_below 0, sweep_grid
+ * 4096, persistent 0, persist_levels 16, persist_grid 0, persist_waves 4,
+ * pull_filter 0, beta 50, filter_from 0.01, pull_unroll 0, compact_below 0). */
 typedef struct g5_batch_tuning {
   double alpha;        /* push while the frontier union has fewer than nnz/alpha edges */
   int64_t pull_grid;   /* workgroups of the vertex-parallel pull (256 threads each) */
@@ -237,7 +240,9 @@ int g5_batch_teps_edges(g5_ctx* ctx, int64_t* counts);
 int g5_batch_stats_get(g5_ctx* ctx, g5_batch_stats* out);
 /* Algorithmic bytes of the last batch under the bit-packed pull model: per
  * level 24 n + sum over vertices lacking some root of (8 + 12 k), k = probes
- * until every lacking root is found; plus 1 byte per (root, reached vertex). */
+ * until every lacking root is found; plus 1 byte per (root, reached vertex).
+ * Under lane_rule 1 only the roots pulled at a level count as lacking, and
+ * every vertex whose frontier word holds a pushed root adds 8 + 12 deg. */
 int g5_batch_model_bytes(g5_ctx* ctx, double* per_level, double* total);
 int g5_batch_end(g5_ctx* ctx);
 

@@ -48,7 +48,8 @@ def s16ctx():
 def test_batch_64_roots_match_reference_s16(s16ctx, alpha, sweep, filt, beta, persistent):
     g = gold("ref_s16_64roots.json")
     roots = [p["root"] for p in g["per_root"]]
-    s16ctx.batch_set_tuning(alpha=alpha, sweep_below=sweep, persistent=persistent, pull_filter=filt, beta=beta)
+    s16ctx.batch_set_tuning(alpha=alpha, sweep_below=sweep, persistent=persistent, pull_filter=filt, beta=beta,
+                            lane_rule=0)
     say("batch s16 alpha", alpha, "sweep", sweep, "filter", filt, "beta", beta, "persistent", persistent)
     s16ctx.run_bfs_batch(roots)
     st = s16ctx.batch_stats()
@@ -71,7 +72,8 @@ def test_batch_compact_frontier_matches_reference_s16(s16ctx, compact):
     frontier spans under 30 % of the rows)."""
     g = gold("ref_s16_64roots.json")
     roots = [p["root"] for p in g["per_root"]]
-    s16ctx.batch_set_tuning(alpha=1e12 if compact > 1 else 16.0, persistent=0, compact_below=compact, beta=0.0)
+    s16ctx.batch_set_tuning(alpha=1e12 if compact > 1 else 16.0, persistent=0, compact_below=compact, beta=0.0,
+                            lane_rule=0)
     say("batch s16 compact", compact)
     s16ctx.run_bfs_batch(roots)
     teps = s16ctx.batch_teps_edges()
@@ -123,6 +125,7 @@ def test_batch_model_bytes_match_oracle(s16ctx, oracle_mod):
     oc = O.Csr(O.generate_edges(16))
     a = oc.arrays()
     roots = oc.find_roots(64)
+    s16ctx.batch_set_tuning(lane_rule=0, alpha=16.0, persistent=0)  # the oracle's model: every level a pull
     s16ctx.run_bfs_batch(roots)
     levels_new = np.stack([s16ctx.batch_levels(i)[a["new2old"]] for i in range(64)])
     tot_o, per_o = oc.msbfs_bytes(levels_new)
@@ -200,7 +203,7 @@ def test_batch_sweep_and_csr_push_small_graphs(oracle_mod):
                 c.construct(mode)
                 c.batch_prepare()
                 for persistent, alpha in ((0, 1e12), (1, 1e12), (0, 1e-9), (1, 1e-9)):
-                    c.batch_set_tuning(alpha=alpha, sweep_below=2.0, persistent=persistent, beta=0.0)
+                    c.batch_set_tuning(alpha=alpha, sweep_below=2.0, persistent=persistent, beta=0.0, lane_rule=0)
                     say("sweep small mode", mode, len(e), "persistent", persistent, "alpha", alpha)
                     c.run_bfs_batch(roots)
                     teps = c.batch_teps_edges()
