@@ -237,7 +237,7 @@ def serve_leg(scales, rates, max_queries, edgefactor, seed=1):
         cand = rng.integers(0, n, size=min(n, 200000))
         has.append(np.array([v for v in cand[:20000] if c.has_edge(int(v))], np.int64))
     runs = []
-    policy = {"answer_buffers": 128, "min_admit": 64, "max_wait_s": 0.3, "lane_rule": 1, "alpha": 64.0}
+    policy = {"answer_buffers": 128, "min_admit": 64, "max_wait_s": 0.3, "lane_rule": 3, "alpha": 64.0}
     with graph500_amd.G5Server(ctxs, **policy) as srv:
         for rate in rates:
             nq = int(min(max_queries, max(64, rate * 1.0)))

@@ -509,6 +509,19 @@ __device__ __forceinline__ void ms_push_list_body(const uint32_t* __restrict__ l
     push_chunk(list[i], chunk_rows, nnz, offsets, cols, front, nz_front, seen, next, L, lanes);
 }
 
+// One bit per vertex whose frontier word holds one of `lanes`.
+__device__ __forceinline__ void ms_select_body(const uint64_t* __restrict__ front, int64_t rows, uint64_t lanes,
+                                               uint64_t* __restrict__ nz_sel, unsigned bid, unsigned nblk) {
+  const int lane = threadIdx.x & 63;
+  const int64_t words = (rows + 63) / 64;
+  const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
+  for (int64_t wd = (int64_t)bid * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
+    const int64_t v = wd * 64 + lane;
+    const uint64_t b = __ballot(v < rows && (front[v] & lanes) != 0);
+    if (lane == 0) nz_sel[wd] = b;
+  }
+}
+
 __device__ __forceinline__ void ms_zero_body(uint64_t* __restrict__ a, int64_t n, unsigned bid, unsigned nblk) {
   for (int64_t i = (int64_t)bid * kBlock + threadIdx.x; i < n; i += (int64_t)nblk * kBlock) a[i] = 0;
 }

@@ -145,14 +145,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull_lanes(uint64_t* __restrict__
 // One bit per vertex whose frontier word has one of `lanes`.
 __global__ void __launch_bounds__(kBlock) k_ms_select(const uint64_t* __restrict__ front, int64_t rows, uint64_t lanes,
                                                       uint64_t* __restrict__ nz_sel) {
-  const int lane = threadIdx.x & 63;
-  const int64_t words = (rows + 63) / 64;
-  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
-  for (int64_t wd = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
-    const int64_t v = wd * 64 + lane;
-    const uint64_t b = __ballot(v < rows && (front[v] & lanes) != 0);
-    if (lane == 0) nz_sel[wd] = b;
-  }
+  ms_select_body(front, rows, lanes, nz_sel, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_ms_collect(const uint32_t* __restrict__ chunk_rows, uint64_t nchunks,

@@ -32,45 +32,77 @@
 
 namespace g5 {
 
-// Writes the level of every newly reached (lane, vertex) pair into the
-// lane's answer array (a wave writes 64 consecutive bytes per lane) and
-// counts each lane's newly reached vertices (block sums in LDS, one atomic
-// per lane and block into lane_nq).
+// Epilogue of a serving level: writes each newly reached (lane, vertex) into
+// the lane's answer array (a wave writes 64 consecutive bytes per lane),
+// counts each lane's new vertices and sums their row lengths (lane_nq[0, 64)
+// and [64, 128)), and marks the vertices with new lanes among `sel` (the
+// top-down lanes) in nz_sel.  The parameters stay in the kernel argument
+// block; ServeLocal holds the per-thread state.
 struct ServeEpi {
   uint8_t* ans[64];
   uint8_t lvl[64];
-  unsigned long long* lane_nq;    // [0, 64): new vertices per lane, [64, 128): their degree sums
+  unsigned long long* lane_nq;
   const uint64_t* offsets;
-  uint64_t* nz_sel;  // per vertex: new lanes among sel (the top-down lanes), one bit
+  uint64_t* nz_sel;
   uint64_t sel;
+};
+
+// Thread i of a wave keeps lane i's answer pointer and level in registers
+// (read by readlane where a wave writes lane r's bytes) and lane i's counts
+// (a readlane walk over the wave's vertices, as LaneEpi), so no shared
+// counter is touched until the block's LDS reduction at the end.
+struct ServeLocal {
+  const ServeEpi& p;
+  mutable uint64_t n_acc = 0, d_acc = 0, my_ans = 0;
+  mutable uint32_t my_lvl = 0;
   __device__ unsigned long long* lds() const {
     __shared__ unsigned long long s_nq[128];
     return s_nq;
   }
   __device__ void begin() const {
+    const int me = threadIdx.x & 63;
+    my_ans = (uint64_t)p.ans[me];
+    my_lvl = p.lvl[me];
     if (threadIdx.x < 128) lds()[threadIdx.x] = 0;
     __syncthreads();
   }
   __device__ void operator()(int64_t v, bool inb, uint64_t nw) const {
-    uint64_t u = wave_or(nw);
-    const uint64_t deg = nw ? offsets[v + 1] - offsets[v] : 0;
+    const int me = threadIdx.x & 63;
+    uint64_t u = wave_or(inb ? nw : 0);
     while (u) {
       const int r = __builtin_ctzll(u);
       u &= u - 1;
-      const bool mine = inb && ((nw >> r) & 1);
-      if (mine) {
-        ans[r][v] = lvl[r];
-        atomicAdd(&lds()[64 + r], (unsigned long long)deg);
-      }
-      const uint64_t b = __ballot(mine);
-      if ((threadIdx.x & 63) == 0) atomicAdd(&lds()[r], (unsigned long long)__popcll(b));
+      const uint32_t alo = __builtin_amdgcn_readlane((uint32_t)my_ans, r);
+      const uint32_t ahi = __builtin_amdgcn_readlane((uint32_t)(my_ans >> 32), r);
+      const uint32_t lv = __builtin_amdgcn_readlane(my_lvl, r);
+      if (inb && ((nw >> r) & 1)) ((uint8_t*)(((uint64_t)ahi << 32) | alo))[v] = (uint8_t)lv;
     }
-    const uint64_t bs = __ballot((nw & sel) != 0);
-    if ((threadIdx.x & 63) == 0) nz_sel[v >> 6] = bs;
+    const uint64_t deg = (inb && nw) ? p.offsets[v + 1] - p.offsets[v] : 0;
+    uint64_t m = __ballot(inb && nw != 0);
+    while (m) {
+      const int t = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const uint32_t wlo = __builtin_amdgcn_readlane((uint32_t)nw, t);
+      const uint32_t whi = __builtin_amdgcn_readlane((uint32_t)(nw >> 32), t);
+      const uint32_t dlo = __builtin_amdgcn_readlane((uint32_t)deg, t);
+      const uint32_t dhi = __builtin_amdgcn_readlane((uint32_t)(deg >> 32), t);
+      const uint32_t bit = me < 32 ? (wlo >> me) & 1 : (whi >> (me - 32)) & 1;
+      if (bit) {
+        ++n_acc;
+        d_acc += ((uint64_t)dhi << 32) | dlo;
+      }
+    }
+    const uint64_t bs = __ballot(inb && (nw & p.sel) != 0);
+    if (me == 0) p.nz_sel[v >> 6] = bs;
   }
   __device__ void end() const {
+    const int me = threadIdx.x & 63;
+    if (n_acc) {
+      atomicAdd(&lds()[me], (unsigned long long)n_acc);
+      atomicAdd(&lds()[64 + me], (unsigned long long)d_acc);
+    }
     __syncthreads();
-    if (threadIdx.x < 128 && lds()[threadIdx.x]) atomicAdd(lane_nq + threadIdx.x, lds()[threadIdx.x]);
+    if (threadIdx.x < 128 && lds()[threadIdx.x]) atomicAdd(p.lane_nq + threadIdx.x, lds()[threadIdx.x]);
   }
 };
 
@@ -88,7 +120,8 @@ __global__ void __launch_bounds__(kBlock) k_sv_pull(uint64_t* __restrict__ seen,
                                                     int filter, unsigned long long* __restrict__ c, uint64_t keep,
                                                     int merge, ServeEpi epi) {
   const FrontView fv{front, nz_front, nullptr, nullptr, filter ? kFrontFilter : kFrontPlain, 0};
-  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x, keep, epi,
+  const ServeLocal el{epi};
+  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x, keep, el,
                merge != 0);
 }
 
@@ -97,7 +130,8 @@ __global__ void __launch_bounds__(kBlock) k_sv_finish(uint64_t* __restrict__ see
                                                       const uint64_t* __restrict__ offsets, int64_t rows,
                                                       unsigned long long* __restrict__ c, uint64_t keep,
                                                       ServeEpi epi) {
-  ms_finish_body(seen, next, nz_next, offsets, rows, c, blockIdx.x, gridDim.x, keep, epi);
+  const ServeLocal el{epi};
+  ms_finish_body(seen, next, nz_next, offsets, rows, c, blockIdx.x, gridDim.x, keep, el);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sv_collect(const uint32_t* __restrict__ chunk_rows, uint64_t nchunks,
@@ -117,6 +151,11 @@ __global__ void __launch_bounds__(kBlock) k_sv_push(const uint32_t* __restrict__
   __shared__ SweepLds L;
   ms_push_list_body(list, count, chunk_rows, nnz, offsets, cols, front, nz_front, seen, next, L, blockIdx.x,
                     gridDim.x, lanes);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sv_select(const uint64_t* __restrict__ front, int64_t rows, uint64_t lanes,
+                                                      uint64_t* __restrict__ nz_sel) {
+  ms_select_body(front, rows, lanes, nz_sel, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sv_zero(uint64_t* __restrict__ a, int64_t n) {
@@ -156,20 +195,39 @@ __global__ void k_sv_admit(const uint32_t* __restrict__ old2new, int64_t rows, S
 __global__ void __launch_bounds__(kBlock) k_sv_teps(const uint64_t* __restrict__ seen,
                                                     const uint32_t* __restrict__ weight, int64_t rows, uint64_t mask,
                                                     unsigned long long* __restrict__ acc) {
-  __shared__ unsigned long long s_acc[64];
-  if (threadIdx.x < 64) s_acc[threadIdx.x] = 0;
+  // vertices reached by every lane of mask add to one sum; for the others,
+  // thread i of a wave sums lane i's weights (readlane walk over the wave's
+  // vertices, no shared counters until the block reduction)
+  __shared__ unsigned long long s_acc[65];
+  if (threadIdx.x < 65) s_acc[threadIdx.x] = 0;
   __syncthreads();
-  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < rows; v += (int64_t)gridDim.x * kBlock) {
-    uint64_t x = seen[v] & mask;
-    if (!x) continue;
-    const uint64_t w = weight[v];
-    while (x) {
-      atomicAdd(&s_acc[__builtin_ctzll(x)], (unsigned long long)w);
-      x &= x - 1;
+  const int me = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  uint64_t full = 0, mine = 0;
+  for (int64_t wd = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); wd * 64 < rows; wd += nwaves) {
+    const int64_t v = wd * 64 + me;
+    const uint64_t x = v < rows ? seen[v] & mask : 0;
+    const uint64_t w = x ? weight[v] : 0;
+    if (x == mask) full += w;
+    uint64_t m = __ballot(x != 0 && x != mask);
+    while (m) {
+      const int t = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const uint32_t xlo = __builtin_amdgcn_readlane((uint32_t)x, t);
+      const uint32_t xhi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), t);
+      const uint32_t wt = __builtin_amdgcn_readlane((uint32_t)w, t);
+      const uint32_t bit = me < 32 ? (xlo >> me) & 1 : (xhi >> (me - 32)) & 1;
+      if (bit) mine += wt;
     }
   }
+  full = wave_sum(full);
+  if (me == 0 && full) atomicAdd(&s_acc[64], (unsigned long long)full);
+  if (mine) atomicAdd(&s_acc[me], (unsigned long long)mine);
   __syncthreads();
-  if (threadIdx.x < 64 && s_acc[threadIdx.x]) atomicAdd(acc + threadIdx.x, s_acc[threadIdx.x]);
+  if (threadIdx.x < 64) {
+    const unsigned long long t = s_acc[threadIdx.x] + (((mask >> threadIdx.x) & 1) ? s_acc[64] : 0);
+    if (t) atomicAdd(acc + threadIdx.x, t);
+  }
 }
 
 // Answer (new ids) -> levels by original id.
@@ -212,7 +270,7 @@ struct Engine {
   uint64_t nchunks = 0;
   DevBuf<uint64_t> seen, fr[2], nz[2];
   DevBuf<uint64_t> nzs[2];  // nonzero bitmap restricted to the lanes that were top-down
-  bool nzs_valid = true;    // false after a lane turned back to top-down
+  uint64_t nzs_lanes[2] = {0, 0};  // the lanes whose frontier vertices nzs[i] marks
   DevBuf<uint32_t> chunk_rows, chunk_list;
   DevBuf<unsigned long long> cnt;    // M_NUM counters, chunk count, 64 + 64 per-lane counts and degree sums
   DevBuf<unsigned long long> tacc;   // per-lane tuple-weight sums of a retirement
@@ -229,6 +287,7 @@ struct Engine {
   bool lane_growing[64];
   int64_t lane_prev_nq[64];
   int64_t lane_visited[64];
+  double lane_vdeg[64];  // row lengths of the vertices each search has reached
   uint64_t active = 0, cooling = 0;
   int level = 0;        // fr[level & 1] is the current frontier
   bool busy = false;    // a level is in flight
@@ -340,9 +399,12 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
     const int64_t nq = (int64_t)e.h[80 + r];
     const double fdeg = (double)e.h[144 + r];
     e.lane_visited[r] += nq;
+    e.lane_vdeg[r] += fdeg;
     const int64_t unvisited = (int64_t)na - e.lane_visited[r];
-    const bool was_fwd = e.lane_fwd[r];
-    if (srv->cfg.lane_rule == 2) {
+    if (srv->cfg.lane_rule == 3) {
+      // pull while the frontier's edges exceed 1/alpha of the edges not yet reached
+      e.lane_fwd[r] = fdeg * srv->cfg.alpha <= (double)e.nnz - e.lane_vdeg[r];
+    } else if (srv->cfg.lane_rule == 2) {
       // one direction for all searches, from the frontier union's edges
       e.lane_fwd[r] = (double)e.last_fdeg < (double)e.nnz / srv->cfg.alpha;
     } else if (srv->cfg.lane_rule == 1) {
@@ -354,7 +416,6 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
       e.lane_fwd[r] = true;
       e.lane_growing[r] = false;
     }
-    if (e.lane_fwd[r] && !was_fwd) e.nzs_valid = false;  // its frontier is not in the top-down bitmap
     e.lane_prev_nq[r] = nq;
   }
   if (finished) {
@@ -422,6 +483,7 @@ static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
     e.lane_growing[r] = true;
     e.lane_prev_nq[r] = 0;
     e.lane_visited[r] = 1;
+    e.lane_vdeg[r] = 0;
     uint8_t* ans = e.bufs[q.buf]->p;
     hipLaunchKernelGGL(k_sv_fill_bytes, dim3(grid_of(e.rows, kBlock, 2048)), dim3(kBlock), 0, e.s, ans, e.rows,
                        (uint8_t)0xFF);
@@ -433,6 +495,7 @@ static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
     hipLaunchKernelGGL(k_sv_admit, dim3(1), dim3(64), 0, e.s, e.ctx->graph.old2new.p, e.rows, rs, e.seen.p,
                        e.fr[e.level & 1].p, e.nz[e.level & 1].p, e.nzs[e.level & 1].p);
     G5_HIP(hipGetLastError());
+    e.nzs_lanes[e.level & 1] |= admitted;
     e.active |= admitted;
   }
   return admitted;
@@ -474,10 +537,16 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
   epi.offsets = g.offsets.p;
   epi.nz_sel = e.nzs[cur ^ 1].p;
   epi.sel = td;
-  // chunks to push: those holding a top-down lane's frontier vertex
-  const uint64_t* nz_push = e.nzs_valid ? e.nzs[cur].p : nz_front;
-  e.nzs_valid = true;
+  // chunks to push: those holding a top-down lane's frontier vertex; the
+  // bitmap from the last level's epilogue (and admissions) serves when it
+  // was made for exactly these lanes, else it is made from the frontier
+  const uint64_t* nz_push = e.nzs[cur].p;
+  const bool reselect = td && e.nzs_lanes[cur] != td;
+  e.nzs_lanes[cur ^ 1] = td;
   G5_HIP(hipEventRecord(e.ev_start, s));
+  if (reselect)
+    hipLaunchKernelGGL(k_sv_select, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, front, e.rows, td,
+                       e.nzs[cur].p);
   G5_HIP(hipMemsetAsync(c, 0, (M_NUM + 1 + 128) * 8, s));
   ++e.st.levels;
   if (td && bu) ++e.st.mixed_levels; else if (td) ++e.st.push_levels; else ++e.st.pull_levels;
@@ -647,7 +716,7 @@ int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_conf
   *out = nullptr;
   if (cfg->answer_buffers < 1 || cfg->answer_buffers > 4096 || !(cfg->alpha > 0) ||
       (cfg->pull_filter != 0 && cfg->pull_filter != 1) || cfg->min_admit < 1 || cfg->min_admit > 64 ||
-      cfg->lane_rule < 0 || cfg->lane_rule > 2 ||
+      cfg->lane_rule < 0 || cfg->lane_rule > 3 ||
       !(cfg->max_wait_s >= 0))
     return G5_ERR_ARG;
   for (int i = 0; i < num_graphs; ++i) {

@@ -462,10 +462,11 @@ class G5Server:
     batching of (graph, root) BFS queries, up to 64 in flight per graph."""
 
     def __init__(self, contexts, answer_buffers=128, alpha=64.0, pull_filter=0, min_admit=64, max_wait_s=0.3,
-                 lane_rule=1):
+                 lane_rule=3):
         """Defaults: a busy graph admits new queries as a cohort of 64 or once
-        the oldest has waited 0.3 s; each search pulls while its frontier has
-        more than nnz/64 edges and pushes otherwise."""
+        the oldest has waited 0.3 s; each search pulls while its frontier's
+        edges exceed 1/64 of the edges it has not reached, and pushes
+        otherwise."""
         self._lib = load_library()
         self._ctxs = list(contexts)  # the server uses their graphs and streams
         arr = (ctypes.c_void_p * len(self._ctxs))(*[c._h for c in self._ctxs])

@@ -263,12 +263,15 @@ typedef struct g5_server_config {
   int32_t answer_buffers;  /* answer arrays preallocated per graph (HBM admission budget) */
   int32_t pull_filter;     /* as g5_batch_tuning.pull_filter */
   double alpha;            /* lane_rule 1: a search pulls when its frontier has more than nnz/alpha
-                              edges, else pushes */
+                              edges, else pushes; lane_rule 3: when its frontier's edges exceed
+                              1/alpha of the edges of the vertices it has not reached */
   int32_t min_admit;       /* a busy graph admits only when at least this many queries can start
                               together (searches started together share their sweeps) ... */
   int32_t lane_rule;       /* 0: each search switches direction by the reference rule on its own
                               frontier size (mpi/bfs.hpp:2799-2829); 1: by its frontier's edges;
-                              2: all searches together, by the frontier union's edges */
+                              2: all searches together, by the frontier union's edges; 3: by its
+                              frontier's edges against its unreached edges (as g5_batch_tuning
+                              lane_rule 1) */
   double max_wait_s;       /* ... or when its oldest waiting query has waited this long */
 } g5_server_config;
 

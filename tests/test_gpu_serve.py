@@ -55,7 +55,7 @@ def _queries(specs):
     return q
 
 
-@pytest.mark.parametrize("lane_rule", [0, 1, 2])
+@pytest.mark.parametrize("lane_rule", [0, 1, 2, 3])
 def test_serve_answers_equal_single_root(graphs, lane_rule):
     import graph500_amd
     rng = np.random.default_rng(7)
