@@ -43,6 +43,9 @@ Other modes:
                           FETCH_SIZE/WRITE_SIZE counters)
   --rocprof-summary DB    per-kernel time, or per-kernel counter averages, of a
                           rocprofv3 results database (the text kept in profiles/)
+  --rocprof-dispatches DB KERNEL
+                          per-dispatch durations or counters of one kernel
+  --tuning JSON           run the batch with this tuning instead of autotuning
 """
 import argparse
 import json
@@ -151,6 +154,28 @@ def rocprof_summary(db, top=30):
         lines.append("%-48s %8s %12s %12s %7s" % ("kernel", "calls", "total ms", "avg us", "pct"))
         for name, calls, tot, avg, pct in rows[:top]:  # the view's durations are in microseconds
             lines.append("%-48s %8d %12.3f %12.2f %7.2f" % (name.split("(")[0][:48], calls, tot / 1e3, avg, pct))
+    return "\n".join(lines)
+
+
+def rocprof_dispatches(db, pattern):
+    """Per-dispatch values, in dispatch order, of the kernels whose name
+    contains `pattern`: duration (ms) from a --kernel-trace database, or each
+    counter from a --pmc database."""
+    import sqlite3
+    cur = sqlite3.connect(db).cursor()
+    tables = {r[0] for r in cur.execute("select name from sqlite_master where type in ('table','view')")}
+    lines = []
+    if "counters_collection" in tables and cur.execute("select count(*) from counters_collection").fetchone()[0]:
+        cols = {r[1] for r in cur.execute("pragma table_info(counters_collection)")}
+        order = "dispatch_id" if "dispatch_id" in cols else "rowid"
+        for k, c, v in cur.execute("select kernel_name, counter_name, value from counters_collection "
+                                   "order by %s" % order):
+            if pattern in k:
+                lines.append("%-40s %-12s %.1f" % (k.split("(")[0][:40], c, v))
+    else:
+        for k, s, e in cur.execute("select name, start, end from kernels order by start"):
+            if pattern in k:
+                lines.append("%-40s %.4f" % (k.split("(")[0][:40], (e - s) / 1e6))
     return "\n".join(lines)
 
 
@@ -291,10 +316,16 @@ def main():
     ap.add_argument("--autotune-budget", type=float, default=30.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--rocprof-summary", metavar="DB")
+    ap.add_argument("--rocprof-dispatches", nargs=2, metavar=("DB", "KERNEL"))
+    ap.add_argument("--tuning", type=json.loads, default=None,
+                    help="batch tuning as JSON (skips the autotuner), e.g. a previous run's batch.tuning")
     a = ap.parse_args()
 
     if a.rocprof_summary:
         print(rocprof_summary(a.rocprof_summary))
+        return 0
+    if a.rocprof_dispatches:
+        print(rocprof_dispatches(*a.rocprof_dispatches))
         return 0
     if a.copy_calibration:
         copy_calibration()
@@ -322,7 +353,9 @@ def main():
 
     ctx.batch_prepare()
     tuned = None
-    if not a.no_autotune:
+    if a.tuning:
+        ctx.batch_set_tuning(**a.tuning)
+    elif not a.no_autotune:
         from graph500_amd import autotune
         tuned = autotune.autotune(ctx, roots, reps=2, budget_s=a.autotune_budget, use_cache=False, log=log)
         log("autotune: %s -> %.3f ms (%d trials, %.1f s)" % (tuned["tuning"], tuned["ms"], tuned["trials"],
