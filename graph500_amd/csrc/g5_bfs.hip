@@ -92,6 +92,8 @@ __global__ void __launch_bounds__(kBlock) k_td_expand(
   __shared__ uint32_t s_qn;
   __shared__ uint64_t s_qbase;
   const uint64_t E = prefix[nf];
+  // both forms are launched before the host knows E; the other one exits
+  if (kBulk ? E <= kTdSmall : E > kTdSmall) return;
   for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
     const uint64_t e0 = chunk * kEpb;
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
@@ -283,23 +285,50 @@ __global__ void __launch_bounds__(kBlock) k_td_bin_apply(const uint64_t* __restr
 
 // After a bulk top-down level: the vertices whose level became 'lvl' join the
 // visited bitmap and form the frontier bitmap; their number goes to counters.
+// A wave takes 16 bitmap words (1024 vertices) per step: lane i reads 16
+// level bytes with one 16-byte load (1 KiB per wave instruction), and lanes
+// 4k..4k+3 assemble word k with two shuffles.
+__device__ __forceinline__ uint32_t bytes_equal4(uint32_t x, uint32_t pat) {
+  const uint32_t y = x ^ pat;
+  uint32_t r = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) r |= (((y >> (8 * b)) & 0xFFu) == 0 ? 1u : 0u) << b;
+  return r;
+}
+
 __global__ void __launch_bounds__(kBlock) k_td_finish(const uint8_t* __restrict__ level, int64_t rows,
                                                       int64_t words, uint32_t lvl, uint64_t* __restrict__ vis,
                                                       uint64_t* __restrict__ fbits,
-                                                      unsigned long long* __restrict__ counters) {
+                                                      unsigned long long* __restrict__ counters,
+                                                      const uint64_t* __restrict__ prefix, int64_t nf) {
+  if (prefix[nf] <= kTdSmall) return;  // a small level: k_td_expand<false> kept the queue exact
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  const int64_t groups = (words + 15) / 16;
+  const uint32_t pat = (lvl & 0xFFu) * 0x01010101u;
   uint64_t total = 0;
-  for (int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < words; w += nwaves) {
-    const int64_t v = w * 64 + lane;
-    const uint64_t nb = __ballot(v < rows && level[v] == (uint8_t)lvl);
-    if (lane == 0) {
-      vis[w] |= nb;
-      fbits[w] = nb;
+  for (int64_t gi = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); gi < groups; gi += nwaves) {
+    const int64_t v0 = gi * 1024 + lane * 16;
+    uint32_t m16 = 0;
+    if (v0 + 16 <= rows) {
+      const uint4 q = *reinterpret_cast<const uint4*>(level + v0);  // v0 is a multiple of 16
+      m16 = bytes_equal4(q.x, pat) | (bytes_equal4(q.y, pat) << 4) | (bytes_equal4(q.z, pat) << 8) |
+            (bytes_equal4(q.w, pat) << 12);
+    } else {
+      for (int j = 0; j < 16; ++j)
+        if (v0 + j < rows && level[v0 + j] == (uint8_t)lvl) m16 |= 1u << j;
     }
-    total += __popcll(nb);
+    uint64_t m = (uint64_t)m16 << (16 * (lane & 3));
+    m |= __shfl_xor(m, 1, 64);
+    m |= __shfl_xor(m, 2, 64);
+    const int64_t w = gi * 16 + (lane >> 2);
+    if ((lane & 3) == 0 && w < words) {
+      vis[w] |= m;
+      fbits[w] = m;
+      total += __popcll(m);
+    }
   }
-  block_add(lane == 0 ? total : 0, counters + C_FOUND);
+  block_add(total, counters + C_FOUND);
 }
 
 __global__ void __launch_bounds__(kBlock) k_bu(
@@ -520,52 +549,59 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     const bool forward = next_forward;
     G5_HIP(hipMemsetAsync(b.counters.p, 0, 8, s));
     G5_HIP(hipEventRecord(b.lev_begin[lvl], s));
-    bool bulk = false;
     if (forward) {
       device_scan(FrontierDegree{b.queue[qi].p, g.offsets.p, nf}, StoreU64{b.prefix.p},
                   (size_t)nf + 1, b.scan_tmp.p, s);
-      G5_HIP(hipMemcpyAsync(b.h_counters + 1, b.prefix.p + nf, 8, hipMemcpyDeviceToHost, s));
-      G5_HIP(hipStreamSynchronize(s));
-      const uint64_t E = b.h_counters[1];
-      const unsigned grid = grid_of((int64_t)E, kEpb, 2048);
-      const uint64_t nchunks = (E + kEpb - 1) / kEpb;
-      G5_CHECK(2 * nchunks <= b.chunk_rows.n, "g5_run_bfs: top-down chunk table too small");
-      if (nchunks)
-        hipLaunchKernelGGL(k_chunk_rows<kEpb>, dim3(grid_of((int64_t)(2 * nchunks), kBlock, 65536)), dim3(kBlock),
-                           0, s, b.prefix.p, nf, nchunks, b.chunk_rows.p);
-      bulk = E > kTdSmall;
-      if (!bulk) {
-        hipLaunchKernelGGL(k_td_expand<false>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf,
-                           b.prefix.p, b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p,
-                           b.parent.p, b.queue[qi ^ 1].p, cnt, (uint32_t)lvl);
-      } else {
-        if (b.bin_edges > 0 && E >= (uint64_t)b.bin_edges) {
-          // binned by target range (see k_td_bin_count)
-          int shift = kBinShift;
-          while ((rows >> shift) >= kBinMaxBuckets) ++shift;
-          const int nb = (int)(((rows - 1) >> shift) + 1);
-          b.bin_pairs.alloc(E);
-          b.bin_cnt.alloc(2 * kBinMaxBuckets);
-          unsigned long long* bc = (unsigned long long*)b.bin_cnt.p;
-          G5_HIP(hipMemsetAsync(bc, 0, nb * 8, s));
-          hipLaunchKernelGGL(k_td_bin_count, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
-                             b.chunk_rows.p, g.offsets.p, g.cols.p, shift, nb, bc);
-          hipLaunchKernelGGL(k_td_bin_offsets, dim3(1), dim3(kBlock), 0, s, bc, nb, bc + kBinMaxBuckets);
-          hipLaunchKernelGGL(k_td_bin_scatter, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
-                             b.chunk_rows.p, g.offsets.p, g.cols.p, shift, nb, bc + kBinMaxBuckets, b.bin_pairs.p);
-          hipLaunchKernelGGL(k_td_bin_apply, dim3(grid_of((int64_t)E, kBlock, 2048)), dim3(kBlock), 0, s,
-                             b.bin_pairs.p, E, b.level.p, b.parent.p, (uint32_t)lvl);
-        } else {
-          hipLaunchKernelGGL(k_td_expand<true>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf,
-                             b.prefix.p, b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p,
-                             b.parent.p, (uint32_t*)nullptr, cnt, (uint32_t)lvl);
-        }
-        G5_HIP(hipGetLastError());
-        // k_td_finish makes vis[cur^1] exactly this level's vertices and adds
-        // them to vis[cur]
-        hipLaunchKernelGGL(k_td_finish, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
-                           b.level.p, rows, b.words, (uint32_t)lvl, b.vis[cur].p, b.vis[cur ^ 1].p, cnt);
+      if (b.bin_edges > 0) {
+        // the binned form sizes its pair buffer from E: wait for it
+        G5_HIP(hipMemcpyAsync(b.h_counters + 1, b.prefix.p + nf, 8, hipMemcpyDeviceToHost, s));
+        G5_HIP(hipStreamSynchronize(s));
       }
+      const uint64_t E_known = b.bin_edges > 0 ? b.h_counters[1] : 0;
+      const bool binned = b.bin_edges > 0 && E_known > kTdSmall && E_known >= (uint64_t)b.bin_edges;
+      // Without binning no launch waits for E on the host: chunk rows and the
+      // expansion take upper-bound grids and read E (prefix[nf]) on the
+      // device; the small and the bulk expansion are both launched and the
+      // one that does not fit E exits at once, as does k_td_finish after a
+      // small level.  The host learns E with the level's counters.
+      const uint64_t max_chunks = b.chunk_rows.n / 2;  // sized for every edge of the graph
+      const uint64_t nchunks = b.bin_edges > 0 ? (E_known + kEpb - 1) / kEpb : max_chunks;
+      const unsigned grid = b.bin_edges > 0 ? grid_of((int64_t)E_known, kEpb, 2048) : 2048u;
+      if (nchunks)
+        hipLaunchKernelGGL(k_chunk_rows<kEpb>,
+                           dim3(b.bin_edges > 0 ? grid_of((int64_t)(2 * nchunks), kBlock, 65536) : 4096u),
+                           dim3(kBlock), 0, s, b.prefix.p, nf, nchunks, b.chunk_rows.p);
+      hipLaunchKernelGGL(k_td_expand<false>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
+                         b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p, b.parent.p,
+                         b.queue[qi ^ 1].p, cnt, (uint32_t)lvl);
+      if (binned) {
+        // binned by target range (see k_td_bin_count)
+        const uint64_t E = E_known;
+        int shift = kBinShift;
+        while ((rows >> shift) >= kBinMaxBuckets) ++shift;
+        const int nb = (int)(((rows - 1) >> shift) + 1);
+        b.bin_pairs.alloc(E);
+        b.bin_cnt.alloc(2 * kBinMaxBuckets);
+        unsigned long long* bc = (unsigned long long*)b.bin_cnt.p;
+        G5_HIP(hipMemsetAsync(bc, 0, nb * 8, s));
+        hipLaunchKernelGGL(k_td_bin_count, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
+                           b.chunk_rows.p, g.offsets.p, g.cols.p, shift, nb, bc);
+        hipLaunchKernelGGL(k_td_bin_offsets, dim3(1), dim3(kBlock), 0, s, bc, nb, bc + kBinMaxBuckets);
+        hipLaunchKernelGGL(k_td_bin_scatter, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
+                           b.chunk_rows.p, g.offsets.p, g.cols.p, shift, nb, bc + kBinMaxBuckets, b.bin_pairs.p);
+        hipLaunchKernelGGL(k_td_bin_apply, dim3(grid_of((int64_t)E, kBlock, 2048)), dim3(kBlock), 0, s,
+                           b.bin_pairs.p, E, b.level.p, b.parent.p, (uint32_t)lvl);
+      } else {
+        hipLaunchKernelGGL(k_td_expand<true>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
+                           b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p, b.parent.p,
+                           (uint32_t*)nullptr, cnt, (uint32_t)lvl);
+      }
+      G5_HIP(hipGetLastError());
+      // after a bulk level, k_td_finish makes vis[cur^1] exactly this level's
+      // vertices and adds them to vis[cur]
+      hipLaunchKernelGGL(k_td_finish, dim3(grid_of((b.words + 15) / 16, kBlock / 64, 8192)), dim3(kBlock), 0, s,
+                         b.level.p, rows, b.words, (uint32_t)lvl, b.vis[cur].p, b.vis[cur ^ 1].p, cnt, b.prefix.p,
+                         nf);
     } else {
       hipLaunchKernelGGL(k_bu, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
                          b.vis[cur].p, b.vis[cur ^ 1].p, g.first_nbr.p, g.offsets.p, g.cols.p,
@@ -574,8 +610,10 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(b.lev_end[lvl], s));
     G5_HIP(hipMemcpyAsync(b.h_counters, b.counters.p, 8, hipMemcpyDeviceToHost, s));
+    if (forward) G5_HIP(hipMemcpyAsync(b.h_counters + 1, b.prefix.p + nf, 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     const int64_t nq = (int64_t)b.h_counters[0];
+    const bool bulk = forward && b.h_counters[1] > kTdSmall;
     {
       float lms = 0;
       G5_HIP(hipEventElapsedTime(&lms, b.lev_begin[lvl], b.lev_end[lvl]));

@@ -55,7 +55,9 @@ template <int kChunk>
 __global__ void __launch_bounds__(kBlock) k_chunk_rows(const uint64_t* __restrict__ prefix, int64_t nf,
                                                        uint64_t nchunks, uint32_t* __restrict__ rows_out) {
   const uint64_t E = prefix[nf];
-  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < 2 * nchunks;
+  // nchunks may be an upper bound (a launch made before E is known on the host)
+  const uint64_t nc = (E + kChunk - 1) / kChunk < nchunks ? (E + kChunk - 1) / kChunk : nchunks;
+  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < 2 * nc;
        t += (uint64_t)gridDim.x * kBlock) {
     const uint64_t c = t >> 1;
     uint64_t key = c * kChunk;
