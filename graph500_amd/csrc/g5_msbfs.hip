@@ -487,6 +487,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
   const bool lanes_mode = m.lane_rule == 1 && m.sweep_chunks && l0 == 1;
   m.level_masks = lanes_mode;
   double lane_fd[64] = {}, lane_vd[64] = {};
+  uint64_t pulled = 0;  // roots that have pulled: they pull from then on
   const uint64_t* h_lane = m.h_counters + M_NUM + 1;
   unsigned* d_list_count = (unsigned*)(cnt + M_NUM);
   int l = l0 - 1;
@@ -500,19 +501,24 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     bool push = (double)fdeg < push_limit && total_pairs - reached >= unseen_frac * total_pairs;
     bool sweep = !push && m.sweep_chunks && reached < sweep_limit;
     uint64_t td = 0, bu = 0;
+    bool lane_stats = false;  // this level's epilogue gathers per-root statistics
     if (lanes_mode) {
       // pull when the frontier's edges exceed 1/alpha of the edges not yet
-      // reached (the roots' own level pushes; all pull in the beta tail)
+      // reached (the roots' own level pushes; all pull in the beta tail); a
+      // root that has pulled keeps pulling, so once every live root pulls
+      // the level is a plain pull with no per-root statistics
       const bool tail = total_pairs - reached < unseen_frac * total_pairs;
       for (int r = 0; r < 64; ++r) {
         const uint64_t bit = UINT64_C(1) << r;
         if (!(live & bit)) continue;
         const double unreached = (double)g.nnz - lane_vd[r];
-        const bool pull_r = tail || (l > 1 && lane_fd[r] * m.alpha > unreached);
+        const bool pull_r = tail || (pulled & bit) || (l > 1 && lane_fd[r] * m.alpha > unreached);
         (pull_r ? bu : td) |= bit;
       }
+      pulled |= bu;
       push = bu == 0;
       sweep = false;
+      lane_stats = td != 0;
     }
     m.level_push[l] = push ? live : td;
     m.level_pull[l] = push ? 0 : (lanes_mode ? bu : live);
@@ -536,10 +542,6 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       hipLaunchKernelGGL(k_ms_pull_lanes, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
                          m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, bu,
                          m.pull_filter == 1 ? 1 : 0, 1, cnt, lepi);
-    } else if (lanes_mode && !push) {
-      hipLaunchKernelGGL(k_ms_pull_lanes, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
-                         m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, bu,
-                         m.pull_filter == 1 ? 1 : 0, 0, cnt, lepi);
     } else if (push) {
       device_scan(UnionCount{nz_front}, UnionEmit{nz_front, m.queue.p}, (size_t)words, m.scan_tmp.p, s);
       const int64_t nf = (int64_t)nz;
@@ -554,7 +556,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)fdeg, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p,
                          nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, m.seen.p, next);
       G5_HIP(hipGetLastError());
-      if (lanes_mode)
+      if (lane_stats)
         hipLaunchKernelGGL(k_ms_finish_lanes, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
                            next, nz_next, g.offsets.p, rows, cnt, lepi);
       else
@@ -594,14 +596,14 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(m.lev_end[l], s));
-    G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, (lanes_mode ? M_NUM + 1 + 128 : M_NUM) * 8,
+    G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, (lane_stats ? M_NUM + 1 + 128 : M_NUM) * 8,
                           hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     float lms = 0;
     G5_HIP(hipEventElapsedTime(&lms, m.lev_begin[l], m.lev_end[l]));
     st.level_ms[l] = lms;
     st.direction[l] = (lanes_mode && td && bu) ? 3 : (push ? 1 : (sweep ? 2 : 0));
-    if (lanes_mode)
+    if (lane_stats)
       for (int r = 0; r < 64; ++r) {
         lane_fd[r] = (double)h_lane[64 + r];
         lane_vd[r] += (double)h_lane[64 + r];
