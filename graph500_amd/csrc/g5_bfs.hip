@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(kBlock) k_td_expand(
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
     if (!kBulk && threadIdx.x == 0) s_qn = 0;
     const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
-    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
+    const auto visit = [&](uint32_t w, uint32_t u) {
       const uint64_t bit = UINT64_C(1) << (w & 63);
       // a hub level's snapshot holds few vertices: the level test alone decides
       if (!kBulk && (vis[w >> 6] & bit)) return;
@@ -114,7 +114,8 @@ __global__ void __launch_bounds__(kBlock) k_td_expand(
         parent[w] = u;
         s_q[atomicAdd(&s_qn, 1u)] = w;
       }
-    });
+    };
+    walk_strided(e0, e1, cnt, cr, cols, visit);
     if (!kBulk) {
       __syncthreads();
       if (threadIdx.x == 0) s_qbase = s_qn ? atomicAdd(counters + C_FOUND, (unsigned long long)s_qn) : 0;
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(kBlock) k_td_bin_count(const uint32_t* __restr
     const uint64_t e0 = chunk * kEpb;
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
     const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
-    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t) { atomicAdd(&s_hist[w >> shift], 1u); });
+    walk_strided(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t) { atomicAdd(&s_hist[w >> shift], 1u); });
     __syncthreads();
   }
   for (int i = threadIdx.x; i < nb; i += kBlock)
@@ -254,12 +255,12 @@ __global__ void __launch_bounds__(kBlock) k_td_bin_scatter(const uint32_t* __res
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
     for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0;
     const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);  // ends with a barrier
-    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t) { atomicAdd(&s_hist[w >> shift], 1u); });
+    walk_strided(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t) { atomicAdd(&s_hist[w >> shift], 1u); });
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += kBlock)
       s_cur[i] = s_hist[i] ? atomicAdd(bucket_cur + i, (unsigned long long)s_hist[i]) : 0;
     __syncthreads();
-    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
+    walk_strided(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
       const unsigned long long pos = atomicAdd(&s_cur[w >> shift], 1ull);
       pairs[pos] = ((uint64_t)w << 32) | u;
     });

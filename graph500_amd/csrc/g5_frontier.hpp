@@ -119,6 +119,28 @@ __device__ __forceinline__ void walk_lane(uint64_t e0, uint64_t e1, int cnt, con
   }
 }
 
+// Calls f(target, source) for the edges e0 + threadIdx.x + j * kBlock of
+// [e0, e1): consecutive lanes take consecutive edges, so a wave reads 256
+// contiguous bytes of cols per step and, within a sorted hub row, probes
+// targets that share few cache lines.  A lane finds each edge's row by
+// binary search past the row of its previous edge (none inside a long row).
+template <class F>
+__device__ __forceinline__ void walk_strided(uint64_t e0, uint64_t e1, int cnt, const ChunkRows& cr,
+                                             const uint32_t* __restrict__ cols, F&& f) {
+  int lo = 0;
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+    if (lo + 1 < cnt && cr.pre[lo + 1] <= e) {
+      int a = lo + 1, b = cnt - 1;  // last row starting at or before e
+      while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (cr.pre[mid] <= e) a = mid; else b = mid - 1;
+      }
+      lo = a;
+    }
+    f(cols[cr.base[lo] + e], cr.src[lo]);
+  }
+}
+
 // Tuple weight of a reached vertex (0xFF = unreached); the traversed-edge
 // count of a run is the sum over all rows, halved.
 struct ReachedWeight {

@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__
     const uint64_t e0 = chunk * kEpb;
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
     const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
-    walk_lane(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
+    walk_strided(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
       const uint64_t m = front[u] & ~seen[w];
       if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
         atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
