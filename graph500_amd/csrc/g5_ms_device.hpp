@@ -452,24 +452,27 @@ __device__ __forceinline__ void push_chunk(uint64_t chunk, const uint32_t* __res
   const uint64_t e1 = (e0 + kSweepEpb < nnz) ? e0 + kSweepEpb : nnz;
   const int nr = sweep_stage(chunk, chunk_rows, offsets, L, false);
   const int len = (int)(e1 - e0);
-  const int my0 = threadIdx.x * kSweepEpt;  // relative to e0
-  if (my0 < len) {
-    int lo = sweep_row_of(L, nr, my0);
-    const int my1 = (my0 + kSweepEpt < len) ? my0 + kSweepEpt : len;
-    uint32_t u = r0 + lo;
-    uint64_t fu = bit_set(nz_front, u) ? front[u] & lanes : 0;
-    for (int e = my0; e < my1; ++e) {
-      if (lo + 1 < nr && L.off[lo + 1] <= e) {
-        while (lo + 1 < nr && L.off[lo + 1] <= e) ++lo;
-        u = r0 + lo;
-        fu = bit_set(nz_front, u) ? front[u] & lanes : 0;
+  // consecutive lanes take consecutive entries (coalesced columns; targets
+  // of a sorted long row close together); a lane re-finds its row only when
+  // an entry lies past it
+  int lo = -1;
+  uint64_t fu = 0;
+  for (int e = threadIdx.x; e < len; e += kBlock) {
+    if (lo < 0 || (lo + 1 < nr && L.off[lo + 1] <= e)) {
+      int a = lo < 0 ? 0 : lo + 1, b = nr - 1;  // last staged row starting at or before e
+      while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (L.off[mid] <= e) a = mid; else b = mid - 1;
       }
-      if (!fu) continue;
-      const uint32_t w = cols[e0 + e];
-      const uint64_t m = fu & ~seen[w];
-      if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
-        atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
+      lo = a;
+      const uint32_t u = r0 + lo;
+      fu = bit_set(nz_front, u) ? front[u] & lanes : 0;
     }
+    if (!fu) continue;
+    const uint32_t w = cols[e0 + e];
+    const uint64_t m = fu & ~seen[w];
+    if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
+      atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
   }
   __syncthreads();
 }
