@@ -479,6 +479,20 @@ int g5_batch_model_bytes(g5_ctx* ctx, double* per_level, double* total) {
   });
 }
 
+int g5_batch_level_lanes(g5_ctx* ctx, uint64_t* push_lanes, uint64_t* pull_lanes) {
+  return guarded(ctx, [&] {
+    const MsState& m = ctx->ms;
+    G5_CHECK(m.have_result, "g5_batch_level_lanes: no batch result");
+    G5_CHECK(push_lanes && pull_lanes, "g5_batch_level_lanes: null output");
+    const uint64_t all = m.nroots == 64 ? ~UINT64_C(0) : ((UINT64_C(1) << m.nroots) - 1);
+    for (int l = 0; l < G5_MAX_LEVELS; ++l) {
+      const bool run = l >= 1 && l < m.stats.num_levels;
+      push_lanes[l] = run && m.level_masks ? m.level_push[l] & all : 0;
+      pull_lanes[l] = run ? (m.level_masks ? m.level_pull[l] & all : all) : 0;
+    }
+  });
+}
+
 int g5_batch_end(g5_ctx* ctx) {
   return guarded(ctx, [&] {
     set_device(ctx);

@@ -39,6 +39,7 @@ EXPORTS = [
     "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_bfs_set_tuning", "g5_bfs_get_tuning", "g5_validate", "g5_validate_pred", "g5_end_bfs",
     "g5_batch_prepare", "g5_batch_set_tuning", "g5_batch_get_tuning", "g5_run_bfs_batch",
     "g5_batch_get_levels", "g5_batch_teps_edges", "g5_batch_stats_get", "g5_batch_model_bytes",
+    "g5_batch_level_lanes",
     "g5_batch_end", "g5_server_create", "g5_server_destroy", "g5_server_last_error", "g5_server_run",
     "g5_server_get_levels", "g5_server_release", "g5_server_graph_stats_get",
 ]
@@ -185,6 +186,7 @@ def load_library(path=LIB_PATH):
         "g5_batch_teps_edges": ([vp, vp], i32),
         "g5_batch_stats_get": ([vp, ctypes.POINTER(BatchStats)], i32),
         "g5_batch_model_bytes": ([vp, vp, ctypes.POINTER(ctypes.c_double)], i32),
+        "g5_batch_level_lanes": ([vp, vp, vp], i32),
         "g5_batch_end": ([vp], i32),
         "g5_server_create": ([vp, i32, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], i32),
         "g5_server_destroy": ([vp], None),
@@ -445,6 +447,13 @@ class G5Context:
         self._check(self._lib.g5_batch_model_bytes(self._h, _ptr(per), ctypes.byref(tot)))
         L = self.batch_stats()["num_levels"]
         return per[1:L].tolist(), tot.value
+
+    def batch_level_lanes(self):
+        """(push, pull): per-level lane masks of the last batch (index = level)."""
+        push = np.zeros(MAX_LEVELS, np.uint64)
+        pull = np.zeros(MAX_LEVELS, np.uint64)
+        self._check(self._lib.g5_batch_level_lanes(self._h, _ptr(push), _ptr(pull)))
+        return push, pull
 
     def batch_end(self):
         self._check(self._lib.g5_batch_end(self._h))

@@ -244,6 +244,11 @@ int g5_batch_stats_get(g5_ctx* ctx, g5_batch_stats* out);
  * Under lane_rule 1 only the roots pulled at a level count as lacking, and
  * every vertex whose frontier word holds a pushed root adds 8 + 12 deg. */
 int g5_batch_model_bytes(g5_ctx* ctx, double* per_level, double* total);
+/* Per level of the last batch (G5_MAX_LEVELS entries each, index = level):
+ * the roots that pushed and the roots costed as pulling in the byte model
+ * (no counterpart in the reference; with lane_rule 0 no root pushes and every
+ * root counts as pulling). */
+int g5_batch_level_lanes(g5_ctx* ctx, uint64_t* push_lanes, uint64_t* pull_lanes);
 int g5_batch_end(g5_ctx* ctx);
 
 /* ---- serving: continuous batching over resident graphs ----

@@ -523,7 +523,7 @@ int or_bfs(const or_csr* g, int64_t root_orig, uint8_t* level_by_orig, int64_t* 
 }
 
 double or_msbfs_bytes(const or_csr* g, int lanes, const uint8_t* levels_new, int* num_levels,
-                      double* per_level) {
+                      double* per_level, const uint64_t* pull_lanes, const uint64_t* push_lanes) {
   const int64_t n = g->num_verts, na = g->num_active;
   int maxl = 0;
   for (int r = 0; r < lanes; ++r)
@@ -547,12 +547,16 @@ double or_msbfs_bytes(const or_csr* g, int lanes, const uint8_t* levels_new, int
       front[v] = f;
     }
     double bytes = 24.0 * (double)n;
+    const uint64_t pull_mask = pull_lanes ? pull_lanes[l] : ~(uint64_t)0;
+    const uint64_t push_mask = push_lanes ? push_lanes[l] : 0;
     for (int64_t v = 0; v < na; ++v) {
+      if (front[v] & push_mask) bytes += 8 + 12.0 * (double)(g->offsets[v + 1] - g->offsets[v]);
       uint64_t need = 0;
       for (int r = 0; r < lanes; ++r) {
         uint8_t lv = levels_new[(size_t)r * n + v];
         if (lv == 0xFF || lv >= l) need |= (uint64_t)1 << r;
       }
+      need &= pull_mask;
       if (!need) continue;
       uint64_t acc = 0;
       int64_t k = 0;

@@ -144,9 +144,12 @@ int or_bfs(const or_csr* g, int64_t root, uint8_t* level_by_orig, int64_t* pred_
 
 /* Batched algorithmic bytes for a 64-lane bit-packed pull sweep given the
  * per-lane levels (by NEW id, 0xFF unreached). levels is lanes x num_verts.
- * Returns total bytes; per-level bytes into per_level (may be NULL). */
+ * Returns total bytes; per-level bytes into per_level (may be NULL).
+ * pull_lanes / push_lanes (indexed by level, may be NULL): with per-root
+ * directions only the pulled lanes count as needed, and a vertex whose
+ * frontier word holds a pushed lane adds 8 + 12 deg (NULL: every lane pulls). */
 double or_msbfs_bytes(const or_csr* g, int lanes, const uint8_t* levels_new, int* num_levels,
-                      double* per_level);
+                      double* per_level, const uint64_t* pull_lanes, const uint64_t* push_lanes);
 
 /* ---------------- Validation (mpi/validate.hpp) ---------------- */
 

@@ -92,7 +92,8 @@ def lib():
     L.or_bfs.argtypes = [ctypes.POINTER(OrCsr), ctypes.c_int64, ctypes.c_void_p,
                          ctypes.c_void_p, ctypes.POINTER(OrBfsTrace)]
     L.or_msbfs_bytes.argtypes = [ctypes.POINTER(OrCsr), ctypes.c_int, ctypes.c_void_p,
-                                 ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
+                                 ctypes.POINTER(ctypes.c_int), ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p]
     L.or_msbfs_bytes.restype = ctypes.c_double
     L.or_validate.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                               ctypes.c_int64, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
@@ -234,13 +235,25 @@ class Csr:
         }
         return lv, pred, trace
 
-    def msbfs_bytes(self, levels_new):
+    def msbfs_bytes(self, levels_new, pull_lanes=None, push_lanes=None):
+        """pull_lanes / push_lanes: per-level lane masks (index = level) of a
+        run with per-root directions; None = every lane pulls."""
         levels_new = np.ascontiguousarray(levels_new, np.uint8)
         lanes = levels_new.shape[0]
         nl = ctypes.c_int(0)
         per = np.zeros(OR_MAX_LEVELS + 2, np.float64)
+        masks = []
+        for m in (pull_lanes, push_lanes):
+            if m is None:
+                masks.append(None)
+            else:
+                a = np.zeros(OR_MAX_LEVELS + 2, np.uint64)
+                m = np.asarray(m, np.uint64)
+                a[: min(len(m), len(a))] = m[: len(a)]
+                masks.append(a)
+        ptrs = [None if m is None else m.ctypes.data_as(ctypes.c_void_p) for m in masks]
         tot = lib().or_msbfs_bytes(self._p, lanes, levels_new.ctypes.data_as(ctypes.c_void_p),
-                                   ctypes.byref(nl), per.ctypes.data_as(ctypes.c_void_p))
+                                   ctypes.byref(nl), per.ctypes.data_as(ctypes.c_void_p), ptrs[0], ptrs[1])
         return tot, per[: nl.value]
 
 

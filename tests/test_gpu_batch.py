@@ -127,11 +127,38 @@ def test_batch_model_bytes_match_oracle(s16ctx, oracle_mod):
     roots = oc.find_roots(64)
     s16ctx.batch_set_tuning(lane_rule=0, alpha=16.0, persistent=0)  # the oracle's model: every level a pull
     s16ctx.run_bfs_batch(roots)
+    push, pull = s16ctx.batch_level_lanes()
+    assert not push.any()
     levels_new = np.stack([s16ctx.batch_levels(i)[a["new2old"]] for i in range(64)])
     tot_o, per_o = oc.msbfs_bytes(levels_new)
     per, tot = s16ctx.batch_model_bytes()
     assert tot == tot_o
     assert per == per_o[1:len(per) + 1].tolist()
+
+
+@pytest.mark.parametrize("alpha", [4.0, 128.0])
+def test_batch_model_bytes_per_root_rule_match_oracle(s16ctx, oracle_mod, alpha):
+    """Byte model of a run with per-root directions (levels where some roots
+    push and the others pull) against the oracle given the run's lane masks."""
+    O = oracle_mod
+    oc = O.Csr(O.generate_edges(16))
+    a = oc.arrays()
+    roots = oc.find_roots(64)
+    s16ctx.batch_set_tuning(lane_rule=1, alpha=alpha, persistent=0, beta=50.0)
+    try:
+        s16ctx.run_bfs_batch(roots)
+        st = s16ctx.batch_stats()
+        push, pull = s16ctx.batch_level_lanes()
+        if alpha == 4.0:
+            assert 3 in st["direction"], st["direction"]
+            assert any(push[l] and pull[l] for l in range(1, st["num_levels"]))
+        levels_new = np.stack([s16ctx.batch_levels(i)[a["new2old"]] for i in range(64)])
+        tot_o, per_o = oc.msbfs_bytes(levels_new, pull.tolist(), push.tolist())
+        per, tot = s16ctx.batch_model_bytes()
+        assert per == per_o[1:len(per) + 1].tolist()
+        assert tot == tot_o
+    finally:
+        s16ctx.batch_set_tuning(lane_rule=0, alpha=16.0)
 
 
 def test_batch_partial_and_odd_roots(oracle_mod):

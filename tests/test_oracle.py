@@ -266,7 +266,11 @@ def test_statistics_formulas(oracle_mod):
     assert s["q1"] == 2 and s["q3"] == 4
 
 
-def test_msbfs_bytes_brute_force(oracle_mod):
+@pytest.mark.parametrize("masked", [False, True])
+def test_msbfs_bytes_brute_force(oracle_mod, masked):
+    """The batched byte model against a set-based restatement; masked: with
+    per-level pulled / pushed lane sets (random, seeded), as a run with
+    per-root directions reports them."""
     O = oracle_mod
     e = O.generate_edges(8)
     c = O.Csr(e)
@@ -277,15 +281,24 @@ def test_msbfs_bytes_brute_force(oracle_mod):
     for i, r in enumerate(roots):
         lv, _, _ = c.bfs(int(r), want_pred=False)
         levels_new[i] = lv[a["new2old"]]
-    tot, per = c.msbfs_bytes(levels_new)
+    L = int(levels_new[levels_new != 0xFF].max()) + 1
+    pull = push = None
+    if masked:
+        rng = np.random.default_rng(4)
+        pull = [int(x) for x in rng.integers(0, 1 << len(roots), L + 1)]
+        push = [int(x) for x in rng.integers(0, 1 << len(roots), L + 1)]
+    tot, per = c.msbfs_bytes(levels_new, pull, push)
     # brute force in python
     na = c.num_active
-    L = int(levels_new[levels_new != 0xFF].max()) + 1
     exp = float((levels_new != 0xFF).sum())
     for l in range(1, L + 1):
         b = 24.0 * n
         for v in range(na):
-            need = [r for r in range(len(roots)) if levels_new[r, v] == 0xFF or levels_new[r, v] >= l]
+            deg = int(a["offsets"][v + 1] - a["offsets"][v])
+            if masked and any((push[l] >> r) & 1 and levels_new[r, v] == l - 1 for r in range(len(roots))):
+                b += 8 + 12.0 * deg
+            need = [r for r in range(len(roots)) if (levels_new[r, v] == 0xFF or levels_new[r, v] >= l)
+                    and (not masked or (pull[l] >> r) & 1)]
             if not need:
                 continue
             got, k = set(), 0
