@@ -425,9 +425,10 @@ __device__ __forceinline__ void ms_sweep_body(const uint32_t* __restrict__ chunk
 
 // Push step over the CSR in sweep chunks (no frontier queue, so no scan): the
 // chunks holding a frontier row are collected first from the nonzero bitmap;
-// in those, each lane takes kSweepEpt consecutive entries and, for those
-// whose row u is in the frontier, sends front[u] & ~seen[w] to the target w.
-// seen is not written.
+// in those, consecutive lanes take consecutive entries and, for those whose
+// row u is in the frontier, send front[u] to the target w.  Lanes that w has
+// already seen are not filtered here (that would cost one more random read
+// per entry): folding next into seen drops them.
 __device__ __forceinline__ bool chunk_active(uint64_t chunk, const uint32_t* __restrict__ chunk_rows,
                                              const uint64_t* __restrict__ nz_front) {
   const uint32_t r0 = chunk_rows[2 * chunk], r1 = chunk_rows[2 * chunk + 1];
@@ -444,8 +445,7 @@ __device__ __forceinline__ bool chunk_active(uint64_t chunk, const uint32_t* __r
 __device__ __forceinline__ void push_chunk(uint64_t chunk, const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
                                            const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols,
                                            const uint64_t* __restrict__ front,
-                                           const uint64_t* __restrict__ nz_front,
-                                           const uint64_t* __restrict__ seen, uint64_t* next, SweepLds& L,
+                                           const uint64_t* __restrict__ nz_front, uint64_t* next, SweepLds& L,
                                            uint64_t lanes = ~UINT64_C(0)) {
   const uint32_t r0 = chunk_rows[2 * chunk];
   const uint64_t e0 = chunk * kSweepEpb;
@@ -470,9 +470,8 @@ __device__ __forceinline__ void push_chunk(uint64_t chunk, const uint32_t* __res
     }
     if (!fu) continue;
     const uint32_t w = cols[e0 + e];
-    const uint64_t m = fu & ~seen[w];
-    if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
-      atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
+    if (fu & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
+      atomicOr((unsigned long long*)(next + w), (unsigned long long)fu);
   }
   __syncthreads();
 }
@@ -497,19 +496,18 @@ __device__ __forceinline__ void ms_collect_chunks_body(const uint32_t* __restric
   }
 }
 
-// Push over the collected chunks: front[u] & ~seen[w] to every target w of
-// a frontier row u, with a filtered 64-bit atomicOr (next zero beforehand).
+// Push over the collected chunks: front[u] to every target w of a frontier
+// row u, with a filtered 64-bit atomicOr (next zero beforehand).
 __device__ __forceinline__ void ms_push_list_body(const uint32_t* __restrict__ list, const unsigned* count,
                                                   const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
                                                   const uint64_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ cols,
                                                   const uint64_t* __restrict__ front,
-                                                  const uint64_t* __restrict__ nz_front,
-                                                  const uint64_t* __restrict__ seen, uint64_t* next, SweepLds& L,
+                                                  const uint64_t* __restrict__ nz_front, uint64_t* next, SweepLds& L,
                                                   unsigned bid, unsigned nblk, uint64_t lanes = ~UINT64_C(0)) {
   const unsigned n = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (unsigned i = bid; i < n; i += nblk)
-    push_chunk(list[i], chunk_rows, nnz, offsets, cols, front, nz_front, seen, next, L, lanes);
+    push_chunk(list[i], chunk_rows, nnz, offsets, cols, front, nz_front, next, L, lanes);
 }
 
 // One bit per vertex whose frontier word holds one of `lanes`.

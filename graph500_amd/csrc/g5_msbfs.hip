@@ -9,9 +9,9 @@
 // Each level is a push or a pull step, chosen from the edge count of the
 // frontier union (the direction-optimizing rule applied to the union):
 //   push  the union's rows form a chunked virtual edge list (as in the
-//         single-root top-down); target w receives front[u] & ~seen[w] with
-//         one 64-bit atomicOr, filtered by plain loads; k_ms_finish then
-//         masks the new lanes and folds them into seen
+//         single-root top-down); target w receives front[u] with one
+//         64-bit atomicOr, filtered by a plain load of next[w]; k_ms_finish
+//         then masks the lanes new to w and folds them into seen
 //   pull  one lane per vertex: the lanes it still lacks (and whose frontier
 //         is not empty) are collected by OR-ing the neighbours' frontier
 //         words in ascending (highest degree first) order until every needed
@@ -78,8 +78,9 @@ __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ chunk_rows,
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ cols,
-                                                    const uint64_t* __restrict__ front,
-                                                    const uint64_t* __restrict__ seen, uint64_t* next) {
+                                                    const uint64_t* __restrict__ front, uint64_t* next) {
+  // front[u] goes to every target; lanes the target has seen are dropped
+  // when k_ms_finish folds next into seen
   __shared__ ChunkRows cr;
   const uint64_t E = prefix[nf];
   for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
@@ -87,7 +88,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
     const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
     walk_strided(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
-      const uint64_t m = front[u] & ~seen[w];
+      const uint64_t m = front[u];
       if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
         atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
     });
@@ -159,12 +160,10 @@ __global__ void __launch_bounds__(kBlock) k_ms_push_list(const uint32_t* __restr
                                                          const uint64_t* __restrict__ offsets,
                                                          const uint32_t* __restrict__ cols,
                                                          const uint64_t* __restrict__ front,
-                                                         const uint64_t* __restrict__ nz_sel,
-                                                         const uint64_t* __restrict__ seen, uint64_t* next,
+                                                         const uint64_t* __restrict__ nz_sel, uint64_t* next,
                                                          uint64_t lanes) {
   __shared__ SweepLds L;
-  ms_push_list_body(list, count, chunk_rows, nnz, offsets, cols, front, nz_sel, seen, next, L, blockIdx.x, gridDim.x,
-                    lanes);
+  ms_push_list_body(list, count, chunk_rows, nnz, offsets, cols, front, nz_sel, next, L, blockIdx.x, gridDim.x, lanes);
 }
 
 __global__ void __launch_bounds__(kBlock) k_ms_sweep_clear(const uint32_t* __restrict__ chunk_rows,
@@ -308,7 +307,7 @@ k_ms_persistent(MsPersistArgs a) {
       ms_collect_chunks_body(a.sweep_rows, a.sweep_chunks, nz_front, a.chunk_list, a.chunk_count, bid, nblk);
       if (!grid_barrier(a.bar, nblk, deadline)) return;
       ms_push_list_body(a.chunk_list, a.chunk_count, a.sweep_rows, a.nnz, a.offsets, a.cols, front, nz_front,
-                        a.seen, next, L, bid, nblk);
+                        next, L, bid, nblk);
       if (!grid_barrier(a.bar, nblk, deadline)) return;
       ms_finish_body(a.seen, next, nz_next, a.offsets, a.rows, c, bid, nblk);
     } else if (sweep) {
@@ -395,7 +394,7 @@ struct PopCount {
 // level l: 24 n + sum over vertices with a needed pulled lane of (8 + 12 k),
 // k = probes until every needed lane is found (the row length if not), +
 // sum over vertices whose frontier word holds a pushed lane of (8 + 12 deg):
-// the word, then per entry the column and the target's seen word.  With one
+// the word, then per entry the column and the target's next word.  With one
 // direction for all roots every level is costed as a pull of all lanes.
 __global__ void __launch_bounds__(kBlock) k_ms_model(const uint64_t* const* __restrict__ fr, int l,
                                                      uint64_t lanes, uint64_t push_lanes,
@@ -537,7 +536,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
                          m.sweep_rows.p, m.sweep_chunks, m.nz_sel.p, m.chunk_list.p, d_list_count);
       hipLaunchKernelGGL(k_ms_push_list, dim3(grid_of((int64_t)m.sweep_chunks, 1, 4096)), dim3(kBlock), 0, s,
                          m.chunk_list.p, (const unsigned*)d_list_count, m.sweep_rows.p, (uint64_t)g.nnz, g.offsets.p,
-                         g.cols.p, front, m.nz_sel.p, m.seen.p, next, td);
+                         g.cols.p, front, m.nz_sel.p, next, td);
       G5_HIP(hipGetLastError());
       hipLaunchKernelGGL(k_ms_pull_lanes, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
                          m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, bu,
@@ -554,7 +553,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
                            dim3(kBlock), 0, s, m.prefix.p, nf, nchunks, m.chunk_rows.p);
       G5_HIP(hipMemsetAsync(next, 0, rows * 8, s));
       hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)fdeg, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p,
-                         nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, m.seen.p, next);
+                         nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, next);
       G5_HIP(hipGetLastError());
       if (lane_stats)
         hipLaunchKernelGGL(k_ms_finish_lanes, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,

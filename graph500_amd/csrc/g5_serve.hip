@@ -145,12 +145,11 @@ __global__ void __launch_bounds__(kBlock) k_sv_push(const uint32_t* __restrict__
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ cols,
                                                     const uint64_t* __restrict__ front,
-                                                    const uint64_t* __restrict__ nz_front,
-                                                    const uint64_t* __restrict__ seen, uint64_t* next,
+                                                    const uint64_t* __restrict__ nz_front, uint64_t* next,
                                                     uint64_t lanes) {
   __shared__ SweepLds L;
-  ms_push_list_body(list, count, chunk_rows, nnz, offsets, cols, front, nz_front, seen, next, L, blockIdx.x,
-                    gridDim.x, lanes);
+  ms_push_list_body(list, count, chunk_rows, nnz, offsets, cols, front, nz_front, next, L, blockIdx.x, gridDim.x,
+                    lanes);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sv_select(const uint64_t* __restrict__ front, int64_t rows, uint64_t lanes,
@@ -557,7 +556,7 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
                        e.chunk_rows.p, e.nchunks, nz_push, e.chunk_list.p, (unsigned*)(c + M_NUM));
     hipLaunchKernelGGL(k_sv_push, dim3(grid_of((int64_t)e.nchunks, 1, 4096)), dim3(kBlock), 0, s, e.chunk_list.p,
                        (const unsigned*)(c + M_NUM), e.chunk_rows.p, (uint64_t)e.nnz, g.offsets.p, g.cols.p, front,
-                       nz_front, e.seen.p, next, td);
+                       nz_front, next, td);
   }
   if (bu) {
     hipLaunchKernelGGL(k_sv_pull, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, e.seen.p, front,
