@@ -27,7 +27,7 @@ DEFAULT_CACHE = os.environ.get("G5_TUNING_CACHE",
 # alpha means a different threshold under each rule.
 SPACE = [
     ("lane_rule,alpha", [(0, 1.0), (0, 2.0), (0, 4.0), (0, 8.0), (0, 16.0), (0, 64.0),
-                         (1, 32.0), (1, 64.0), (1, 128.0), (1, 256.0)]),
+                         (1, 32.0), (1, 64.0), (1, 96.0), (1, 128.0), (1, 192.0), (1, 256.0)]),
     ("persistent", [0, 1]),
     ("sweep_below", [0.0, 0.25]),
     ("beta", [0.0, 20.0, 50.0, 200.0]),

@@ -45,6 +45,7 @@ struct ServeEpi {
   const uint64_t* offsets;
   uint64_t* nz_sel;
   uint64_t sel;
+  uint64_t stat_lanes;  // lanes whose counts and degree sums the host still needs
 };
 
 // Thread i of a wave keeps lane i's answer pointer and level in registers
@@ -77,13 +78,14 @@ struct ServeLocal {
       const uint32_t lv = __builtin_amdgcn_readlane(my_lvl, r);
       if (inb && ((nw >> r) & 1)) ((uint8_t*)(((uint64_t)ahi << 32) | alo))[v] = (uint8_t)lv;
     }
-    const uint64_t deg = (inb && nw) ? p.offsets[v + 1] - p.offsets[v] : 0;
-    uint64_t m = __ballot(inb && nw != 0);
+    const uint64_t want = inb ? nw & p.stat_lanes : 0;
+    const uint64_t deg = want ? p.offsets[v + 1] - p.offsets[v] : 0;
+    uint64_t m = __ballot(want != 0);
     while (m) {
       const int t = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;
-      const uint32_t wlo = __builtin_amdgcn_readlane((uint32_t)nw, t);
-      const uint32_t whi = __builtin_amdgcn_readlane((uint32_t)(nw >> 32), t);
+      const uint32_t wlo = __builtin_amdgcn_readlane((uint32_t)want, t);
+      const uint32_t whi = __builtin_amdgcn_readlane((uint32_t)(want >> 32), t);
       const uint32_t dlo = __builtin_amdgcn_readlane((uint32_t)deg, t);
       const uint32_t dhi = __builtin_amdgcn_readlane((uint32_t)(deg >> 32), t);
       const uint32_t bit = me < 32 ? (wlo >> me) & 1 : (whi >> (me - 32)) & 1;
@@ -283,6 +285,7 @@ struct Engine {
   int lane_l0[64];
   // per-lane direction state, the reference rule per search (mpi/bfs.hpp:2799-2829)
   bool lane_fwd[64];
+  uint64_t pulled = 0;  // lane rule 3: searches that have pulled keep pulling
   bool lane_growing[64];
   int64_t lane_prev_nq[64];
   int64_t lane_visited[64];
@@ -401,8 +404,11 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
     e.lane_vdeg[r] += fdeg;
     const int64_t unvisited = (int64_t)na - e.lane_visited[r];
     if (srv->cfg.lane_rule == 3) {
-      // pull while the frontier's edges exceed 1/alpha of the edges not yet reached
-      e.lane_fwd[r] = fdeg * srv->cfg.alpha <= (double)e.nnz - e.lane_vdeg[r];
+      // pull while the frontier's edges exceed 1/alpha of the edges not yet
+      // reached; a search that pulls keeps pulling (its counts are no longer
+      // gathered)
+      if (!((e.pulled >> r) & 1)) e.lane_fwd[r] = fdeg * srv->cfg.alpha <= (double)e.nnz - e.lane_vdeg[r];
+      if (!e.lane_fwd[r]) e.pulled |= UINT64_C(1) << r;
     } else if (srv->cfg.lane_rule == 2) {
       // one direction for all searches, from the frontier union's edges
       e.lane_fwd[r] = (double)e.last_fdeg < (double)e.nnz / srv->cfg.alpha;
@@ -479,6 +485,7 @@ static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
     e.lane_req[r] = id;
     e.lane_l0[r] = e.level;
     e.lane_fwd[r] = true;
+    e.pulled &= ~(UINT64_C(1) << r);
     e.lane_growing[r] = true;
     e.lane_prev_nq[r] = 0;
     e.lane_visited[r] = 1;
@@ -536,6 +543,7 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
   epi.offsets = g.offsets.p;
   epi.nz_sel = e.nzs[cur ^ 1].p;
   epi.sel = td;
+  epi.stat_lanes = srv->cfg.lane_rule == 3 ? ~e.pulled : ~UINT64_C(0);
   // chunks to push: those holding a top-down lane's frontier vertex; the
   // bitmap from the last level's epilogue (and admissions) serves when it
   // was made for exactly these lanes, else it is made from the frontier
