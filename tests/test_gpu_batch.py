@@ -161,7 +161,11 @@ def test_batch_model_bytes_per_root_rule_match_oracle(s16ctx, oracle_mod, alpha)
         s16ctx.batch_set_tuning(lane_rule=0, alpha=16.0)
 
 
-def test_batch_partial_and_odd_roots(oracle_mod):
+@pytest.mark.parametrize("mode,rule", [(2, 1), (2, 0), (1, 1), (0, 1)])
+def test_batch_partial_and_odd_roots(oracle_mod, mode, rule):
+    """Duplicate and isolated roots, partial batches, under each reorder mode
+    (mode 0 keeps rows without edges; the per-root rule then falls back to
+    one direction per level)."""
     import graph500_amd
     O = oracle_mod
     e = O.generate_edges(12)
@@ -170,10 +174,11 @@ def test_batch_partial_and_odd_roots(oracle_mod):
     roots = list(oc.find_roots(5)) + iso + [int(oc.find_roots(1)[0])]  # duplicate root, isolated roots
     with graph500_amd.G5Context(0) as c:
         c.edges_append(e)
-        c.construct(2)
+        c.construct(mode)
         c.batch_prepare()
+        c.batch_set_tuning(lane_rule=rule, alpha=4.0 if rule else 16.0)
         for rs in (roots, roots[:1]):
-            say("batch odd roots", rs)
+            say("batch odd roots mode", mode, "rule", rule, rs)
             c.run_bfs_batch(rs)
             teps = c.batch_teps_edges()
             for i, r in enumerate(rs):
