@@ -1,0 +1,80 @@
+"""CPU checks of bench.py's contract and of its rocprof helpers: the
+multi-GPU refusal, the traffic figure read from the committed profile, and
+the summaries of a rocprofv3 results database (a small stand-in database with
+the tables and columns the helpers read)."""
+import os
+import sqlite3
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one MI355X" in r.stderr
+
+
+def test_traffic_from_committed_profile():
+    t = bench.traffic_from_profile("k_ms_mixed", 28)
+    assert t is not None and t > 1e10
+    text = open(os.path.join(ROOT, "profiles", "k_ms_mixed_s28_traffic.txt")).read()
+    assert "traffic_bytes_per_launch: %.0f" % t in text
+    assert bench.traffic_from_profile("no_such_kernel", 28) is None
+
+
+def _trace_db(path):
+    con = sqlite3.connect(path)
+    con.execute("create table kernels (name text, start integer, end integer)")
+    con.execute("create table top_kernels (name text, total_calls integer, total_duration real, average real, "
+                "percentage real)")
+    rows = [("g5::k_ms_pull_lanes(unsigned long*)", 0, 26_000_000), ("g5::k_ms_push_list(unsigned int const*)",
+            27_000_000, 29_000_000), ("g5::k_ms_pull_lanes(unsigned long*)", 30_000_000, 31_500_000)]
+    con.executemany("insert into kernels values (?, ?, ?)", rows)
+    con.execute("insert into top_kernels values ('g5::k_ms_pull_lanes(unsigned long*)', 2, 27500.0, 13750.0, "
+                "93.2)")
+    con.commit()
+    con.close()
+
+
+def _pmc_db(path):
+    con = sqlite3.connect(path)
+    con.execute("create table counters_collection (dispatch_id integer, kernel_name text, counter_name text, "
+                "value real)")
+    con.executemany("insert into counters_collection values (?, ?, ?, ?)",
+                    [(2, "g5::k_ms_pull_lanes(unsigned long*)", "FETCH_SIZE", 300.0),
+                     (1, "g5::k_ms_pull_lanes(unsigned long*)", "FETCH_SIZE", 100.0),
+                     (3, "g5::k_ms_select(unsigned long const*)", "FETCH_SIZE", 7.0)])
+    con.commit()
+    con.close()
+
+
+def test_rocprof_dispatches_in_order(tmp_path):
+    db = str(tmp_path / "trace.db")
+    _trace_db(db)
+    lines = bench.rocprof_dispatches(db, "k_ms_pull_lanes").splitlines()
+    assert [float(l.split()[-1]) for l in lines] == [26.0, 1.5]
+    db2 = str(tmp_path / "pmc.db")
+    _pmc_db(db2)
+    vals = [float(l.split()[-1]) for l in bench.rocprof_dispatches(db2, "k_ms_pull_lanes").splitlines()]
+    assert vals == [100.0, 300.0]  # dispatch order, not insertion order
+
+
+def test_rocprof_summary_both_kinds(tmp_path):
+    db = str(tmp_path / "trace.db")
+    _trace_db(db)
+    s = bench.rocprof_summary(db)
+    assert "g5::k_ms_pull_lanes" in s and "27.500" in s  # total ms from the view's microseconds
+    db2 = str(tmp_path / "pmc.db")
+    _pmc_db(db2)
+    s2 = bench.rocprof_summary(db2)
+    line = [l for l in s2.splitlines() if "k_ms_pull_lanes" in l][0]
+    assert line.split()[-2] == "2" and float(line.split()[-1]) == pytest.approx(200.0)
