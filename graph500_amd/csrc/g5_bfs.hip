@@ -115,7 +115,23 @@ __global__ void __launch_bounds__(kBlock) k_td_expand(
         s_q[atomicAdd(&s_qn, 1u)] = w;
       }
     };
-    walk_strided(e0, e1, cnt, cr, cols, visit);
+    if (kBulk) {
+      // every probe of the lane's edges in flight before any claim
+      uint32_t w[kEpt], u[kEpt];
+      bool ok[kEpt];
+      gather_strided(e0, e1, cnt, cr, cols, w, u, ok);
+      uint8_t lv[kEpt];
+#pragma unroll
+      for (int j = 0; j < kEpt; ++j) lv[j] = ok[j] ? level[w[j]] : 0;
+#pragma unroll
+      for (int j = 0; j < kEpt; ++j)
+        if (ok[j] && lv[j] == 0xFF) {
+          level[w[j]] = (uint8_t)lvl;
+          parent[w[j]] = u[j];
+        }
+    } else {
+      walk_strided(e0, e1, cnt, cr, cols, visit);
+    }
     if (!kBulk) {
       __syncthreads();
       if (threadIdx.x == 0) s_qbase = s_qn ? atomicAdd(counters + C_FOUND, (unsigned long long)s_qn) : 0;

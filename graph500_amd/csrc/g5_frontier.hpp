@@ -141,6 +141,34 @@ __device__ __forceinline__ void walk_strided(uint64_t e0, uint64_t e1, int cnt, 
   }
 }
 
+// The same edges as walk_strided, gathered first: a lane's (up to kEpt)
+// targets and sources land in registers before any of them is used, so the
+// column loads, and then the caller's probes, are in flight together.
+__device__ __forceinline__ void gather_strided(uint64_t e0, uint64_t e1, int cnt, const ChunkRows& cr,
+                                               const uint32_t* __restrict__ cols, uint32_t (&w)[kEpt],
+                                               uint32_t (&u)[kEpt], bool (&ok)[kEpt]) {
+  int lo = 0;
+#pragma unroll
+  for (int j = 0; j < kEpt; ++j) {
+    const uint64_t e = e0 + threadIdx.x + (uint64_t)j * kBlock;
+    ok[j] = e < e1;
+    w[j] = 0;
+    u[j] = 0;
+    if (ok[j]) {
+      if (lo + 1 < cnt && cr.pre[lo + 1] <= e) {
+        int a = lo + 1, b = cnt - 1;
+        while (a < b) {
+          const int mid = (a + b + 1) >> 1;
+          if (cr.pre[mid] <= e) a = mid; else b = mid - 1;
+        }
+        lo = a;
+      }
+      w[j] = cols[cr.base[lo] + e];
+      u[j] = cr.src[lo];
+    }
+  }
+}
+
 // Tuple weight of a reached vertex (0xFF = unreached); the traversed-edge
 // count of a run is the sum over all rows, halved.
 struct ReachedWeight {
