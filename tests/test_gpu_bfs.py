@@ -297,3 +297,40 @@ def test_driver_g5bench_s16(extra):
     assert int(kv["NBFS"]) == 16
     assert float(kv["min_nedge"]) == float(kv["max_nedge"]) == 1048570
     assert float(kv["harmonic_mean_TEPS"]) > 0
+
+
+def test_driver_logfile_resume_and_tmpfile(tmp_path):
+    """LOGFILE: a run keeps its per-root results in the reference's binary
+    record and a second run resumes after them (read_log_file /
+    update_log_file, mpi/benchmark_helper.hpp:530-604); a record of another
+    SCALE is refused.  TMPFILE with --host-edges: the edge list goes through
+    a file, which is removed at the end."""
+    import struct
+    import subprocess
+    from graph500_amd import build
+    log = tmp_path / "g5.log"
+    env = dict(os.environ, LOGFILE=str(log), TMPFILE=str(tmp_path / "edges"))
+    say("g5bench 16 LOGFILE, 4 roots")
+    r = subprocess.run([build.DRIVER, "16", "--roots", "4", "--host-edges"], capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert not os.path.exists(str(tmp_path / "edges") + "-000")
+    b = log.read_bytes()
+    assert len(b) == 1576
+    scale, ef, size, runs = struct.unpack_from("<iiii", b, 0)
+    assert (scale, ef, size, runs) == (16, 16, 1, 4)
+    first = [struct.unpack_from("<ddq", b, 40 + 24 * i) for i in range(4)]
+    assert all(t[2] == 1048570 and t[0] > 0 for t in first)
+    say("g5bench 16 resume to 8 roots")
+    r = subprocess.run([build.DRIVER, "16", "--roots", "8"], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert "Resume from 4 th run" in r.stdout
+    assert "Running BFS 3 ==" not in r.stdout and "========== Running BFS 4 ==========" in r.stdout
+    b2 = log.read_bytes()
+    assert struct.unpack_from("<i", b2, 12)[0] == 8
+    assert [struct.unpack_from("<ddq", b2, 40 + 24 * i) for i in range(4)] == first
+    kv = dict(l.split(":", 1) for l in r.stdout.splitlines() if ":" in l and not l.startswith("="))
+    assert int(kv["NBFS"]) == 8 and float(kv["max_nedge"]) == 1048570
+    say("g5bench 14 with the SCALE 16 record")
+    r = subprocess.run([build.DRIVER, "14", "--roots", "2"], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 1 and "Log file is not match" in r.stdout
