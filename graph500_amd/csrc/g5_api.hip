@@ -147,13 +147,15 @@ int g5_edges_release(g5_ctx* ctx) {
   });
 }
 
-int g5_generate_kronecker(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1, uint64_t seed2,
-                          int a_num, int bc_num) {
+int g5_generate_graph(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1, uint64_t seed2, int a_num,
+                      int bc_num, int initial_edges) {
   return guarded(ctx, [&] {
     G5_CHECK(edgefactor >= 1, "g5_generate_kronecker: edgefactor must be >= 1");
     G5_CHECK(scale >= 1 && scale <= 40, "g5_generate_kronecker: scale must be in [1, 40]");
+    G5_CHECK(initial_edges == 0 || initial_edges == 1, "g5_generate_graph: initial_edges must be 0 or 1");
     set_device(ctx);
-    const int64_t m = ((int64_t)1 << scale) * edgefactor;
+    const int64_t n_init = initial_edges == 1 ? ((int64_t)1 << scale) - 1 : 0;
+    const int64_t m = ((int64_t)1 << scale) * edgefactor + n_init;
     ctx->num_edges = 0;
     if ((size_t)m > ctx->edges.n) {
       ctx->edges.release();
@@ -163,7 +165,10 @@ int g5_generate_kronecker(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1
     G5_HIP(hipEventCreate(&t0));
     G5_HIP(hipEventCreate(&t1));
     G5_HIP(hipEventRecord(t0, ctx->stream));
-    launch_kronecker(ctx->device, ctx->edges.p, 0, m, scale, seed1, seed2, a_num, bc_num, ctx->stream);
+    // the initial tuples come first; the R-MAT tuples keep their global indices
+    launch_initial_tree(ctx->edges.p, n_init, scale, seed1, seed2, ctx->stream);
+    launch_kronecker(ctx->device, ctx->edges.p + n_init, n_init, m - n_init, scale, seed1, seed2, a_num, bc_num,
+                     ctx->stream);
     G5_HIP(hipEventRecord(t1, ctx->stream));
     G5_HIP(hipEventSynchronize(t1));
     float ms = 0;
@@ -173,6 +178,11 @@ int g5_generate_kronecker(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1
     ctx->generate_ms = ms;
     ctx->num_edges = m;
   });
+}
+
+int g5_generate_kronecker(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1, uint64_t seed2,
+                          int a_num, int bc_num) {
+  return g5_generate_graph(ctx, scale, edgefactor, seed1, seed2, a_num, bc_num, 0);
 }
 
 int g5_construct(g5_ctx* ctx, int reorder_mode, int release_edges) {

@@ -213,6 +213,8 @@ namespace g5 {
 // g5_csr.hip
 void build_graph(g5_ctx* ctx, int reorder_mode, bool release_edges);
 // g5_gen.hip
+void launch_initial_tree(g5_packed_edge* out, int64_t count, int scale, uint64_t s1, uint64_t s2,
+                         hipStream_t stream);
 void launch_kronecker(int device, g5_packed_edge* out, int64_t start, int64_t count, int scale,
                       uint64_t s1, uint64_t s2, int a_num, int bc_num, hipStream_t stream);
 // g5_bfs.hip

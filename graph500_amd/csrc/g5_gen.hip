@@ -259,6 +259,34 @@ __global__ void __launch_bounds__(256) k_kronecker(g5_packed_edge* __restrict__ 
   }
 }
 
+// Binary-tree initial tuples (GraphGeneratorBase::generateInitialEdge,
+// generator/graph_generator.hpp:515-524): tuple i joins vertex i+1 to its
+// heap parent (i+1)/2, both scrambled.
+__global__ void __launch_bounds__(256) k_initial_tree(g5_packed_edge* __restrict__ out, int64_t count, int scale,
+                                                      uint64_t val0, uint64_t val1) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const uint64_t v0 = dscramble((uint64_t)(i + 1), scale, val0, val1);
+    const uint64_t v1 = dscramble((uint64_t)((i + 1) / 2), scale, val0, val1);
+    g5_packed_edge r;
+    r.v0_low = (uint32_t)v0;
+    r.v1_low = (uint32_t)v1;
+    r.high = (uint32_t)(((v0 >> 32) & 0xFFFF) | ((v1 >> 16) & 0xFFFF0000u));
+    out[i] = r;
+  }
+}
+
+void launch_initial_tree(g5_packed_edge* out, int64_t count, int scale, uint64_t s1, uint64_t s2,
+                         hipStream_t stream) {
+  if (count <= 0) return;
+  uint64_t val0, val1;
+  host_scramble_values(s1, s2, &val0, &val1);
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_initial_tree, dim3((unsigned)blocks), dim3(256), 0, stream, out, count, scale, val0, val1);
+  G5_HIP(hipGetLastError());
+}
+
 // Owns the device copy of the middle-word byte tables.
 struct GenTables {
   DevBuf<DevEntry> tab;

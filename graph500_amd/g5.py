@@ -33,7 +33,7 @@ ERRORS = {-1: "G5_ERR_ARG", -2: "G5_ERR_STATE", -3: "G5_ERR_HIP", -4: "G5_ERR_ME
 EXPORTS = [
     "g5_version", "g5_device_count", "g5_create", "g5_destroy", "g5_last_error",
     "g5_synchronize", "g5_edges_reserve", "g5_edges_append", "g5_edges_count",
-    "g5_edges_download", "g5_edges_release", "g5_generate_kronecker", "g5_construct",
+    "g5_edges_download", "g5_edges_release", "g5_generate_kronecker", "g5_generate_graph", "g5_construct",
     "g5_graph_info_get", "g5_has_edge", "g5_pred_size", "g5_find_roots", "g5_graph_download",
     "g5_prepare_bfs", "g5_run_bfs", "g5_get_pred", "g5_get_levels", "g5_teps_edges",
     "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_bfs_set_tuning", "g5_bfs_get_tuning", "g5_validate", "g5_validate_pred", "g5_end_bfs",
@@ -160,6 +160,7 @@ def load_library(path=LIB_PATH):
         "g5_edges_download": ([vp, i64, i64, vp], i32),
         "g5_edges_release": ([vp], i32),
         "g5_generate_kronecker": ([vp, i32, i32, u64, u64, i32, i32], i32),
+        "g5_generate_graph": ([vp, i32, i32, u64, u64, i32, i32, i32], i32),
         "g5_construct": ([vp, i32, i32], i32),
         "g5_graph_info_get": ([vp, ctypes.POINTER(GraphInfo)], i32),
         "g5_has_edge": ([vp, i64], i32),
@@ -246,6 +247,11 @@ class G5Context:
     def generate_kronecker(self, scale, edgefactor=16, seed1=2, seed2=3, a_num=5700, bc_num=1900):
         self._check(self._lib.g5_generate_kronecker(self._h, scale, edgefactor, seed1, seed2,
                                                     a_num, bc_num))
+
+    def generate_graph_spec2012(self, scale, edgefactor=16, seed1=2, seed2=3, a_num=5500, bc_num=100):
+        """generate_graph_spec2012 without weights: binary-tree tuples first,
+        then R-MAT tuples (initiator 5500/100) at their global indices."""
+        self._check(self._lib.g5_generate_graph(self._h, scale, edgefactor, seed1, seed2, a_num, bc_num, 1))
 
     def edges_append(self, tuples):
         t = np.ascontiguousarray(tuples, EDGE_DTYPE)

@@ -148,6 +148,14 @@ int g5_edges_release(g5_ctx* ctx);
  * (2,3) and a_num=5700, bc_num=1900 (per 10000). */
 int g5_generate_kronecker(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1,
                           uint64_t seed2, int a_num, int bc_num);
+/* generate_graph with an initial-edge type (GraphGeneratorBase, generator/
+ * graph_generator.hpp:385-393, 515-524): initial_edges 1 puts 2^scale - 1
+ * binary-tree tuples first and the R-MAT tuples after them at their global
+ * indices, as generate_graph_spec2012 does (mpi/benchmark_helper.hpp:316-322,
+ * a_num 5500, bc_num 100; the reference's weights are not generated);
+ * initial_edges 0 is g5_generate_kronecker. */
+int g5_generate_graph(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1, uint64_t seed2,
+                      int a_num, int bc_num, int initial_edges);
 
 /* ---- graph: BfsBase::construct -> GraphConstructor2DCSR::construct
  *      (mpi/bfs.hpp:102-113, mpi/graph_constructor.hpp:554-574) ---- */

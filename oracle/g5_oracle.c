@@ -203,6 +203,17 @@ static void set_edge(or_packed_edge* e, int64_t v0, int64_t v1) {
   e->high = (uint32_t)(((v0 >> 32) & 0xFFFF) | ((v1 >> 16) & 0xFFFF0000u));
 }
 
+/* GraphGeneratorBase::generateInitialEdge, BINARY_TREE
+ * (generator/graph_generator.hpp:515-524): tuple i joins vertex i+1 to its
+ * parent (i+1)/2 in the heap order, both scrambled; 2^scale - 1 of them come
+ * before the R-MAT tuples (:385-393, :612-619). */
+void or_initial_tree_edges(int scale, uint64_t s1, uint64_t s2, int64_t start, int64_t end, or_packed_edge* out) {
+  uint64_t sv[2];
+  or_scramble_values(s1, s2, sv);
+  for (int64_t i = start; i < end; ++i)
+    set_edge(&out[i - start], or_scramble(i + 1, scale, sv[0], sv[1]), or_scramble((i + 1) / 2, scale, sv[0], sv[1]));
+}
+
 void or_generate_edges(int scale, int edgefactor, uint64_t s1, uint64_t s2, int a_num, int bc_num,
                        int64_t start, int64_t end, or_packed_edge* out) {
   (void)edgefactor;
@@ -626,7 +637,7 @@ int or_validate(const or_packed_edge* e, int64_t m, int64_t nglobalverts, int64_
       FAIL("BFS predecessors do not form a tree; see vertices %lld and %lld", (long long)i, (long long)p);
   }
   /* edge checks and TEPS count, validate.hpp:588-792 */
-  valid = (unsigned char*)calloc((size_t)nlocalverts, 1);
+  valid = (unsigned char*)calloc(nlocalverts > 0 ? (size_t)nlocalverts : 1, 1);
   int64_t count = 0;
   for (int64_t i = 0; i < m; ++i) {
     int64_t s = or_edge_v0(&e[i]), t = or_edge_v1(&e[i]);

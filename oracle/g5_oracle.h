@@ -69,6 +69,13 @@ int64_t or_scramble(int64_t v, int scale, uint64_t val0, uint64_t val1);
 void or_generate_edges(int scale, int edgefactor, uint64_t userseed1, uint64_t userseed2,
                        int a_num, int bc_num, int64_t start, int64_t end, or_packed_edge* out);
 
+/* Binary-tree initial tuples [start, end) of the spec-2012 generator
+ * (generateInitialEdge, graph_generator.hpp:515-524); the R-MAT tuples then
+ * take indices 2^scale - 1 onward (or_generate_edges with that start).
+ * Pinned: the reference spec-2012 tuple stream (golden, SCALE 14 and 16). */
+void or_initial_tree_edges(int scale, uint64_t userseed1, uint64_t userseed2, int64_t start, int64_t end,
+                           or_packed_edge* out);
+
 static inline int64_t or_edge_v0(const or_packed_edge* e) {
   return (int64_t)(e->v0_low | ((uint64_t)(e->high & 0xFFFF) << 32));
 }

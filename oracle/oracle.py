@@ -91,6 +91,8 @@ def lib():
                                 ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     L.or_bfs.argtypes = [ctypes.POINTER(OrCsr), ctypes.c_int64, ctypes.c_void_p,
                          ctypes.c_void_p, ctypes.POINTER(OrBfsTrace)]
+    L.or_initial_tree_edges.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64,
+                                        ctypes.c_int64, ctypes.c_void_p]
     L.or_msbfs_bytes.argtypes = [ctypes.POINTER(OrCsr), ctypes.c_int, ctypes.c_void_p,
                                  ctypes.POINTER(ctypes.c_int), ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p]
@@ -152,6 +154,17 @@ def generate_edges(scale, edgefactor=16, s1=2, s2=3, a_num=5700, bc_num=1900, st
     lib().or_generate_edges(scale, edgefactor, s1, s2, a_num, bc_num, start, end,
                             out.ctypes.data_as(ctypes.c_void_p))
     return out
+
+
+def generate_graph_spec2012(scale, edgefactor=16, s1=2, s2=3, a_num=5500, bc_num=100):
+    """generate_graph_spec2012 (mpi/benchmark_helper.hpp:316-322) without
+    weights: 2^scale - 1 binary-tree tuples, then edgefactor * 2^scale R-MAT
+    tuples at indices 2^scale - 1 onward."""
+    n_init = (1 << scale) - 1
+    init = np.zeros(n_init, EDGE_DTYPE)
+    lib().or_initial_tree_edges(scale, s1, s2, 0, n_init, init.ctypes.data_as(ctypes.c_void_p))
+    rmat = generate_edges(scale, edgefactor, s1, s2, a_num, bc_num, n_init, n_init + (edgefactor << scale))
+    return np.concatenate([init, rmat])
 
 
 def edge_endpoints(edges):

@@ -12,7 +12,9 @@
 //   nedge.txt       validator traversed-edge count per root
 // Per-level directions come from the reference's verbose output on stderr.
 //
-// Usage: refprobe SCALE EDGEFACTOR OUTDIR
+// Usage: refprobe SCALE EDGEFACTOR OUTDIR [spec2012]
+// spec2012: generate_graph_spec2012 (initiator 5500/100, binary-tree initial
+// edges; the unweighted tuple type, so no weights) instead of spec2010.
 #include <mpi.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -45,17 +47,19 @@ static FILE* open_out(const std::string& dir, const char* name, const char* mode
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    fprintf(stderr, "usage: %s SCALE EDGEFACTOR OUTDIR\n", argv[0]);
+    fprintf(stderr, "usage: %s SCALE EDGEFACTOR OUTDIR [spec2012]\n", argv[0]);
     return 1;
   }
   const int scale = atoi(argv[1]);
   const int ef = atoi(argv[2]);
   const std::string out = argv[3];
+  const bool spec2012 = argc > 4 && !strcmp(argv[4], "spec2012");
   setup_globals(argc, argv, scale, ef);
 
   typedef EdgeListStorage<UnweightedPackedEdge, 8 * 1024 * 1024> EdgeList;
   EdgeList edges((int64_t(1) << scale) * ef / mpi.size_2d, NULL);
-  generate_graph_spec2010(&edges, scale, ef);
+  if (spec2012) generate_graph_spec2012(&edges, scale, ef, 255);
+  else generate_graph_spec2010(&edges, scale, ef);
 
   {
     FILE* f = open_out(out, "tuples.bin", "wb");

@@ -27,12 +27,13 @@ REFBIN = os.path.join(REPO, "oracle", "_ref")
 MPIRUN = "/opt/conda/bin/mpirun"
 
 
-def run_probe(scale, roots64, threads):
+def run_probe(scale, roots64, threads, spec2012=False):
     exe = os.path.join(REFBIN, "refprobe64" if roots64 else "refprobe")
     d = tempfile.mkdtemp(prefix="g5gold%d_" % scale, dir="/tmp")
     with open(os.path.join(d, "stderr.txt"), "w") as err:
         subprocess.check_call([MPIRUN, "-np", "1", "-genv", "OMP_NUM_THREADS", str(threads), exe,
-                               str(scale), "16", d], stdout=subprocess.DEVNULL, stderr=err)
+                               str(scale), "16", d] + (["spec2012"] if spec2012 else []),
+                              stdout=subprocess.DEVNULL, stderr=err)
     return d
 
 
@@ -90,6 +91,8 @@ def main():
     ap.add_argument("--ref", default="/root/reference")
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("scales", nargs="*", type=int, default=[16, 20])
+    ap.add_argument("--spec2012", type=int, nargs="*", default=[],
+                    help="SCALEs for spec-2012 fixtures (initiator 5500/100, binary-tree initial edges)")
     a = ap.parse_args()
     if not os.path.exists(os.path.join(REFBIN, "refprobe")):
         sys.exit("build the reference probe first: make -C oracle ref")
@@ -105,6 +108,14 @@ def main():
             json.dump(res, open(os.path.join(HERE, name), "w"), indent=1)
             subprocess.call(["rm", "-rf", d])
             print("wrote", name)
+    for s in a.spec2012:
+        d = run_probe(s, False, a.threads, spec2012=True)
+        res = summarize(s, d, 16)
+        res["generator"] = "spec2012: initiator 5500/100 per 10000, 2^SCALE-1 binary-tree initial edges first"
+        name = "ref_spec2012_s%d_16roots.json" % s
+        json.dump(res, open(os.path.join(HERE, name), "w"), indent=1)
+        subprocess.call(["rm", "-rf", d])
+        print("wrote", name)
 
 
 if __name__ == "__main__":

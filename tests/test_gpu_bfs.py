@@ -334,3 +334,37 @@ def test_driver_logfile_resume_and_tmpfile(tmp_path):
     say("g5bench 14 with the SCALE 16 record")
     r = subprocess.run([build.DRIVER, "14", "--roots", "2"], capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 1 and "Log file is not match" in r.stdout
+
+
+@pytest.mark.parametrize("scale", [14, 16])
+def test_spec2012_graph_on_gpu(scale, oracle_mod):
+    """g5_generate_graph with binary-tree initial tuples (the spec-2012
+    generator): the tuple stream, CSR counts and roots equal the reference
+    run's, and single-root and batched levels equal the oracle's."""
+    import graph500_amd
+    g = gold("ref_spec2012_s%d_16roots.json" % scale)
+    with graph500_amd.G5Context(0) as c:
+        c.generate_graph_spec2012(scale)
+        t = c.edges_download().view(np.uint32).reshape(-1, 3)
+        assert sha(t) == g["tuples_sha256"]
+        c.construct(2)
+        info = c.graph_info()
+        assert info["num_active"] == g["num_active"] and info["nnz"] == g["nnz"]
+        roots = c.find_roots(16)
+        assert roots.tolist() == g["roots"]
+        oc = oracle_mod.Csr(oracle_mod.generate_graph_spec2012(scale))
+        c.prepare_bfs()
+        for pr in g["per_root"][:8]:
+            c.run_bfs(pr["root"])
+            lv_o, _, tr = oc.bfs(pr["root"], want_pred=False)
+            assert np.array_equal(c.get_levels(), lv_o), pr["root"]
+            st = c.bfs_stats()
+            assert "".join("T" if d else "B" for d in st["direction"]) == pr["directions"]
+            assert c.teps_edges() == pr["nedge"] == len(t)
+        c.batch_prepare()
+        c.run_bfs_batch(roots)
+        teps = c.batch_teps_edges()
+        for i, r in enumerate(roots[:6]):
+            lv_o, _, _ = oc.bfs(int(r), want_pred=False)
+            assert np.array_equal(c.batch_levels(i), lv_o), r
+            assert teps[i] == len(t)

@@ -311,3 +311,28 @@ def test_msbfs_bytes_brute_force(oracle_mod, masked):
             b += 8 + 12.0 * k
         exp += b
     assert tot == pytest.approx(exp)
+
+
+@pytest.mark.parametrize("scale", [14, 16])
+def test_spec2012_graph_matches_reference(oracle_mod, scale):
+    """generate_graph_spec2012 (initiator 5500/100, binary-tree initial
+    tuples first, mpi/benchmark_helper.hpp:316-322): the tuple stream, the
+    CSR counts, the roots and every root's levels, schedule and traversed
+    edges equal the reference run's."""
+    O = oracle_mod
+    g = gold("ref_spec2012_s%d_16roots.json" % scale)
+    e = O.generate_graph_spec2012(scale)
+    t = e.view(np.uint32).reshape(-1, 3)
+    assert len(t) == g["num_tuples"] == (16 << scale) + (1 << scale) - 1
+    assert hashlib.sha256(t.tobytes()).hexdigest() == g["tuples_sha256"]
+    c = O.Csr(e)
+    assert c.num_active == g["num_active"] == 1 << scale  # the tree reaches every vertex
+    assert c.nnz == g["nnz"]
+    assert c.find_roots(16).tolist() == g["roots"]
+    for pr in g["per_root"][:6]:
+        lv, _, tr = c.bfs(pr["root"], want_pred=False)
+        # the reference's pred arrays are rounded up to 2^16 entries (unreached)
+        lv = np.concatenate([lv, np.full(g["pred_size"] - len(lv), 0xFF, np.uint8)])
+        assert hashlib.sha256(lv.tobytes()).hexdigest() == pr["levels_sha256"], pr["root"]
+        assert "".join("T" if d else "B" for d in tr["direction"]) == pr["directions"]
+        assert tr["teps_edges"] == pr["nedge"] == len(t)
