@@ -374,8 +374,13 @@ int g5_validate_pred(g5_ctx* ctx, int64_t root, const int64_t* pred, int64_t* er
 
 int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* t) {
   return guarded(ctx, [&] {
-    G5_CHECK(t && t->bin_edges >= 0, "g5_bfs_set_tuning: bad tuning");
+    G5_CHECK(t && t->bin_edges >= 0 && t->td_pieces >= 0 && t->td_pieces <= 4096 &&
+             (t->block_shift == 0 || (t->block_shift >= 12 && t->block_shift <= 31)) && t->td_head >= 0,
+             "g5_bfs_set_tuning: bad tuning");
     ctx->bfs.bin_edges = t->bin_edges;
+    ctx->bfs.td_pieces = t->td_pieces;
+    ctx->bfs.hub_shift = t->block_shift;
+    ctx->bfs.td_head = t->td_head;
   });
 }
 
@@ -384,6 +389,9 @@ int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* t) {
     G5_CHECK(t, "g5_bfs_get_tuning: null output");
     memset(t, 0, sizeof(*t));
     t->bin_edges = ctx->bfs.bin_edges;
+    t->td_pieces = ctx->bfs.td_pieces;
+    t->block_shift = ctx->bfs.hub_shift;
+    t->td_head = ctx->bfs.td_head;
   });
 }
 

@@ -39,7 +39,9 @@ constexpr int kBinShift = 19;           // binned top-down: at least 2^19 target
 constexpr int kShortRow = 32;    // bottom-up: lane-sequential scan up to this length
 constexpr uint32_t kNoNbr = 0xFFFFFFFFu;
 
-enum { C_FOUND = 0, C_AUX = 1, C_NUM = 4 };
+// counters: vertices found; the root (new id); head words (k_td_head_size);
+// frontier edges of the next level, summed by a small top-down level
+enum { C_FOUND = 0, C_AUX = 1, C_HEAD = 2, C_NEXT_E = 3, C_NUM = 4 };
 
 
 __global__ void __launch_bounds__(kBlock) k_bfs_init(uint8_t* __restrict__ level, uint64_t* __restrict__ vis,
@@ -75,18 +77,60 @@ __global__ void k_bfs_root(const uint32_t* __restrict__ old2new, int64_t root_or
 //   kBulk = false (small levels): test-and-set on the visited bitmap itself
 //           (atomicOr after a plain-load filter); winners are staged in LDS
 //           and appended to 'next' with one global atomic per chunk, which
-//           also counts them in counters[C_FOUND].
-//   kBulk = true  (hub levels): no atomics and no bitmap probe (the snapshot
-//           holds few vertices at such a level): a target gets level and
-//           parent unless its level is already set (racing writers store the
-//           same level and a valid parent); k_td_finish then takes the exact
-//           new set from the level array.
+//           also counts them in counters[C_FOUND]; their degree sum (the
+//           next level's frontier edges) goes to counters[C_NEXT_E].
+//   kBulk = true  (hub levels): no atomics: a target gets level and parent
+//           unless its level is already set (racing writers store the same
+//           level and a valid parent); k_td_finish then takes the exact new
+//           set from the level array.  The visited-bitmap snapshot is probed
+//           first only when probe_vis is set (levels after the bottom-up
+//           phase, where it drops most targets); while the search grows it
+//           holds few vertices and the level test alone decides.
+// Rows below the returned id were settled by k_td_head (none without a head).
+__device__ __forceinline__ uint32_t head_bound(const unsigned long long* head_words) {
+  if (!head_words) return 0;
+  const unsigned long long v = *head_words * 64;
+  return v < UINT32_MAX ? (uint32_t)v : UINT32_MAX;
+}
+
+__device__ __forceinline__ void bulk_claims(uint64_t e0, uint64_t e1, int cnt, const ChunkRows& cr,
+                                            const uint32_t* __restrict__ cols, const uint64_t* vis,
+                                            uint8_t* level, uint32_t* __restrict__ parent, uint32_t lvl,
+                                            int probe_vis, uint32_t head) {
+  // every probe of the lane's edges in flight before any claim
+  uint32_t w[kEpt], u[kEpt];
+  bool ok[kEpt];
+  gather_strided(e0, e1, cnt, cr, cols, w, u, ok);
+  if (probe_vis) {
+    // most targets are visited already: the bitmap (1/8 of the level
+    // array's footprint) drops them before any level probe
+    uint64_t vw[kEpt];
+#pragma unroll
+    for (int j = 0; j < kEpt; ++j) vw[j] = ok[j] ? vis[w[j] >> 6] : 0;
+#pragma unroll
+    for (int j = 0; j < kEpt; ++j) ok[j] = ok[j] && !((vw[j] >> (w[j] & 63)) & 1);
+  }
+  // targets below 'head' were settled by k_td_head
+#pragma unroll
+  for (int j = 0; j < kEpt; ++j) ok[j] = ok[j] && w[j] >= head;
+  uint8_t lv[kEpt];
+#pragma unroll
+  for (int j = 0; j < kEpt; ++j) lv[j] = ok[j] ? level[w[j]] : 0;
+#pragma unroll
+  for (int j = 0; j < kEpt; ++j)
+    if (ok[j] && lv[j] == 0xFF) {
+      level[w[j]] = (uint8_t)lvl;
+      parent[w[j]] = u[j];
+    }
+}
+
 template <bool kBulk>
 __global__ void __launch_bounds__(kBlock) k_td_expand(
     const uint32_t* __restrict__ queue, int64_t nf, const uint64_t* __restrict__ prefix,
     const uint32_t* __restrict__ chunk_rows, const uint64_t* __restrict__ offsets,
     const uint32_t* __restrict__ cols, uint64_t* vis, uint8_t* level, uint32_t* __restrict__ parent,
-    uint32_t* __restrict__ next, unsigned long long* __restrict__ counters, uint32_t lvl) {
+    uint32_t* __restrict__ next, unsigned long long* __restrict__ counters, uint32_t lvl, int probe_vis,
+    const unsigned long long* __restrict__ head_words) {
   __shared__ ChunkRows cr;
   __shared__ uint32_t s_q[kBulk ? 1 : kEpb];
   __shared__ uint32_t s_qn;
@@ -94,6 +138,8 @@ __global__ void __launch_bounds__(kBlock) k_td_expand(
   const uint64_t E = prefix[nf];
   // both forms are launched before the host knows E; the other one exits
   if (kBulk ? E <= kTdSmall : E > kTdSmall) return;
+  const uint32_t head = head_bound(head_words);
+  uint64_t next_e = 0;  // degree sum of this lane's winners (small levels)
   for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
     const uint64_t e0 = chunk * kEpb;
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
@@ -101,34 +147,16 @@ __global__ void __launch_bounds__(kBlock) k_td_expand(
     const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
     const auto visit = [&](uint32_t w, uint32_t u) {
       const uint64_t bit = UINT64_C(1) << (w & 63);
-      // a hub level's snapshot holds few vertices: the level test alone decides
-      if (!kBulk && (vis[w >> 6] & bit)) return;
-      if (kBulk) {
-        if (level[w] != 0xFF) return;
-        level[w] = (uint8_t)lvl;
-        parent[w] = u;
-      } else {
-        const uint64_t old = atomicOr((unsigned long long*)(vis + (w >> 6)), (unsigned long long)bit);
-        if (old & bit) return;
-        level[w] = (uint8_t)lvl;
-        parent[w] = u;
-        s_q[atomicAdd(&s_qn, 1u)] = w;
-      }
+      if (vis[w >> 6] & bit) return;
+      const uint64_t old = atomicOr((unsigned long long*)(vis + (w >> 6)), (unsigned long long)bit);
+      if (old & bit) return;
+      level[w] = (uint8_t)lvl;
+      parent[w] = u;
+      s_q[atomicAdd(&s_qn, 1u)] = w;
+      next_e += offsets[w + 1] - offsets[w];
     };
     if (kBulk) {
-      // every probe of the lane's edges in flight before any claim
-      uint32_t w[kEpt], u[kEpt];
-      bool ok[kEpt];
-      gather_strided(e0, e1, cnt, cr, cols, w, u, ok);
-      uint8_t lv[kEpt];
-#pragma unroll
-      for (int j = 0; j < kEpt; ++j) lv[j] = ok[j] ? level[w[j]] : 0;
-#pragma unroll
-      for (int j = 0; j < kEpt; ++j)
-        if (ok[j] && lv[j] == 0xFF) {
-          level[w[j]] = (uint8_t)lvl;
-          parent[w[j]] = u[j];
-        }
+      bulk_claims(e0, e1, cnt, cr, cols, vis, level, parent, lvl, probe_vis, head);
     } else {
       walk_strided(e0, e1, cnt, cr, cols, visit);
     }
@@ -138,6 +166,228 @@ __global__ void __launch_bounds__(kBlock) k_td_expand(
       __syncthreads();
       for (uint32_t j = threadIdx.x; j < s_qn; j += kBlock) next[s_qbase + j] = s_q[j];
     }
+    __syncthreads();
+  }
+  // the next level's frontier edges, so that the host can skip the launches
+  // of the other form there
+  if (!kBulk) block_add(next_e, counters + C_NEXT_E);
+}
+
+// --- piece-ordered hub levels ---------------------------------------------
+// A hub level's frontier rows hold up to billions of edges whose targets,
+// sorted within each row, spread over the whole level array, so taking the
+// rows one after another keeps the grid's probes scattered over all of it.
+// Instead every row is cut into P pieces by position (piece k of a row of
+// degree d holds its positions [k*d/P, (k+1)*d/P)) and pieces are taken piece
+// index first.  The rows of a Kronecker graph share nearly one neighbour
+// distribution (a target bit is 0 with probability A/(A+B) = 0.75 or
+// C/(C+D) = 0.79 for the two values of the row bit under the 0.57/0.19/0.19
+// initiator), so the k-th pieces of all rows cover about the same target
+// range and the level and parent bytes the grid touches at a time stay
+// cache-sized.  One scan compacts the non-empty pieces into a segment list
+// (the scanned value carries the piece count above bit 40 and its edges
+// below) that the chunked bulk expansion walks like frontier rows.
+constexpr int kSegCountShift = 40;
+constexpr uint64_t kSegEdgeMask = (UINT64_C(1) << kSegCountShift) - 1;
+constexpr int64_t kSegMax = (INT64_C(1) << 24) - 1;  // segments of one level (count field width)
+
+struct PieceLen {
+  const uint32_t* queue;
+  const uint64_t* offsets;
+  const uint64_t* prefix;
+  int64_t nf;
+  int64_t pieces;
+  // piece s = k * nf + i: frontier row i's piece k -> source, first cols index, length
+  __device__ void piece(size_t s, uint32_t& u, uint64_t& a, uint64_t& len) const {
+    const int64_t k = (int64_t)s / nf, i = (int64_t)s - k * nf;
+    u = queue[i];
+    const uint64_t o = offsets[u], d = offsets[u + 1] - o;
+    a = o + (uint64_t)k * d / (uint64_t)pieces;
+    len = o + (uint64_t)(k + 1) * d / (uint64_t)pieces - a;
+  }
+  __device__ uint64_t operator()(size_t s) const {
+    // a small level (E <= kTdSmall) makes no segments
+    if ((int64_t)s >= nf * pieces || prefix[nf] <= kTdSmall) return 0;
+    uint32_t u;
+    uint64_t a, len;
+    piece(s, u, a, len);
+    return len ? (UINT64_C(1) << kSegCountShift) | len : 0;
+  }
+};
+// Exact target blocks (block = 2^shift consecutive target ids): the first
+// hub_rows rows (the highest-degree ones in degree order) carry the cols
+// index of their first entry in every block, relative to the row start,
+// bnd[v * (nb + 1) + b] for b in [0, nb] (bnd[.. + nb] = degree).  Piece
+// s = k * nf + i is row i's block k for a hub row and, for k = nb, the whole
+// row of any other frontier vertex (whose edges are few at a hub level).
+struct BlockLen {
+  const uint32_t* queue;
+  const uint64_t* offsets;
+  const uint64_t* prefix;
+  int64_t nf;
+  int64_t pieces;  // nb + 1
+  const uint32_t* bnd;
+  int64_t hub_rows;
+  __device__ void piece(size_t s, uint32_t& u, uint64_t& a, uint64_t& len) const {
+    const int64_t k = (int64_t)s / nf, i = (int64_t)s - k * nf;
+    u = queue[i];
+    const uint64_t o = offsets[u];
+    a = o;
+    len = 0;
+    if ((int64_t)u < hub_rows) {
+      if (k + 1 < pieces) {
+        const uint32_t* r = bnd + (int64_t)u * pieces + k;
+        a = o + r[0];
+        len = r[1] - r[0];
+      }
+    } else if (k + 1 == pieces) {
+      len = offsets[u + 1] - o;
+    }
+  }
+  __device__ uint64_t operator()(size_t s) const {
+    if ((int64_t)s >= nf * pieces || prefix[nf] <= kTdSmall) return 0;
+    uint32_t u;
+    uint64_t a, len;
+    piece(s, u, a, len);
+    return len ? (UINT64_C(1) << kSegCountShift) | len : 0;
+  }
+};
+
+// bnd of the block index (one lane per row and block; the lanes of a wave
+// share a row, so the first steps of their binary searches coincide)
+__global__ void __launch_bounds__(kBlock) k_hub_bounds(const uint64_t* __restrict__ offsets,
+                                                       const uint32_t* __restrict__ cols, int64_t hub_rows,
+                                                       int64_t pieces, int shift, uint32_t* __restrict__ bnd) {
+  const int64_t total = hub_rows * pieces;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = t / pieces, b = t - v * pieces;
+    const uint64_t key = (uint64_t)b << shift;
+    uint64_t lo = offsets[v], hi = offsets[v + 1];
+    const uint64_t o = lo;
+    while (lo < hi) {  // first entry >= key
+      const uint64_t mid = (lo + hi) >> 1;
+      if (cols[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    bnd[t] = (uint32_t)(lo - o);
+  }
+}
+
+// With a head (k_td_head), the edges a frontier row still has to expand are
+// its suffix of targets >= the head bound (rows are sorted): one segment per
+// row, whose start k_suffix_start finds by binary search (one lane per row),
+// so the expansion no longer reads the columns of edges into the head.
+__global__ void __launch_bounds__(kBlock) k_suffix_start(const uint32_t* __restrict__ queue, int64_t nf,
+                                                         const uint64_t* __restrict__ prefix,
+                                                         const uint64_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ cols,
+                                                         const unsigned long long* __restrict__ head_words,
+                                                         uint64_t* __restrict__ start) {
+  if (prefix[nf] <= kTdSmall) return;
+  const uint32_t h = head_bound(head_words);
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nf; i += (int64_t)gridDim.x * kBlock) {
+    const uint32_t u = queue[i];
+    uint64_t lo = offsets[u], hi = offsets[u + 1];
+    while (lo < hi) {  // first entry >= h
+      const uint64_t mid = (lo + hi) >> 1;
+      if (cols[mid] < h) lo = mid + 1; else hi = mid;
+    }
+    start[i] = lo;
+  }
+}
+
+struct SuffixLen {
+  const uint32_t* queue;
+  const uint64_t* offsets;
+  const uint64_t* prefix;
+  int64_t nf;
+  int64_t pieces;         // 1
+  const uint64_t* start;  // k_suffix_start
+  __device__ void piece(size_t s, uint32_t& u, uint64_t& a, uint64_t& len) const {
+    u = queue[s];
+    a = start[s];
+    len = offsets[u + 1] - a;
+  }
+  __device__ uint64_t operator()(size_t s) const {
+    if ((int64_t)s >= nf || prefix[nf] <= kTdSmall) return 0;
+    uint32_t u;
+    uint64_t a, len;
+    piece(s, u, a, len);
+    return len ? (UINT64_C(1) << kSegCountShift) | len : 0;
+  }
+};
+
+template <class Len>
+struct PieceEmit {
+  Len in;
+  uint64_t* seg_pre;   // first virtual edge of each segment; seg_pre[count] = E
+  uint64_t* seg_base;  // cols index - seg_pre
+  uint32_t* seg_src;
+  uint64_t* seg_count;
+  __device__ void operator()(size_t s, uint64_t pre) const {
+    const uint64_t pos = pre >> kSegCountShift, ep = pre & kSegEdgeMask;
+    if ((int64_t)s == in.nf * in.pieces) {  // the extra last element carries the totals
+      seg_count[0] = pos;
+      seg_pre[pos] = ep;
+      return;
+    }
+    if (in.prefix[in.nf] <= kTdSmall) return;
+    uint32_t u;
+    uint64_t a, len;
+    in.piece(s, u, a, len);
+    if (!len) return;
+    seg_pre[pos] = ep;
+    seg_base[pos] = a - ep;
+    seg_src[pos] = u;
+  }
+};
+
+// k_chunk_rows over the segment list, whose length is known on the device only.
+__global__ void __launch_bounds__(kBlock) k_chunk_segs(const uint64_t* __restrict__ seg_pre,
+                                                       const uint64_t* __restrict__ seg_count, uint64_t nchunks,
+                                                       uint32_t* __restrict__ rows_out) {
+  const int64_t ns = (int64_t)seg_count[0];
+  if (ns == 0) return;
+  const uint64_t E = seg_pre[ns];
+  const uint64_t nc = (E + kEpb - 1) / kEpb < nchunks ? (E + kEpb - 1) / kEpb : nchunks;
+  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < 2 * nc; t += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t c = t >> 1;
+    uint64_t key = c * kEpb;
+    if (t & 1) key = ((key + kEpb < E) ? key + kEpb : E) - 1;
+    int64_t lo = 0, hi = ns - 1;  // last segment with seg_pre <= key
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (seg_pre[mid] <= key) lo = mid; else hi = mid - 1;
+    }
+    rows_out[t] = (uint32_t)lo;
+  }
+}
+
+// The bulk expansion over the segment list: a bulk level (E = prefix[nf] >
+// kTdSmall) walks the seg_pre[count] edges of its segments.
+__global__ void __launch_bounds__(kBlock) k_td_expand_segs(
+    const uint64_t* __restrict__ prefix, int64_t nf, const uint64_t* __restrict__ seg_pre,
+    const uint64_t* __restrict__ seg_base, const uint32_t* __restrict__ seg_src,
+    const uint64_t* __restrict__ seg_count, const uint32_t* __restrict__ chunk_rows,
+    const uint32_t* __restrict__ cols, const uint64_t* vis, uint8_t* level, uint32_t* __restrict__ parent,
+    uint32_t lvl, const unsigned long long* __restrict__ head_words) {
+  __shared__ ChunkRows cr;
+  if (prefix[nf] <= kTdSmall) return;
+  const uint64_t E = seg_pre[*seg_count];
+  const uint32_t head = head_bound(head_words);
+  for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
+    const uint64_t e0 = chunk * kEpb;
+    const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
+    // every segment is non-empty, so at most kEpb + 1 of them overlap a chunk
+    const int64_t i0 = chunk_rows[2 * chunk];
+    int cnt = (int)((int64_t)chunk_rows[2 * chunk + 1] - i0 + 1);
+    cnt = cnt > kEpb + 1 ? kEpb + 1 : cnt;
+    for (int j = threadIdx.x; j < cnt; j += kBlock) {
+      cr.pre[j] = seg_pre[i0 + j];
+      cr.base[j] = seg_base[i0 + j];
+      cr.src[j] = seg_src[i0 + j];
+    }
+    __syncthreads();
+    bulk_claims(e0, e1, cnt, cr, cols, vis, level, parent, lvl, 0, head);
     __syncthreads();
   }
 }
@@ -210,6 +460,63 @@ __device__ __forceinline__ bool first_marked_nbr(bool active, uint32_t v, const 
     }
   }
   return found;
+}
+
+// Head of a growing bulk top-down level: the rows [0, 64 * head_words) (the
+// highest-degree vertices in degree order) are settled bottom-up before the
+// expansion, which then skips every edge into them.  An unvisited vertex
+// whose neighbour is visited is reached at this level, because a neighbour
+// from an older level would have reached it earlier, so the visited bitmap
+// stands for the frontier and the vertex gets the same level it would get
+// top-down, with a valid parent.  The hubs' rows are long, but most of them
+// meet a visited neighbour within their first (highest-degree) entries,
+// whereas top-down every frontier row probes them again and again.
+__global__ void __launch_bounds__(kBlock) k_td_head(const uint64_t* __restrict__ vis,
+                                                    const uint32_t* __restrict__ first_nbr,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ cols, uint8_t* __restrict__ level,
+                                                    uint32_t* __restrict__ parent,
+                                                    const unsigned long long* __restrict__ head_words_p, uint32_t lvl) {
+  const int64_t head_words = (int64_t)*head_words_p;  // 0 at a small level
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < head_words; w += nwaves) {
+    const uint64_t cur = vis[w];
+    if (cur == ~UINT64_C(0)) continue;
+    const uint32_t v = (uint32_t)(w * 64 + lane);
+    uint32_t p = 0;
+    if (first_marked_nbr(!((cur >> lane) & 1), v, vis, first_nbr, offsets, cols, p)) {
+      level[v] = (uint8_t)lvl;
+      parent[v] = p;
+    }
+  }
+}
+
+// How far the head reaches, from the level's frontier edge count E: a row
+// of degree d belongs to it when d >= factor * (nnz / E)^2.  Settling such a
+// vertex bottom-up takes about nnz / E probes (a random neighbour is a
+// frontier vertex with probability about E / nnz), while top-down its
+// level byte is probed about d * E / nnz times; the two balance at
+// d = (nnz / E)^2.  Rows are in descending degree order, so the head is the
+// prefix of rows above the threshold (found by binary search), in bitmap
+// words.  A small level (E <= kTdSmall) gets none.
+__global__ void k_td_head_size(const uint64_t* __restrict__ prefix, int64_t nf,
+                               const uint64_t* __restrict__ offsets, int64_t rows, int64_t words, double nnz,
+                               double factor, unsigned long long* __restrict__ head_words) {
+  const uint64_t E = prefix[nf];
+  int64_t h = 0;
+  if (E > kTdSmall) {
+    const double r = nnz / (double)E;
+    const double thr = factor * r * r;
+    int64_t lo = 0, hi = rows;  // first row with degree < thr
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if ((double)(offsets[mid + 1] - offsets[mid]) >= thr) lo = mid + 1; else hi = mid;
+    }
+    h = (lo + 63) / 64;
+    if (h > words) h = words;
+  }
+  *head_words = (unsigned long long)h;
 }
 
 // Binned top-down for hub levels (hundreds of millions of frontier edges):
@@ -525,6 +832,24 @@ void bfs_prepare(g5_ctx* ctx) {
   b.prepared = true;
 }
 
+// The block index of the first rows: as many as fit kHubIndexBytes.
+constexpr int64_t kHubIndexBytes = INT64_C(512) << 20;
+
+static void build_hub_index(g5_ctx* ctx) {
+  Graph& g = ctx->graph;
+  BfsState& b = ctx->bfs;
+  const int shift = (int)b.hub_shift;
+  const int64_t nb = ((g.n_rows - 1) >> shift) + 1;
+  b.hub_pieces = nb + 1;
+  b.hub_rows = std::min<int64_t>(g.n_rows, kHubIndexBytes / (4 * b.hub_pieces));
+  b.hub_bnd.alloc((size_t)(b.hub_rows * b.hub_pieces));
+  hipLaunchKernelGGL(k_hub_bounds, dim3(grid_of(b.hub_rows * b.hub_pieces, kBlock, 65536)), dim3(kBlock), 0,
+                     ctx->stream, g.offsets.p, g.cols.p, b.hub_rows, b.hub_pieces, shift, b.hub_bnd.p);
+  G5_HIP(hipGetLastError());
+  G5_HIP(hipStreamSynchronize(ctx->stream));
+  b.hub_built_shift = b.hub_shift;
+}
+
 void bfs_run(g5_ctx* ctx, int64_t root) {
   Graph& g = ctx->graph;
   BfsState& b = ctx->bfs;
@@ -537,6 +862,8 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
   b.root_orig = root;
   g5_bfs_stats& st = b.stats;
   st = g5_bfs_stats{};
+
+  if (b.hub_shift > 0 && b.hub_built_shift != b.hub_shift) build_hub_index(ctx);
 
   G5_HIP(hipEventRecord(b.ev0, s));
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, b.level.p,
@@ -553,6 +880,9 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
   int64_t nf = 1;
   bool next_forward = true, growing = true;
   int64_t prev_nq = 0, visited = 1;
+  // frontier edges of the coming top-down level when a small level summed
+  // them (-1: unknown, both forms are launched and the device picks)
+  int64_t e_next = -1;
   const double na = (double)g.n_active;
   int lvl = 0;
   if (!root_active) {
@@ -564,7 +894,7 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     ++lvl;
     G5_CHECK(lvl < 255, "g5_run_bfs: more than 254 levels (uint8 level array)");
     const bool forward = next_forward;
-    G5_HIP(hipMemsetAsync(b.counters.p, 0, 8, s));
+    G5_HIP(hipMemsetAsync(b.counters.p, 0, C_NUM * sizeof(uint64_t), s));
     G5_HIP(hipEventRecord(b.lev_begin[lvl], s));
     if (forward) {
       device_scan(FrontierDegree{b.queue[qi].p, g.offsets.p, nf}, StoreU64{b.prefix.p},
@@ -575,6 +905,28 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
         G5_HIP(hipStreamSynchronize(s));
       }
       const uint64_t E_known = b.bin_edges > 0 ? b.h_counters[1] : 0;
+      // past the bottom-up phase most targets are visited: a bulk level then
+      // tests the visited bitmap before the level array
+      const int probe_vis = (double)visited > na / 4 ? 1 : 0;
+      // while the search grows, a bulk level takes its rows block by block
+      // (exact target blocks) or piece by piece (positional pieces)
+      const bool known_small = e_next >= 0 && (uint64_t)e_next <= kTdSmall;
+      const bool known_bulk = e_next >= 0 && (uint64_t)e_next > kTdSmall;
+      const bool blocked = b.hub_shift > 0 && !probe_vis && nf * b.hub_pieces <= kSegMax;
+      // rows [0, head) settled bottom-up (a multiple of 64 up to the bitmap's end)
+      // a growing bulk level settles its highest-degree targets bottom-up
+      // first (degree order only)
+      bool head = b.td_head > 0 && !probe_vis && g.reorder_mode == 2;
+      if (head && e_next > (int64_t)kTdSmall) {
+        // E known on the host: no head when no row reaches the threshold
+        // (the same test k_td_head_size makes)
+        const double r = (double)g.nnz / (double)e_next;
+        if ((double)b.td_head / 16.0 * r * r > (double)g.max_degree) head = false;
+      }
+      unsigned long long* head_words = head ? cnt + C_HEAD : nullptr;
+      const bool pieces = blocked || (b.td_pieces > 0 && !probe_vis && nf * b.td_pieces <= kSegMax);
+      // with a head and row order, the expansion walks row suffixes
+      const bool suffix = head && !pieces;
       const bool binned = b.bin_edges > 0 && E_known > kTdSmall && E_known >= (uint64_t)b.bin_edges;
       // Without binning no launch waits for E on the host: chunk rows and the
       // expansion take upper-bound grids and read E (prefix[nf]) on the
@@ -588,10 +940,21 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
         hipLaunchKernelGGL(k_chunk_rows<kEpb>,
                            dim3(b.bin_edges > 0 ? grid_of((int64_t)(2 * nchunks), kBlock, 65536) : 4096u),
                            dim3(kBlock), 0, s, b.prefix.p, nf, nchunks, b.chunk_rows.p);
-      hipLaunchKernelGGL(k_td_expand<false>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
-                         b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p, b.parent.p,
-                         b.queue[qi ^ 1].p, cnt, (uint32_t)lvl);
-      if (binned) {
+      if (!known_bulk)
+        hipLaunchKernelGGL(k_td_expand<false>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
+                           b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p, b.parent.p,
+                           b.queue[qi ^ 1].p, cnt, (uint32_t)lvl, 0, nullptr);
+      if (head && !binned && !known_small) {
+        hipLaunchKernelGGL(k_td_head_size, dim3(1), dim3(1), 0, s, b.prefix.p, nf, g.offsets.p, rows, b.words,
+                           (double)g.nnz, (double)b.td_head / 16.0, head_words);
+        hipLaunchKernelGGL(k_td_head, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
+                           b.vis[cur].p, g.first_nbr.p, g.offsets.p, g.cols.p, b.level.p, b.parent.p, head_words,
+                           (uint32_t)lvl);
+        G5_HIP(hipGetLastError());
+      }
+      if (known_small) {
+        // nothing else to launch: the small expansion keeps the queue exact
+      } else if (binned) {
         // binned by target range (see k_td_bin_count)
         const uint64_t E = E_known;
         int shift = kBinShift;
@@ -608,17 +971,48 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
                            b.chunk_rows.p, g.offsets.p, g.cols.p, shift, nb, bc + kBinMaxBuckets, b.bin_pairs.p);
         hipLaunchKernelGGL(k_td_bin_apply, dim3(grid_of((int64_t)E, kBlock, 2048)), dim3(kBlock), 0, s,
                            b.bin_pairs.p, E, b.level.p, b.parent.p, (uint32_t)lvl);
+      } else if (pieces || suffix) {
+        // segment-list bulk level; a small level makes an empty segment list
+        // and the expansion exits (both read E on the device)
+        const size_t np = (size_t)(suffix ? nf : nf * (blocked ? b.hub_pieces : b.td_pieces)) + 1;
+        b.seg_pre.alloc(np);
+        b.seg_base.alloc(np);
+        b.seg_src.alloc(np);
+        b.seg_count.alloc(1);
+        b.seg_scan_tmp.alloc(scan_tmp_elems(np));
+        if (suffix) {
+          b.seg_start.alloc(np);
+          hipLaunchKernelGGL(k_suffix_start, dim3(grid_of(nf, kBlock, 4096)), dim3(kBlock), 0, s, b.queue[qi].p, nf,
+                             b.prefix.p, g.offsets.p, g.cols.p, head_words, b.seg_start.p);
+          const SuffixLen sl{b.queue[qi].p, g.offsets.p, b.prefix.p, nf, 1, b.seg_start.p};
+          device_scan(sl, PieceEmit<SuffixLen>{sl, b.seg_pre.p, b.seg_base.p, b.seg_src.p, b.seg_count.p}, np,
+                      b.seg_scan_tmp.p, s);
+        } else if (blocked) {
+          const BlockLen bl{b.queue[qi].p, g.offsets.p, b.prefix.p, nf, b.hub_pieces, b.hub_bnd.p, b.hub_rows};
+          device_scan(bl, PieceEmit<BlockLen>{bl, b.seg_pre.p, b.seg_base.p, b.seg_src.p, b.seg_count.p}, np,
+                      b.seg_scan_tmp.p, s);
+        } else {
+          const PieceLen pl{b.queue[qi].p, g.offsets.p, b.prefix.p, nf, b.td_pieces};
+          device_scan(pl, PieceEmit<PieceLen>{pl, b.seg_pre.p, b.seg_base.p, b.seg_src.p, b.seg_count.p}, np,
+                      b.seg_scan_tmp.p, s);
+        }
+        hipLaunchKernelGGL(k_chunk_segs, dim3(4096), dim3(kBlock), 0, s, b.seg_pre.p, b.seg_count.p, max_chunks,
+                           b.chunk_rows.p);
+        hipLaunchKernelGGL(k_td_expand_segs, dim3(grid), dim3(kBlock), 0, s, b.prefix.p, nf, b.seg_pre.p,
+                           b.seg_base.p, b.seg_src.p, b.seg_count.p, b.chunk_rows.p, g.cols.p, b.vis[cur].p,
+                           b.level.p, b.parent.p, (uint32_t)lvl, head_words);
       } else {
         hipLaunchKernelGGL(k_td_expand<true>, dim3(grid), dim3(kBlock), 0, s, b.queue[qi].p, nf, b.prefix.p,
                            b.chunk_rows.p, g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p, b.parent.p,
-                           (uint32_t*)nullptr, cnt, (uint32_t)lvl);
+                           (uint32_t*)nullptr, cnt, (uint32_t)lvl, probe_vis, binned ? nullptr : head_words);
       }
       G5_HIP(hipGetLastError());
       // after a bulk level, k_td_finish makes vis[cur^1] exactly this level's
       // vertices and adds them to vis[cur]
-      hipLaunchKernelGGL(k_td_finish, dim3(grid_of((b.words + 15) / 16, kBlock / 64, 8192)), dim3(kBlock), 0, s,
-                         b.level.p, rows, b.words, (uint32_t)lvl, b.vis[cur].p, b.vis[cur ^ 1].p, cnt, b.prefix.p,
-                         nf);
+      if (!known_small)
+        hipLaunchKernelGGL(k_td_finish, dim3(grid_of((b.words + 15) / 16, kBlock / 64, 8192)), dim3(kBlock), 0,
+                           s, b.level.p, rows, b.words, (uint32_t)lvl, b.vis[cur].p, b.vis[cur ^ 1].p, cnt,
+                           b.prefix.p, nf);
     } else {
       hipLaunchKernelGGL(k_bu, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
                          b.vis[cur].p, b.vis[cur ^ 1].p, g.first_nbr.p, g.offsets.p, g.cols.p,
@@ -626,11 +1020,13 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(b.lev_end[lvl], s));
-    G5_HIP(hipMemcpyAsync(b.h_counters, b.counters.p, 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipMemcpyAsync(b.h_counters, b.counters.p, C_NUM * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     if (forward) G5_HIP(hipMemcpyAsync(b.h_counters + 1, b.prefix.p + nf, 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     const int64_t nq = (int64_t)b.h_counters[0];
     const bool bulk = forward && b.h_counters[1] > kTdSmall;
+    // a small top-down level summed its winners' degrees
+    e_next = (forward && !bulk) ? (int64_t)b.h_counters[C_NEXT_E] : -1;
     {
       float lms = 0;
       G5_HIP(hipEventElapsedTime(&lms, b.lev_begin[lvl], b.lev_end[lvl]));

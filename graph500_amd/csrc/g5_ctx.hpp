@@ -62,6 +62,19 @@ struct BfsState {
   int64_t bin_edges = 0;  // top-down levels with at least this many edges are binned (0: never)
   DevBuf<uint64_t> bin_pairs;     // (target << 32 | source) of a binned level
   DevBuf<uint64_t> bin_cnt;       // bucket counts and cursors
+  int64_t td_pieces = 0;          // pieces per row of a piece-ordered hub level (0: row order)
+  DevBuf<uint64_t> seg_pre;       // segment list of a piece-ordered level (g5_bfs.hip)
+  DevBuf<uint64_t> seg_base;
+  DevBuf<uint32_t> seg_src;
+  DevBuf<uint64_t> seg_count;
+  DevBuf<uint64_t> seg_scan_tmp;
+  DevBuf<uint64_t> seg_start;     // first cols index of each row suffix
+  int64_t td_head = 4;            // head threshold factor in 1/16 (0: no head; see k_td_head_size)
+  int64_t hub_shift = 0;          // target block = 2^hub_shift ids (0: no block index)
+  int64_t hub_built_shift = 0;    // shift the index was built for (0: none)
+  int64_t hub_pieces = 0;         // blocks + 1
+  int64_t hub_rows = 0;           // rows [0, hub_rows) are indexed
+  DevBuf<uint32_t> hub_bnd;       // hub_rows * hub_pieces
   g5_bfs_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> lev_begin, lev_end;  // per level
@@ -78,6 +91,14 @@ struct BfsState {
     level_out.release();
     bin_pairs.release();
     bin_cnt.release();
+    seg_pre.release();
+    seg_base.release();
+    seg_src.release();
+    seg_count.release();
+    seg_scan_tmp.release();
+    seg_start.release();
+    hub_bnd.release();
+    hub_built_shift = 0;
     if (h_counters) (void)hipHostFree(h_counters);
     h_counters = nullptr;
     if (ev0) (void)hipEventDestroy(ev0);

@@ -353,17 +353,34 @@ class G5Context:
         L = self.bfs_stats()["num_levels"]
         return per[1:L].tolist(), tot.value
 
-    def bfs_set_tuning(self, bin_edges):
-        """Top-down levels with at least bin_edges frontier edges are binned by
-        target range (0 = never)."""
+    def bfs_set_tuning(self, bin_edges=None, td_pieces=None, block_shift=None, td_head=None):
+        """Single-root tuning; a knob left None keeps its value.  bin_edges:
+        top-down levels with at least this many frontier edges are binned by
+        target range (0 = never).  td_pieces: a growing bulk top-down level
+        takes its rows in this many pieces, piece index first (0 = row order).
+        block_shift: a growing bulk top-down level takes its edges by target
+        block of 2^block_shift ids (0 = off; it then falls back to td_pieces).
+        td_head: a growing bulk top-down level with E frontier edges settles
+        its targets of degree >= td_head / 16 * (nnz / E)^2 (a prefix of the
+        degree-ordered rows) with a bottom-up probe and skips their edges
+        (0 = off; degree-ordered graphs only)."""
         t = np.zeros(4, np.int64)
-        t[0] = bin_edges
+        self._check(self._lib.g5_bfs_get_tuning(self._h, _ptr(t)))
+        if bin_edges is not None:
+            t[0] = bin_edges
+        if td_pieces is not None:
+            t[1] = td_pieces
+        if block_shift is not None:
+            t[2] = block_shift
+        if td_head is not None:
+            t[3] = td_head
         self._check(self._lib.g5_bfs_set_tuning(self._h, _ptr(t)))
 
     def bfs_tuning(self):
         t = np.zeros(4, np.int64)
         self._check(self._lib.g5_bfs_get_tuning(self._h, _ptr(t)))
-        return {"bin_edges": int(t[0])}
+        return {"bin_edges": int(t[0]), "td_pieces": int(t[1]), "block_shift": int(t[2]),
+                "td_head": int(t[3])}
 
     VALIDATION_RULES = ["root", "self_parent", "depth", "edge", "parent_edge", "range"]
 

@@ -211,7 +211,9 @@ int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
  * fit the 256 MB last-level cache and the plain probes are twice as fast). */
 typedef struct g5_bfs_tuning {
   int64_t bin_edges;
-  int64_t reserved[3];
+  int64_t td_pieces;
+  int64_t block_shift;
+  int64_t td_head;
 } g5_bfs_tuning;
 int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* tuning);
 int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* tuning);

@@ -123,16 +123,29 @@ def test_bfs_64_roots_bit_exact_s16(ctx, s16, oracle_mod):
         assert cnt == pr["nedge"]
 
 
-@pytest.mark.parametrize("scale,bin_edges", [(18, None), (20, None), (22, None), (20, 1), (22, 1 << 22)])
-def test_bfs_reference_fixtures_larger_scales(scale, bin_edges, oracle_mod):
-    """bin_edges None keeps the default; 1 bins every bulk top-down level."""
+@pytest.mark.parametrize("scale,tuning", [(18, {}), (20, {}), (22, {}), (20, {"bin_edges": 1}),
+                                          (22, {"bin_edges": 1 << 22}), (20, {"td_pieces": 0}),
+                                          (20, {"td_pieces": 1}), (22, {"td_pieces": 7}),
+                                          (22, {"td_pieces": 4096}), (20, {"block_shift": 0}),
+                                          (20, {"block_shift": 17}), (22, {"block_shift": 12}),
+                                          (20, {"td_head": 1}), (22, {"td_head": 16}), (22, {"td_head": 0}),
+                                          (22, {"td_head": 1 << 30}), (22, {"td_head": 4, "block_shift": 14}),
+                                          (22, {"td_head": 4, "td_pieces": 64})])
+def test_bfs_reference_fixtures_larger_scales(scale, tuning, oracle_mod):
+    """An empty tuning keeps the defaults; bin_edges 1 bins every bulk
+    top-down level; td_pieces 0 takes bulk rows in row order, 1 through a
+    one-piece-per-row segment list, 4096 is the largest piece count (levels
+    whose segment list would not fit fall back to row order); block_shift 0
+    turns target blocks off, 12 at SCALE 22 indexes only the first rows, so
+    both indexed and whole-row segments occur."""
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
-    say("scale", scale, "bin_edges", bin_edges)
+    say("scale", scale, "tuning", tuning)
     with graph500_amd.G5Context(0) as c:
-        if bin_edges is not None:
-            c.bfs_set_tuning(bin_edges)
-            assert c.bfs_tuning() == {"bin_edges": bin_edges}
+        if tuning:
+            c.bfs_set_tuning(**tuning)
+            got = c.bfs_tuning()
+            assert all(got[k] == v for k, v in tuning.items())
         c.generate_kronecker(scale, 16, 2, 3)
         t = c.edges_download().view(np.uint32).reshape(-1, 3)
         assert sha(t) == g["tuples_sha256"]
@@ -239,6 +252,10 @@ def test_error_paths():
             c.generate_kronecker(0)  # scale out of range
         with pytest.raises(E):
             c.bfs_set_tuning(-1)  # negative threshold
+        with pytest.raises(E):
+            c.bfs_set_tuning(td_pieces=4097)  # more pieces than allowed
+        with pytest.raises(E):
+            c.bfs_set_tuning(block_shift=5)  # blocks below 2^12 ids
         c.generate_kronecker(8)
         with pytest.raises(E):
             c.construct(7)  # unknown reorder mode
