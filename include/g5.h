@@ -204,11 +204,22 @@ int g5_bfs_stats_get(g5_ctx* ctx, g5_bfs_stats* out);
  * from the final levels, outside any timed region). per_level has
  * G5_MAX_LEVELS entries, index = level. */
 int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
-/* Single-root tuning (no counterpart in the reference; results do not depend
- * on it).  bin_edges: top-down levels with at least this many frontier edges
- * first group their (target, source) pairs by target range so that the level
- * probes stay in L2 (0 = never, the default: up to SCALE 28 the level bytes
- * fit the 256 MB last-level cache and the plain probes are twice as fast). */
+/* Single-root tuning (no counterpart in the reference; levels do not depend
+ * on it, parents may).  All knobs act on bulk top-down levels (more than 2^20
+ * frontier edges).
+ *   bin_edges: levels with at least this many frontier edges first group
+ *     their (target, source) pairs by target range (0 = never, the default:
+ *     twice as slow as the plain level at SCALE 26-28).
+ *   td_pieces: while the search grows, rows are cut into this many pieces by
+ *     position and taken piece index first (0 = row order, the default).
+ *   block_shift: while the search grows, edges are taken by target block of
+ *     2^block_shift ids through a block index of the first rows built at the
+ *     first run (0 = off, the default; 12..31).
+ *   td_head: while the search grows, targets whose degree is at least
+ *     td_head / 16 * (nnz / E)^2 (E = the level's frontier edges; a prefix of
+ *     the degree-ordered rows) are settled bottom-up first and the expansion
+ *     walks only the row suffixes beyond them (default 4; 0 = off; degree
+ *     order only). */
 typedef struct g5_bfs_tuning {
   int64_t bin_edges;
   int64_t td_pieces;
