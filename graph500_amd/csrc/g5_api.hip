@@ -70,6 +70,7 @@ void g5_destroy(g5_ctx* ctx) {
   ctx->ms.release();
   ctx->graph.clear();
   ctx->edges.release();
+  ctx->wstage.release();
   ctx->acc.release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -135,6 +136,29 @@ int g5_edges_download(g5_ctx* ctx, int64_t start, int64_t count, g5_packed_edge*
     G5_HIP(hipMemcpyAsync(out, ctx->edges.p + start, count * sizeof(g5_packed_edge),
                           hipMemcpyDeviceToHost, ctx->stream));
     G5_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int g5_edges_download_weighted(g5_ctx* ctx, int64_t start, int64_t count, int max_weight, uint64_t seed1,
+                               uint64_t seed2, g5_weighted_edge* out) {
+  return guarded(ctx, [&] {
+    G5_CHECK(out || count == 0, "g5_edges_download_weighted: null output");
+    G5_CHECK(start >= 0 && count >= 0 && start + count <= ctx->num_edges,
+             "g5_edges_download_weighted: range outside the edge list");
+    if (!count) return;
+    set_device(ctx);
+    // through a device staging buffer, 8 Mi tuples at a time (the chunk of
+    // EdgeListStorage, generator/graph_generator.hpp:35)
+    const int64_t kChunk = INT64_C(1) << 23;
+    ctx->wstage.alloc((size_t)std::min(count, kChunk));
+    for (int64_t off = 0; off < count; off += kChunk) {
+      const int64_t n = std::min(kChunk, count - off);
+      launch_weighted(ctx->device, ctx->edges.p + start + off, start + off, n, max_weight, seed1, seed2,
+                      ctx->wstage.p, ctx->stream);
+      G5_HIP(hipMemcpyAsync(out + off, ctx->wstage.p, n * sizeof(g5_weighted_edge), hipMemcpyDeviceToHost,
+                            ctx->stream));
+      G5_HIP(hipStreamSynchronize(ctx->stream));
+    }
   });
 }
 

@@ -223,6 +223,7 @@ struct g5_ctx {
   // device edge list
   g5::DevBuf<g5_packed_edge> edges;
   int64_t num_edges = 0;
+  g5::DevBuf<g5_weighted_edge> wstage;  // weighted tuples on their way to the host
   double generate_ms = 0;
   g5::Graph graph;
   g5::BfsState bfs;
@@ -238,6 +239,8 @@ void launch_initial_tree(g5_packed_edge* out, int64_t count, int scale, uint64_t
                          hipStream_t stream);
 void launch_kronecker(int device, g5_packed_edge* out, int64_t start, int64_t count, int scale,
                       uint64_t s1, uint64_t s2, int a_num, int bc_num, hipStream_t stream);
+void launch_weighted(int device, const g5_packed_edge* in, int64_t start_index, int64_t count, int max_weight,
+                     uint64_t s1, uint64_t s2, g5_weighted_edge* out, hipStream_t stream);
 // g5_bfs.hip
 void bfs_prepare(g5_ctx* ctx);
 void bfs_run(g5_ctx* ctx, int64_t root_orig);

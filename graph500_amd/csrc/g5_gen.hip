@@ -287,24 +287,61 @@ void launch_initial_tree(g5_packed_edge* out, int64_t count, int scale, uint64_t
   G5_HIP(hipGetLastError());
 }
 
-// Owns the device copy of the middle-word byte tables.
+// Edge weights (GraphGeneratorBase::generateWeight,
+// generator/graph_generator.hpp:480-506), written with the tuples as
+// WeightedEdge: max_weight <= 0 gives the constant 0xBEEF of the reference's
+// compiled branch; otherwise each lane skips 'base' (the seed already skipped
+// by (30, 46, 0)) by its edge index in the low exponent word and draws once.
+// low_tab: the 8 byte-tables of the low exponent word (entries [0, 8*256)).
+__global__ void __launch_bounds__(256) k_weighted(const g5_packed_edge* __restrict__ in, int64_t start_index,
+                                                  int64_t count, int max_weight, DState base,
+                                                  const DevEntry* __restrict__ low_tab,
+                                                  g5_weighted_edge* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const g5_packed_edge t = in[i];
+    int32_t wgt = 0xBEEF;
+    if (max_weight > 0) {
+      DState st = base;
+      uint64_t e = (uint64_t)(start_index + i);
+      for (int b = 0; e; ++b, e >>= 8) {
+        const uint32_t v = (uint32_t)(e & 0xFF);
+        if (v) dapply(low_tab[b * 256 + v], st);
+      }
+      wgt = (int32_t)(dstep(st) % (uint32_t)max_weight) + 1;
+    }
+    g5_weighted_edge r;
+    r.v0_low = t.v0_low;
+    r.v1_low = t.v1_low;
+    r.high = t.high;
+    r.weight = wgt;
+    out[i] = r;
+  }
+}
+
+// Owns the device copies of the middle-word byte tables (tuples) and the
+// low-word byte tables (weights).
 struct GenTables {
   DevBuf<DevEntry> tab;
-  void ensure() {
-    if (tab.p) return;
+  DevBuf<DevEntry> low;
+  static void upload(DevBuf<DevEntry>& dst, int first_row) {
     const MrgEntry* t = host_mrg_table();
     std::vector<DevEntry> h(8 * 256);
     for (int j = 0; j < 8; ++j)
       for (int v = 0; v < 256; ++v) {
-        const MrgEntry& e = t[(8 + j) * 256 + v];
+        const MrgEntry& e = t[(first_row + j) * 256 + v];
         DevEntry d{};
         d.d = e.d; d.c = e.c; d.b = e.b; d.a = e.a;
         d.s = e.s; d.t = e.t; d.u = e.u; d.v = e.v; d.w = e.w;
         d.sy = hmul(e.s, kY); d.ay = hmul(e.a, kY); d.by = hmul(e.b, kY); d.cy = hmul(e.c, kY);
         h[j * 256 + v] = d;
       }
-    tab.alloc(h.size());
-    G5_HIP(hipMemcpy(tab.p, h.data(), h.size() * sizeof(DevEntry), hipMemcpyHostToDevice));
+    dst.alloc(h.size());
+    G5_HIP(hipMemcpy(dst.p, h.data(), h.size() * sizeof(DevEntry), hipMemcpyHostToDevice));
+  }
+  void ensure() {
+    if (!tab.p) upload(tab, 8);
+    if (!low.p) upload(low, 0);
   }
 };
 
@@ -340,6 +377,21 @@ void launch_kronecker(int device, g5_packed_edge* out, int64_t start, int64_t co
   int64_t blocks = (count + 255) / 256;
   if (blocks > 256 * 256) blocks = 256 * 256;
   hipLaunchKernelGGL(k_kronecker, dim3((unsigned)blocks), dim3(256), 0, stream, out, t.tab.p, p);
+  G5_HIP(hipGetLastError());
+}
+
+void launch_weighted(int device, const g5_packed_edge* in, int64_t start_index, int64_t count, int max_weight,
+                     uint64_t s1, uint64_t s2, g5_weighted_edge* out, hipStream_t stream) {
+  if (count <= 0) return;
+  GenTables& t = tables_for_device(device);
+  MrgState ms;
+  host_make_mrg_seed(s1, s2, &ms);
+  host_mrg_skip(&ms, 30, 46, 0);
+  const DState base{ms.z1, ms.z2, ms.z3, ms.z4, ms.z5};
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_weighted, dim3((unsigned)blocks), dim3(256), 0, stream, in, start_index, count, max_weight,
+                     base, t.low.p, out);
   G5_HIP(hipGetLastError());
 }
 

@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libg5.so")
 
 EDGE_DTYPE = np.dtype([("v0_low", "<u4"), ("v1_low", "<u4"), ("high", "<u4")])
+WEIGHTED_EDGE_DTYPE = np.dtype([("v0_low", "<u4"), ("v1_low", "<u4"), ("high", "<u4"), ("weight", "<i4")])
 MAX_LEVELS = 256
 
 G5_OK = 0
@@ -33,7 +34,8 @@ ERRORS = {-1: "G5_ERR_ARG", -2: "G5_ERR_STATE", -3: "G5_ERR_HIP", -4: "G5_ERR_ME
 EXPORTS = [
     "g5_version", "g5_device_count", "g5_create", "g5_destroy", "g5_last_error",
     "g5_synchronize", "g5_edges_reserve", "g5_edges_append", "g5_edges_count",
-    "g5_edges_download", "g5_edges_release", "g5_generate_kronecker", "g5_generate_graph", "g5_construct",
+    "g5_edges_download", "g5_edges_download_weighted", "g5_edges_release", "g5_generate_kronecker",
+    "g5_generate_graph", "g5_construct",
     "g5_graph_info_get", "g5_has_edge", "g5_pred_size", "g5_find_roots", "g5_graph_download",
     "g5_prepare_bfs", "g5_run_bfs", "g5_get_pred", "g5_get_levels", "g5_teps_edges",
     "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_bfs_set_tuning", "g5_bfs_get_tuning", "g5_validate", "g5_validate_pred", "g5_end_bfs",
@@ -158,6 +160,7 @@ def load_library(path=LIB_PATH):
         "g5_edges_append": ([vp, vp, i64], i32),
         "g5_edges_count": ([vp, ctypes.POINTER(i64)], i32),
         "g5_edges_download": ([vp, i64, i64, vp], i32),
+        "g5_edges_download_weighted": ([vp, i64, i64, i32, u64, u64, vp], i32),
         "g5_edges_release": ([vp], i32),
         "g5_generate_kronecker": ([vp, i32, i32, u64, u64, i32, i32], i32),
         "g5_generate_graph": ([vp, i32, i32, u64, u64, i32, i32, i32], i32),
@@ -267,6 +270,16 @@ class G5Context:
             count = self.edges_count() - start
         out = np.zeros(count, EDGE_DTYPE)
         self._check(self._lib.g5_edges_download(self._h, start, count, _ptr(out)))
+        return out
+
+    def edges_download_weighted(self, start=0, count=None, max_weight=0, seed1=2, seed2=3):
+        """Tuples [start, start+count) as WeightedEdge records (weights made
+        on the GPU): 0xBEEF when max_weight <= 0, else the per-edge draw."""
+        if count is None:
+            count = self.edges_count() - start
+        out = np.zeros(count, WEIGHTED_EDGE_DTYPE)
+        self._check(self._lib.g5_edges_download_weighted(self._h, start, count, max_weight, seed1, seed2,
+                                                         _ptr(out)))
         return out
 
     def edges_release(self):

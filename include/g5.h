@@ -40,6 +40,15 @@ typedef struct g5_packed_edge {
   uint32_t high;
 } g5_packed_edge;
 
+/* WeightedEdge (mpi/primitives.hpp:87-107): the packed tuple and an int
+ * weight, 16 bytes. */
+typedef struct g5_weighted_edge {
+  uint32_t v0_low;
+  uint32_t v1_low;
+  uint32_t high;
+  int32_t weight;
+} g5_weighted_edge;
+
 /* Graph facts after construction (Graph2DCSR members,
  * mpi/graph_constructor.hpp:113-143). */
 typedef struct g5_graph_info {
@@ -139,6 +148,16 @@ int g5_edges_count(g5_ctx* ctx, int64_t* count);
 /* Copies tuples [start, start+count) to the host (EdgeListStorage::read,
  * graph_generator.hpp:120-154). */
 int g5_edges_download(g5_ctx* ctx, int64_t start, int64_t count, g5_packed_edge* out);
+/* Copies tuples [start, start+count) to the host as WeightedEdge tuples
+ * with the weights GraphGeneratorBase::generateWeight gives them
+ * (generator/graph_generator.hpp:480-506; edge index = position in the edge
+ * list): max_weight <= 0 stores 0xBEEF in every tuple, as the reference's
+ * compiled branch does (:499-500); max_weight > 0 draws the per-edge weight
+ * of its disabled branch (:483-488): the seed (seed1, seed2) state skipped by
+ * (30, 46, edge_index), one draw, modulo max_weight, plus 1.  The weights are
+ * made on the GPU. */
+int g5_edges_download_weighted(g5_ctx* ctx, int64_t start, int64_t count, int max_weight, uint64_t seed1,
+                               uint64_t seed2, g5_weighted_edge* out);
 /* Frees the device edge list. */
 int g5_edges_release(g5_ctx* ctx);
 /* generate_graph_spec2010 / RmatGraphGenerator::generateRange

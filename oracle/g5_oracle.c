@@ -214,6 +214,27 @@ void or_initial_tree_edges(int scale, uint64_t s1, uint64_t s2, int64_t start, i
     set_edge(&out[i - start], or_scramble(i + 1, scale, sv[0], sv[1]), or_scramble((i + 1) / 2, scale, sv[0], sv[1]));
 }
 
+/* Edge weights of GraphGeneratorBase::generateWeight
+ * (generator/graph_generator.hpp:480-506).  max_weight <= 0: the reference's
+ * compiled branch, which stores the constant 0xBEEF in every tuple
+ * (:499-500).  max_weight > 0: its per-edge form (:483-488, disabled in the
+ * reference by #if 0): the seed state skipped by (30, 46, edge_index), one
+ * draw, modulo max_weight, plus 1. */
+void or_edge_weights(uint64_t s1, uint64_t s2, int64_t start, int64_t count, int max_weight, int32_t* out) {
+  uint32_t seed[5];
+  or_make_mrg_seed(s1, s2, seed);
+  for (int64_t i = 0; i < count; ++i) {
+    if (max_weight <= 0) {
+      out[i] = 0xBEEF;
+      continue;
+    }
+    uint32_t z[5];
+    memcpy(z, seed, sizeof(z));
+    or_mrg_skip(z, 30, 46, (uint64_t)(start + i));
+    out[i] = (int32_t)(or_mrg_get_uint(z) % (uint32_t)max_weight) + 1;
+  }
+}
+
 void or_generate_edges(int scale, int edgefactor, uint64_t s1, uint64_t s2, int a_num, int bc_num,
                        int64_t start, int64_t end, or_packed_edge* out) {
   (void)edgefactor;

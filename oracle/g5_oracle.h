@@ -73,6 +73,11 @@ void or_generate_edges(int scale, int edgefactor, uint64_t userseed1, uint64_t u
  * (generateInitialEdge, graph_generator.hpp:515-524); the R-MAT tuples then
  * take indices 2^scale - 1 onward (or_generate_edges with that start).
  * Pinned: the reference spec-2012 tuple stream (golden, SCALE 14 and 16). */
+/* Tuple weights of generateWeight, generator/graph_generator.hpp:480-506:
+ * 0xBEEF for max_weight <= 0 (the reference's compiled branch), else the
+ * per-edge draw (skip (30, 46, edge_index)) % max_weight + 1. */
+void or_edge_weights(uint64_t userseed1, uint64_t userseed2, int64_t start, int64_t count, int max_weight,
+                     int32_t* out);
 void or_initial_tree_edges(int scale, uint64_t userseed1, uint64_t userseed2, int64_t start, int64_t end,
                            or_packed_edge* out);
 

@@ -93,6 +93,8 @@ def lib():
                          ctypes.c_void_p, ctypes.POINTER(OrBfsTrace)]
     L.or_initial_tree_edges.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64,
                                         ctypes.c_int64, ctypes.c_void_p]
+    L.or_edge_weights.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                  ctypes.c_void_p]
     L.or_msbfs_bytes.argtypes = [ctypes.POINTER(OrCsr), ctypes.c_int, ctypes.c_void_p,
                                  ctypes.POINTER(ctypes.c_int), ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p]
@@ -165,6 +167,15 @@ def generate_graph_spec2012(scale, edgefactor=16, s1=2, s2=3, a_num=5500, bc_num
     lib().or_initial_tree_edges(scale, s1, s2, 0, n_init, init.ctypes.data_as(ctypes.c_void_p))
     rmat = generate_edges(scale, edgefactor, s1, s2, a_num, bc_num, n_init, n_init + (edgefactor << scale))
     return np.concatenate([init, rmat])
+
+
+def edge_weights(start, count, max_weight=0, s1=2, s2=3):
+    """Weights of tuples [start, start+count) (generateWeight,
+    generator/graph_generator.hpp:480-506): 0xBEEF when max_weight <= 0, else
+    the per-edge draw % max_weight + 1."""
+    out = np.zeros(count, np.int32)
+    lib().or_edge_weights(s1, s2, start, count, max_weight, out.ctypes.data_as(ctypes.c_void_p))
+    return out
 
 
 def edge_endpoints(edges):

@@ -353,6 +353,32 @@ def test_driver_logfile_resume_and_tmpfile(tmp_path):
     assert r.returncode == 1 and "Log file is not match" in r.stdout
 
 
+def test_weighted_tuples_on_gpu(oracle_mod):
+    """g5_edges_download_weighted: WeightedEdge records whose tuple fields
+    equal the packed tuples and whose weights equal the oracle's, in both
+    weight forms, across the 8 Mi-tuple staging chunks (2^19 * 17 tuples)."""
+    import graph500_amd
+    with graph500_amd.G5Context(0) as c:
+        c.generate_kronecker(19, 17, 2, 3)
+        m = c.edges_count()
+        assert m > 1 << 23
+        t = c.edges_download()
+        say("weighted download", m, "tuples")
+        w0 = c.edges_download_weighted(max_weight=0)
+        assert (w0["weight"] == 0xBEEF).all()
+        for f in ("v0_low", "v1_low", "high"):
+            assert np.array_equal(w0[f], t[f])
+        w = c.edges_download_weighted(max_weight=1000)
+        for f in ("v0_low", "v1_low", "high"):
+            assert np.array_equal(w[f], t[f])
+        for lo, hi in [(0, 2000), ((1 << 23) - 1000, (1 << 23) + 1000), (m - 2000, m)]:
+            assert np.array_equal(w["weight"][lo:hi], oracle_mod.edge_weights(lo, hi - lo, 1000)), lo
+        part = c.edges_download_weighted(start=12345, count=777, max_weight=255, seed1=5, seed2=9)
+        assert np.array_equal(part["weight"], oracle_mod.edge_weights(12345, 777, 255, 5, 9))
+        with pytest.raises(graph500_amd.G5Error):
+            c.edges_download_weighted(start=m - 1, count=2)
+
+
 @pytest.mark.parametrize("scale", [14, 16])
 def test_spec2012_graph_on_gpu(scale, oracle_mod):
     """g5_generate_graph with binary-tree initial tuples (the spec-2012

@@ -9,6 +9,9 @@ Known answers used, per oracle function:
   find_roots              reference roots (golden)
   bfs                     reference level-array SHA-256, directions, frontier
                           sizes (golden) + brute-force BFS on small graphs
+  edge_weights            the compiled branch's 0xBEEF constant; the per-edge
+                          draw against the MRG recurrence raised to the skip
+                          exponent with Python integers (math)
   validate                pf_nedge (mpi/utils.hpp:3681) + one planted error per rule
   statistics              direct formulas of mpi/logfile.h:19-44
 """
@@ -336,3 +339,38 @@ def test_spec2012_graph_matches_reference(oracle_mod, scale):
         assert hashlib.sha256(lv.tobytes()).hexdigest() == pr["levels_sha256"], pr["root"]
         assert "".join("T" if d else "B" for d in tr["direction"]) == pr["directions"]
         assert tr["teps_edges"] == pr["nedge"] == len(t)
+
+
+def _mrg_matrix_power(e):
+    """A^e for the MRG step z1' = (107374182 z1 + 104480 z5) mod M, z_k' = z_(k-1)
+    (generator/splittable_mrg.c:170-177), by square-and-multiply on Python ints."""
+    A = [[107374182, 0, 0, 0, 104480], [1, 0, 0, 0, 0], [0, 1, 0, 0, 0], [0, 0, 1, 0, 0], [0, 0, 0, 1, 0]]
+
+    def mul(X, Y):
+        return [[sum(X[i][k] * Y[k][j] for k in range(5)) % M for j in range(5)] for i in range(5)]
+    R = [[int(i == j) for j in range(5)] for i in range(5)]
+    while e:
+        if e & 1:
+            R = mul(R, A)
+        A = mul(A, A)
+        e >>= 1
+    return R
+
+
+def test_edge_weights(oracle_mod):
+    """generateWeight (generator/graph_generator.hpp:480-506): the compiled
+    branch stores 0xBEEF; the per-edge form is one draw from the seed state
+    advanced by 30 * 2^128 + 46 * 2^64 + edge_index steps, mod max_weight, + 1.
+    A dropped skip word, a wrong word order or an off-by-one draw fails the
+    brute-force comparison."""
+    O = oracle_mod
+    assert (O.edge_weights(5, 1000, 0) == 0xBEEF).all()
+    z0 = [int(x) for x in O.make_mrg_seed(2, 3)]
+    for idx in [0, 1, 255, 256, 65537, (1 << 40) + 7]:
+        P = _mrg_matrix_power(30 * 2**128 + 46 * 2**64 + idx)
+        z = [sum(P[i][k] * z0[k] for k in range(5)) % M for i in range(5)]
+        draw = (107374182 * z[0] + 104480 * z[4]) % M
+        for mw in [1, 255, 1000]:
+            assert int(O.edge_weights(idx, 1, mw)[0]) == draw % mw + 1, (idx, mw)
+    w = O.edge_weights(100, 5000, 255)
+    assert w.min() >= 1 and w.max() <= 255 and len(np.unique(w)) > 200
