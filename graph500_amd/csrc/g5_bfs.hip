@@ -804,6 +804,35 @@ __global__ void k_make_levels(const uint32_t* __restrict__ old2new, const uint8_
   }
 }
 
+// Level end: the counters the host decides the next level with (and the
+// level's frontier edge count after a top-down level) are written straight
+// into coherent host memory, the sequence number last, so the host need not
+// wait for a copy and a stream synchronisation.
+enum { P_SEQ = C_NUM, P_NUM = C_NUM + 1 };
+
+__global__ void k_publish(const unsigned long long* __restrict__ counters, const uint64_t* __restrict__ prefix,
+                          int64_t nf, int forward, uint64_t* pub, uint64_t seq) {
+  for (int i = 0; i < C_NUM; ++i) pub[i] = counters[i];
+  if (forward) pub[1] = prefix[nf];
+  __threadfence_system();
+  pub[P_SEQ] = seq;
+}
+
+// Spins until the level's counters are published; a stream that finished or
+// failed without publishing ends the wait with an error.
+static void wait_published(BfsState& b, hipStream_t s) {
+  volatile uint64_t* h = b.h_pub;
+  for (uint64_t spins = 1; h[P_SEQ] != b.pub_seq; ++spins) {
+    if (spins % 1024) continue;
+    const hipError_t e = hipStreamQuery(s);
+    if (e == hipErrorNotReady) continue;
+    G5_HIP(e);
+    G5_CHECK(h[P_SEQ] == b.pub_seq, "g5_run_bfs: level counters were not published");
+  }
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  for (int i = 0; i < C_NUM; ++i) b.h_counters[i] = h[i];
+}
+
 void bfs_prepare(g5_ctx* ctx) {
   Graph& g = ctx->graph;
   G5_CHECK(g.built, "g5_prepare_bfs: no graph (call g5_construct first)");
@@ -821,6 +850,10 @@ void bfs_prepare(g5_ctx* ctx) {
   b.scan_tmp.alloc(scan_tmp_elems(scan_n));
   b.counters.alloc(C_NUM);
   G5_HIP(hipHostMalloc((void**)&b.h_counters, C_NUM * sizeof(uint64_t), hipHostMallocDefault));
+  G5_HIP(hipHostMalloc((void**)&b.h_pub, P_NUM * sizeof(uint64_t), hipHostMallocCoherent | hipHostMallocMapped));
+  for (int i = 0; i < P_NUM; ++i) b.h_pub[i] = 0;
+  G5_HIP(hipHostGetDevicePointer((void**)&b.d_pub, b.h_pub, 0));
+  b.pub_seq = 0;
   G5_HIP(hipEventCreate(&b.ev0));
   G5_HIP(hipEventCreate(&b.ev1));
   b.lev_begin.assign(G5_MAX_LEVELS, nullptr);
@@ -1020,18 +1053,15 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(b.lev_end[lvl], s));
-    G5_HIP(hipMemcpyAsync(b.h_counters, b.counters.p, C_NUM * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    if (forward) G5_HIP(hipMemcpyAsync(b.h_counters + 1, b.prefix.p + nf, 8, hipMemcpyDeviceToHost, s));
-    G5_HIP(hipStreamSynchronize(s));
+    ++b.pub_seq;
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, cnt, b.prefix.p, nf, forward ? 1 : 0, b.d_pub,
+                       b.pub_seq);
+    G5_HIP(hipGetLastError());
+    wait_published(b, s);
     const int64_t nq = (int64_t)b.h_counters[0];
     const bool bulk = forward && b.h_counters[1] > kTdSmall;
     // a small top-down level summed its winners' degrees
     e_next = (forward && !bulk) ? (int64_t)b.h_counters[C_NEXT_E] : -1;
-    {
-      float lms = 0;
-      G5_HIP(hipEventElapsedTime(&lms, b.lev_begin[lvl], b.lev_end[lvl]));
-      st.level_ms[lvl] = lms;
-    }
     st.direction[lvl] = forward ? 1 : 0;
     st.nq[lvl] = nq;
     st.num_levels = lvl + 1;
@@ -1066,6 +1096,11 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
   float ms = 0;
   G5_HIP(hipEventElapsedTime(&ms, b.ev0, b.ev1));
   st.device_ms = ms;
+  for (int l = 1; l <= lvl && root_active; ++l) {
+    float lms = 0;
+    G5_HIP(hipEventElapsedTime(&lms, b.lev_begin[l], b.lev_end[l]));
+    st.level_ms[l] = lms;
+  }
   st.reached = visited;
   b.have_result = true;
 }

@@ -57,6 +57,9 @@ struct BfsState {
   DevBuf<uint64_t> scan_tmp;
   DevBuf<uint64_t> counters;      // device counters (see g5_bfs.hip)
   uint64_t* h_counters = nullptr; // pinned host mirror
+  uint64_t* h_pub = nullptr;      // coherent host memory the level counters are published to
+  uint64_t* d_pub = nullptr;      // its device address
+  uint64_t pub_seq = 0;
   DevBuf<int64_t> pred_out;       // n, built on demand by get_pred
   DevBuf<uint8_t> level_out;      // n, built on demand by get_levels
   int64_t bin_edges = 0;  // top-down levels with at least this many edges are binned (0: never)
@@ -101,6 +104,8 @@ struct BfsState {
     hub_built_shift = 0;
     if (h_counters) (void)hipHostFree(h_counters);
     h_counters = nullptr;
+    if (h_pub) (void)hipHostFree(h_pub);
+    h_pub = d_pub = nullptr;
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     ev0 = ev1 = nullptr;
