@@ -46,6 +46,8 @@ Other modes:
   --rocprof-dispatches DB KERNEL
                           per-dispatch durations or counters of one kernel
   --tuning JSON           run the batch with this tuning instead of autotuning
+  --single-root-only N    only the single-root leg, over the first N roots (a
+                          short run to profile under rocprofv3)
 """
 import argparse
 import json
@@ -317,6 +319,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--rocprof-summary", metavar="DB")
     ap.add_argument("--rocprof-dispatches", nargs=2, metavar=("DB", "KERNEL"))
+    ap.add_argument("--single-root-only", type=int, metavar="N")
     ap.add_argument("--tuning", type=json.loads, default=None,
                     help="batch tuning as JSON (skips the autotuner), e.g. a previous run's batch.tuning")
     a = ap.parse_args()
@@ -345,6 +348,12 @@ def main():
     log("graph: n_active %d nnz %d gen %.1f ms construct %.1f ms" %
         (info["num_active"], info["nnz"], info["generate_ms"], info["construct_ms"]))
     roots = ctx.find_roots(64)
+
+    if a.single_root_only:
+        single = single_root_leg(ctx, roots[:a.single_root_only], a.warmup)
+        ctx.close()
+        print(json.dumps({"mode": "single-root only", "scale": a.scale, "single_root": single}))
+        return 0
 
     single = None
     if not a.no_single_root:
