@@ -941,12 +941,12 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
       // past the bottom-up phase most targets are visited: a bulk level then
       // tests the visited bitmap before the level array
       const int probe_vis = (double)visited > na / 4 ? 1 : 0;
-      // while the search grows, a bulk level takes its rows block by block
-      // (exact target blocks) or piece by piece (positional pieces)
+      // the level's frontier edge count when the previous level summed it
       const bool known_small = e_next >= 0 && (uint64_t)e_next <= kTdSmall;
       const bool known_bulk = e_next >= 0 && (uint64_t)e_next > kTdSmall;
+      // while the search grows, a bulk level can take its rows block by
+      // block (exact target blocks) or piece by piece (positional pieces)
       const bool blocked = b.hub_shift > 0 && !probe_vis && nf * b.hub_pieces <= kSegMax;
-      // rows [0, head) settled bottom-up (a multiple of 64 up to the bitmap's end)
       // a growing bulk level settles its highest-degree targets bottom-up
       // first (degree order only)
       bool head = b.td_head > 0 && !probe_vis && g.reorder_mode == 2;
