@@ -17,6 +17,12 @@
 //              chunk); hub levels use no atomics: the visited bitmap is a
 //              read-only snapshot, new targets get level/parent with benign
 //              races, and one pass over the level array gives the frontier.
+//              While the search grows, a hub level first settles its
+//              highest-degree targets bottom-up (k_td_head) and expands only
+//              the row suffixes beyond them (k_td_expand_segs).
+//   host loop  one round trip per level: a one-lane kernel publishes the
+//              level's counters to coherent host memory, where the host
+//              reads them for the alpha/beta rule.
 //   queues     a top-down frontier that comes from a bitmap is rebuilt by an
 //              exclusive scan of per-word popcounts (ascending vertex order).
 //   bottom-up  k_bu: one wave per 64-vertex bitmap word, one lane per vertex.
