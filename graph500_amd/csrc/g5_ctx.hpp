@@ -168,6 +168,7 @@ struct MsState {
   std::vector<std::unique_ptr<DevBuf<uint64_t>>> fr;  // frontier words per level
   DevBuf<const uint64_t*> frp;  // device copy of the fr pointers
   DevBuf<uint32_t> queue;
+  DevBuf<uint32_t> touch;    // vertices a sparse push level touched
   DevBuf<uint64_t> prefix;
   DevBuf<uint32_t> chunk_rows;
   DevBuf<uint64_t> scan_tmp;
@@ -183,6 +184,7 @@ struct MsState {
     fr.clear();
     frp.release();
     queue.release();
+    touch.release();
     prefix.release();
     chunk_rows.release();
     scan_tmp.release();

@@ -167,6 +167,45 @@ __device__ __forceinline__ void ms_finish_body(uint64_t* __restrict__ seen, uint
 }
 
 
+// The same fold over a list of the vertices a sparse push touched (each once,
+// in any order): 64 list entries per wave and step; nz_next (cleared
+// beforehand) gets its bits by atomicOr since a word's vertices may sit in
+// different waves.
+template <class Epi = NoEpilogue>
+__device__ __forceinline__ void ms_finish_list_body(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
+                                                    uint64_t* __restrict__ nz_next,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ list, uint64_t n,
+                                                    unsigned long long* __restrict__ c, unsigned bid, unsigned nblk,
+                                                    const Epi& epi = Epi()) {
+  uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
+  const int lane = threadIdx.x & 63;
+  const uint64_t nwaves = (uint64_t)nblk * (kBlock / 64);
+  epi.begin();
+  for (uint64_t base = ((uint64_t)bid * (kBlock / 64) + (threadIdx.x >> 6)) * 64; base < n; base += nwaves * 64) {
+    const uint64_t i = base + lane;
+    const bool inb = i < n;
+    const int64_t v = inb ? (int64_t)list[i] : 0;
+    uint64_t nw = 0;
+    if (inb) {
+      const uint64_t s0 = seen[v];
+      nw = next[v] & ~s0;
+      next[v] = nw;
+      if (nw) {
+        seen[v] = s0 | nw;
+        live |= nw;
+        pairs += __popcll(nw);
+        ++nz;
+        fdeg += offsets[v + 1] - offsets[v];
+        atomicOr((unsigned long long*)(nz_next + (v >> 6)), (unsigned long long)(UINT64_C(1) << (v & 63)));
+      }
+    }
+    epi(v, inb, nw);
+  }
+  epi.end();
+  ms_block_counters(live, pairs, nz, fdeg, c);
+}
+
 // Pull step: one lane per vertex, the wave helps with long rows.
 // How the pull reads a neighbour's frontier word:
 //   kFrontPlain    front[w] (8-byte random read into an n_rows-word array)

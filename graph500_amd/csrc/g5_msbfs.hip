@@ -96,6 +96,58 @@ __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__
   }
 }
 
+// k_ms_push that also lists the vertices it touches: the one atomicOr that
+// finds a target's word still zero (next was cleared for the level) appends
+// the target, staged in LDS and appended with one global atomic per chunk.
+__global__ void __launch_bounds__(kBlock) k_ms_push_touch(const uint32_t* __restrict__ queue, int64_t nf,
+                                                          const uint64_t* __restrict__ prefix,
+                                                          const uint32_t* __restrict__ chunk_rows,
+                                                          const uint64_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ cols,
+                                                          const uint64_t* __restrict__ front, uint64_t* next,
+                                                          uint32_t* __restrict__ list,
+                                                          unsigned long long* __restrict__ list_count) {
+  __shared__ ChunkRows cr;
+  __shared__ uint32_t s_q[kEpb];
+  __shared__ uint32_t s_qn;
+  __shared__ uint64_t s_qbase;
+  const uint64_t E = prefix[nf];
+  for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
+    const uint64_t e0 = chunk * kEpb;
+    const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
+    if (threadIdx.x == 0) s_qn = 0;
+    const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
+    walk_strided(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
+      const uint64_t m = front[u];
+      if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED)) {
+        const uint64_t old = atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
+        if (old == 0) s_q[atomicAdd(&s_qn, 1u)] = w;
+      }
+    });
+    __syncthreads();
+    if (threadIdx.x == 0) s_qbase = s_qn ? atomicAdd(list_count, (unsigned long long)s_qn) : 0;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < s_qn; j += kBlock) list[s_qbase + j] = s_q[j];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ms_finish_list(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
+                                                           uint64_t* __restrict__ nz_next,
+                                                           const uint64_t* __restrict__ offsets,
+                                                           const uint32_t* __restrict__ list,
+                                                           const unsigned long long* __restrict__ list_count,
+                                                           unsigned long long* __restrict__ c, LaneEpi epi_in,
+                                                           int lane_stats) {
+  const uint64_t n = *list_count;
+  if (lane_stats) {
+    const LaneEpi epi = epi_in;  // a copy in registers: the epilogue accumulates per thread
+    ms_finish_list_body(seen, next, nz_next, offsets, list, n, c, blockIdx.x, gridDim.x, epi);
+  } else {
+    ms_finish_list_body(seen, next, nz_next, offsets, list, n, c, blockIdx.x, gridDim.x);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_ms_finish(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
                                                       uint64_t* __restrict__ nz_next,
                                                       const uint64_t* __restrict__ offsets, int64_t rows,
@@ -195,6 +247,10 @@ __global__ void __launch_bounds__(kBlock) k_ms_sweep(const uint32_t* __restrict_
 // counts in on the top counter, and the last group flips the generation that
 // everyone waits on; this keeps the atomics on any one address to a few tens
 // per barrier instead of one per workgroup.
+// A push level lists the vertices it touches when its frontier edges times
+// this are fewer than the rows (the touched vertices are at most the edges).
+constexpr double kTouchBelow = 8.0;
+
 constexpr unsigned kBarGroup = 16;
 constexpr unsigned kBarMaxGroups = 1024;
 struct GridBarrier {
@@ -552,10 +608,28 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
         hipLaunchKernelGGL(k_chunk_rows<kEpb>, dim3(grid_of((int64_t)(2 * nchunks), kBlock, 65536)),
                            dim3(kBlock), 0, s, m.prefix.p, nf, nchunks, m.chunk_rows.p);
       G5_HIP(hipMemsetAsync(next, 0, rows * 8, s));
-      hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)fdeg, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p,
-                         nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, next);
+      // a push whose frontier edges are few next to the rows lists what it
+      // touches, and only those vertices are folded
+      const uint64_t touch_cap = std::min<uint64_t>(fdeg, (uint64_t)rows);
+      const bool touch = (double)fdeg * kTouchBelow < (double)rows;
+      if (touch) {
+        m.touch.alloc(std::max<uint64_t>(touch_cap, 1));
+        unsigned long long* touch_count = cnt + M_NUM;  // cleared with the counters
+        G5_HIP(hipMemsetAsync(nz_next, 0, words * 8, s));
+        hipLaunchKernelGGL(k_ms_push_touch, dim3(grid_of((int64_t)fdeg, kEpb, 2048)), dim3(kBlock), 0, s,
+                           m.queue.p, nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, next, m.touch.p,
+                           touch_count);
+        hipLaunchKernelGGL(k_ms_finish_list, dim3(grid_of((int64_t)touch_cap, kBlock, 8192)), dim3(kBlock), 0, s,
+                           m.seen.p, next, nz_next, g.offsets.p, m.touch.p, (const unsigned long long*)touch_count,
+                           cnt, lepi, lane_stats ? 1 : 0);
+      } else {
+        hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)fdeg, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p,
+                           nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, next);
+      }
       G5_HIP(hipGetLastError());
-      if (lane_stats)
+      if (touch) {
+        // folded above
+      } else if (lane_stats)
         hipLaunchKernelGGL(k_ms_finish_lanes, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, m.seen.p,
                            next, nz_next, g.offsets.p, rows, cnt, lepi);
       else
