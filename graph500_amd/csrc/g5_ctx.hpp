@@ -169,6 +169,9 @@ struct MsState {
   DevBuf<const uint64_t*> frp;  // device copy of the fr pointers
   DevBuf<uint32_t> queue;
   DevBuf<uint32_t> touch;    // vertices a sparse push level touched
+  DevBuf<uint64_t> miss;     // one bit per vertex still missing a live root (tail pull levels)
+  DevBuf<uint32_t> ulist;    // the same vertices as a list
+  DevBuf<uint64_t> ulist_count;
   DevBuf<uint64_t> prefix;
   DevBuf<uint32_t> chunk_rows;
   DevBuf<uint64_t> scan_tmp;
@@ -185,6 +188,9 @@ struct MsState {
     frp.release();
     queue.release();
     touch.release();
+    miss.release();
+    ulist.release();
+    ulist_count.release();
     prefix.release();
     chunk_rows.release();
     scan_tmp.release();

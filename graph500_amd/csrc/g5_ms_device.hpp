@@ -58,6 +58,30 @@ struct UnionCount {
   const uint64_t* nz;
   __device__ uint64_t operator()(size_t w) const { return (uint64_t)__popcll(nz[w]); }
 };
+// The same over words [0, words) with one more element whose prefix, the
+// total, goes to *count.
+struct BitsCountN {
+  const uint64_t* bits;
+  int64_t words;
+  __device__ uint64_t operator()(size_t w) const { return (int64_t)w < words ? (uint64_t)__popcll(bits[w]) : 0; }
+};
+struct BitsEmitN {
+  const uint64_t* bits;
+  int64_t words;
+  uint32_t* list;
+  uint64_t* count;
+  __device__ void operator()(size_t w, uint64_t pre) const {
+    if ((int64_t)w == words) {
+      *count = pre;
+      return;
+    }
+    uint64_t x = bits[w];
+    while (x) {
+      list[pre++] = (uint32_t)(w * 64 + __builtin_ctzll(x));
+      x &= x - 1;
+    }
+  }
+};
 struct UnionEmit {
   const uint64_t* nz;
   uint32_t* queue;
@@ -263,6 +287,70 @@ struct CompactEmit {
   }
 };
 
+// The probes of one pull step for each lane's vertex v (need: the lanes v
+// still has to find; 0 for a lane with nothing to do): its first neighbour,
+// then its short row lane-sequentially or its long row by the whole wave.
+// Every lane of the wave calls it.  Returns the OR of the frontier words
+// probed (at least the needed lanes when they are all present).
+template <int kU, int kW>
+__device__ __forceinline__ uint64_t ms_pull_probe(const FrontView& fv, int64_t v, uint64_t need,
+                                                  const uint32_t* __restrict__ first_nbr,
+                                                  const uint64_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ cols) {
+  const int lane = threadIdx.x & 63;
+  uint64_t acc = 0;
+  bool more = false;
+  uint64_t rb = 0, re = 0;
+  if (need) {
+    const uint32_t f = first_nbr[v];
+    if (f != kMsNoNbr) {
+      acc = front_of(fv, f);
+      if ((acc & need) != need) {
+        rb = offsets[v] + 1;
+        re = offsets[v + 1];
+        more = rb < re;
+      }
+    }
+  }
+  if (more && re - rb <= (uint64_t)kMsShortRow) {
+    // kU independent probes per step (entries past the row repeat the first)
+    for (uint64_t k = rb; k < re; k += kU) {
+      uint32_t w[kU];
+      w[0] = cols[k];
+#pragma unroll
+      for (int j = 1; j < kU; ++j) w[j] = k + j < re ? cols[k + j] : w[0];
+      uint64_t x = 0;
+#pragma unroll
+      for (int j = 0; j < kU; ++j) x |= front_of(fv, w[j]);
+      acc |= x;
+      if ((acc & need) == need) break;
+    }
+    more = false;
+  }
+  uint64_t lm = __ballot(more);
+  while (lm) {
+    const int src = __ffsll((unsigned long long)lm) - 1;
+    lm &= lm - 1;
+    const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
+    const uint64_t nd = __shfl(need, src, 64);
+    uint64_t a = __shfl(acc, src, 64);
+    // 64 kW entries per step, kW independent probes per lane
+    for (uint64_t base = b; base < e; base += 64 * kW) {
+      uint32_t w[kW];
+#pragma unroll
+      for (int j = 0; j < kW; ++j) w[j] = base + lane + 64 * j < e ? cols[base + lane + 64 * j] : 0;
+      uint64_t x = 0;
+#pragma unroll
+      for (int j = 0; j < kW; ++j)
+        if (base + lane + 64 * j < e) x |= front_of(fv, w[j]);
+      a |= wave_or(x);
+      if ((a & nd) == nd) break;
+    }
+    if (lane == src) acc = a;
+  }
+  return acc;
+}
+
 // kU: independent probes per step of a lane's short row; kW: entries per lane
 // and step of the wave's scan of a long row (kW x 64 per step).
 template <class Epi = NoEpilogue, int kU = 4, int kW = 2>
@@ -272,7 +360,8 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
                                              const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols,
                                              int64_t rows, uint64_t live_in, unsigned long long* __restrict__ c,
                                              unsigned bid, unsigned nblk, uint64_t keep = ~UINT64_C(0),
-                                             const Epi& epi = Epi(), bool merge = false) {
+                                             const Epi& epi = Epi(), bool merge = false,
+                                             uint64_t* __restrict__ miss = nullptr) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
   uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
@@ -286,58 +375,14 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
     const uint64_t need = ~s & live_in;
     // lanes pushed into next earlier in this level (merge): new ones join nw
     const uint64_t pushed = (merge && inb) ? next[v] & ~s : 0;
-    uint64_t acc = 0;
-    bool more = false;
-    uint64_t rb = 0, re = 0;
-    if (need) {
-      const uint32_t f = first_nbr[v];
-      if (f != kMsNoNbr) {
-        acc = front_of(fv, f);
-        if ((acc & need) != need) {
-          rb = offsets[v] + 1;
-          re = offsets[v + 1];
-          more = rb < re;
-        }
-      }
-    }
-    if (more && re - rb <= (uint64_t)kMsShortRow) {
-      // kU independent probes per step (entries past the row repeat the first)
-      for (uint64_t k = rb; k < re; k += kU) {
-        uint32_t w[kU];
-        w[0] = cols[k];
-#pragma unroll
-        for (int j = 1; j < kU; ++j) w[j] = k + j < re ? cols[k + j] : w[0];
-        uint64_t x = 0;
-#pragma unroll
-        for (int j = 0; j < kU; ++j) x |= front_of(fv, w[j]);
-        acc |= x;
-        if ((acc & need) == need) break;
-      }
-      more = false;
-    }
-    uint64_t lm = __ballot(more);
-    while (lm) {
-      const int src = __ffsll((unsigned long long)lm) - 1;
-      lm &= lm - 1;
-      const uint64_t b = __shfl(rb, src, 64), e = __shfl(re, src, 64);
-      const uint64_t nd = __shfl(need, src, 64);
-      uint64_t a = __shfl(acc, src, 64);
-      // 64 kW entries per step, kW independent probes per lane
-      for (uint64_t base = b; base < e; base += 64 * kW) {
-        uint32_t w[kW];
-#pragma unroll
-        for (int j = 0; j < kW; ++j) w[j] = base + lane + 64 * j < e ? cols[base + lane + 64 * j] : 0;
-        uint64_t x = 0;
-#pragma unroll
-        for (int j = 0; j < kW; ++j)
-          if (base + lane + 64 * j < e) x |= front_of(fv, w[j]);
-        a |= wave_or(x);
-        if ((a & nd) == nd) break;
-      }
-      if (lane == src) acc = a;
-    }
+    const uint64_t acc = ms_pull_probe<kU, kW>(fv, v, need, first_nbr, offsets, cols);
     const uint64_t nw = (acc & need) | pushed;
     epi(v, inb, nw);
+    // one bit per vertex still missing a live lane, for the list-driven tail levels
+    if (miss) {
+      const uint64_t mb = __ballot(inb && (live_in & ~(s | nw)) != 0);
+      if (lane == 0) miss[wd] = mb;
+    }
     const uint64_t b = __ballot(nw != 0);
     if (lane == 0) nz_next[wd] = b;
     if (inb) {
@@ -352,6 +397,45 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
     }
   }
   epi.end();
+  ms_block_counters(live, pairs, nz, fdeg, c);
+}
+
+
+// A pull step over a list of the vertices that may still miss a live lane
+// (the tail levels of a batch): the level's frontier array, nz_next and miss
+// were cleared beforehand, so only listed vertices with new lanes are
+// written, and the ones still missing a live lane set their bit in miss.
+template <int kU = 4, int kW = 2>
+__device__ __forceinline__ void ms_pull_list_body(uint64_t* __restrict__ seen, const FrontView fv,
+                                                  uint64_t* __restrict__ next, uint64_t* __restrict__ nz_next,
+                                                  const uint32_t* __restrict__ first_nbr,
+                                                  const uint64_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ cols, const uint32_t* __restrict__ list,
+                                                  uint64_t n, uint64_t live_in, uint64_t* __restrict__ miss,
+                                                  unsigned long long* __restrict__ c, unsigned bid, unsigned nblk) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nwaves = (uint64_t)nblk * (kBlock / 64);
+  uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
+  for (uint64_t base = ((uint64_t)bid * (kBlock / 64) + (threadIdx.x >> 6)) * 64; base < n; base += nwaves * 64) {
+    const uint64_t i = base + lane;
+    const bool inb = i < n;
+    const int64_t v = inb ? (int64_t)list[i] : 0;
+    const uint64_t s = inb ? seen[v] : ~UINT64_C(0);
+    const uint64_t need = ~s & live_in;
+    const uint64_t acc = ms_pull_probe<kU, kW>(fv, v, need, first_nbr, offsets, cols);
+    const uint64_t nw = acc & need;
+    if (nw) {
+      next[v] = nw;
+      seen[v] = s | nw;
+      live |= nw;
+      pairs += __popcll(nw);
+      ++nz;
+      fdeg += offsets[v + 1] - offsets[v];
+      atomicOr((unsigned long long*)(nz_next + (v >> 6)), (unsigned long long)(UINT64_C(1) << (v & 63)));
+    }
+    if (inb && (live_in & ~(s | nw)) != 0)
+      atomicOr((unsigned long long*)(miss + (v >> 6)), (unsigned long long)(UINT64_C(1) << (v & 63)));
+  }
   ms_block_counters(live, pairs, nz, fdeg, c);
 }
 

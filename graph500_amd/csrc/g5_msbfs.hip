@@ -164,10 +164,25 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
                                                     const uint32_t* __restrict__ cols, int64_t rows,
                                                     uint64_t live_in, int mode, const uint64_t* __restrict__ pack,
                                                     const uint64_t* __restrict__ compact, uint32_t filter_from,
-                                                    unsigned long long* __restrict__ c) {
+                                                    unsigned long long* __restrict__ c, uint64_t* __restrict__ miss) {
   const FrontView fv{front, nz_front, pack, compact, mode, filter_from};
   ms_pull_body<NoEpilogue, kU, kW>(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x,
-                                   gridDim.x);
+                                   gridDim.x, ~UINT64_C(0), NoEpilogue(), false, miss);
+}
+
+// The pull of a tail level over the listed vertices (see ms_pull_list_body).
+__global__ void __launch_bounds__(kBlock) k_ms_pull_list(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
+                                                         uint64_t* __restrict__ next, uint64_t* __restrict__ nz_next,
+                                                         const uint32_t* __restrict__ first_nbr,
+                                                         const uint64_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ cols,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint64_t* __restrict__ list_count, uint64_t live_in,
+                                                         uint64_t* __restrict__ miss,
+                                                         unsigned long long* __restrict__ c) {
+  const FrontView fv{front, nullptr, nullptr, nullptr, kFrontPlain, 0};
+  ms_pull_list_body(seen, fv, next, nz_next, first_nbr, offsets, cols, list, *list_count, live_in, miss, c,
+                    blockIdx.x, gridDim.x);
 }
 
 // --- per-root directions (lane_rule 1) ---
@@ -247,6 +262,10 @@ __global__ void __launch_bounds__(kBlock) k_ms_sweep(const uint32_t* __restrict_
 // counts in on the top counter, and the last group flips the generation that
 // everyone waits on; this keeps the atomics on any one address to a few tens
 // per barrier instead of one per workgroup.
+// Tail pull levels go list-driven once fewer than rows / kListBelow (root,
+// vertex) pairs are unreached.
+constexpr double kListBelow = 16.0;
+
 // A push level lists the vertices it touches when its frontier edges times
 // this are fewer than the rows (the touched vertices are at most the edges).
 constexpr double kTouchBelow = 8.0;
@@ -545,9 +564,11 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
   uint64_t pulled = 0;  // roots that have pulled: they pull from then on
   const uint64_t* h_lane = m.h_counters + M_NUM + 1;
   unsigned* d_list_count = (unsigned*)(cnt + M_NUM);
+  bool have_miss = false;  // m.miss holds the vertices still missing a live root
   int l = l0 - 1;
   while (live) {
     ++l;
+    bool list_level = false;  // this level writes m.miss
     G5_CHECK(l < G5_MAX_LEVELS - 1, "g5_run_bfs_batch: more than 254 levels (uint8 level arrays)");
     uint64_t* next = ms_level(m, l, rows);
     const uint64_t* front = m.fr[l - 1]->p;  // after ms_level: the vector may have grown
@@ -663,9 +684,33 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
         case 4: kp = k_ms_pull<2, 1>; break;
         default: break;
       }
-      hipLaunchKernelGGL(kp, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s, m.seen.p, front,
-                         nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, live, mode, m.pack.p,
-                         m.compact.p, filter_from, cnt);
+      // Tail levels: once fewer than rows / kListBelow (root, vertex) pairs
+      // are unreached, a pull also marks the vertices still missing a live
+      // root (one bit per vertex in m.miss, written 64 at a time), and the
+      // next pull takes only those, listed by a scan of the bits (after
+      // clearing its frontier array, which must hold zeros elsewhere).
+      const double unreached = std::max(0.0, total_pairs - reached);
+      if (have_miss && mode == kFrontPlain) {
+        const uint64_t ub = (uint64_t)std::min<double>((double)rows, unreached);  // list length bound
+        m.ulist.alloc(std::max<uint64_t>(ub, 1));
+        m.ulist_count.alloc(1);
+        device_scan(BitsCountN{m.miss.p, words}, BitsEmitN{m.miss.p, words, m.ulist.p, m.ulist_count.p},
+                    (size_t)words + 1, m.scan_tmp.p, s);
+        G5_HIP(hipMemsetAsync(next, 0, rows * 8, s));
+        G5_HIP(hipMemsetAsync(nz_next, 0, words * 8, s));
+        G5_HIP(hipMemsetAsync(m.miss.p, 0, words * 8, s));
+        hipLaunchKernelGGL(k_ms_pull_list, dim3(grid_of((int64_t)ub, kBlock, m.pull_grid)), dim3(kBlock), 0, s,
+                           m.seen.p, front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, m.ulist.p,
+                           m.ulist_count.p, live, m.miss.p, cnt);
+        list_level = true;
+      } else {
+        const bool mark = unreached * kListBelow < (double)rows;
+        if (mark) m.miss.alloc(words);
+        hipLaunchKernelGGL(kp, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s, m.seen.p, front,
+                           nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, live, mode,
+                           m.pack.p, m.compact.p, filter_from, cnt, mark ? m.miss.p : nullptr);
+        list_level = mark;
+      }
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(m.lev_end[l], s));
@@ -681,6 +726,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
         lane_fd[r] = (double)h_lane[64 + r];
         lane_vd[r] += (double)h_lane[64 + r];
       }
+    have_miss = list_level;  // the next pull level can take the listed vertices only
     st.pairs[l] = (int64_t)m.h_counters[M_PAIRS];
     reached += (double)m.h_counters[M_PAIRS];
     st.num_levels = l + 1;
