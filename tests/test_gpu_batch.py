@@ -191,8 +191,11 @@ def test_batch_partial_and_odd_roots(oracle_mod, mode, rule):
                 assert np.array_equal(c.batch_levels(i), lv), (i, r)
 
 
-@pytest.mark.parametrize("scale,sweep,rule", [(20, 0.25, 0), (20, 2.0, 0), (20, 0.0, 1)])
-def test_batch_matches_single_root_gpu(scale, sweep, rule):
+@pytest.mark.parametrize("scale,sweep,rule,beta", [(20, 0.25, 0, 50.0), (20, 2.0, 0, 50.0), (20, 0.0, 1, 50.0),
+                                                   (20, 0.0, 0, 0.0), (20, 0.0, 1, 0.0)])
+def test_batch_matches_single_root_gpu(scale, sweep, rule, beta):
+    """beta 0 lets small tail frontiers push again after the list-driven
+    tail pulls (whose frontier words and nonzero bits the push then reads)."""
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
     with graph500_amd.G5Context(0) as c:
@@ -200,8 +203,8 @@ def test_batch_matches_single_root_gpu(scale, sweep, rule):
         c.construct(2, release_edges=True)
         roots = c.find_roots(64)
         c.batch_prepare()
-        c.batch_set_tuning(sweep_below=sweep, lane_rule=rule, persistent=0)
-        say("batch s%d sweep %g lane_rule %d" % (scale, sweep, rule))
+        c.batch_set_tuning(sweep_below=sweep, lane_rule=rule, persistent=0, beta=beta)
+        say("batch s%d sweep %g lane_rule %d beta %g" % (scale, sweep, rule, beta))
         c.run_bfs_batch(roots)
         teps = c.batch_teps_edges()
         for i, pr in enumerate(g["per_root"]):
