@@ -114,6 +114,7 @@ struct NoEpilogue {
 struct LaneEpi {
   unsigned long long* lane_nq;
   const uint64_t* offsets;
+  uint64_t stat;  // the lanes whose statistics are kept (the ones still choosing a direction)
   mutable uint64_t n_acc = 0, d_acc = 0;
   __device__ unsigned long long* lds() const {
     __shared__ unsigned long long s_lane[128];
@@ -123,7 +124,8 @@ struct LaneEpi {
     if (threadIdx.x < 128) lds()[threadIdx.x] = 0;
     __syncthreads();
   }
-  __device__ void operator()(int64_t v, bool inb, uint64_t nw) const {
+  __device__ void operator()(int64_t v, bool inb, uint64_t nw_all) const {
+    const uint64_t nw = nw_all & stat;
     const uint64_t deg = (inb && nw) ? offsets[v + 1] - offsets[v] : 0;
     const int me = threadIdx.x & 63;
     uint64_t m = __ballot(inb && nw != 0);

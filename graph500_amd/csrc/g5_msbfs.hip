@@ -600,7 +600,8 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     m.level_pull[l] = push ? 0 : (lanes_mode ? bu : live);
     G5_HIP(hipMemsetAsync(m.counters.p, 0, (M_NUM + 1 + 128) * 8, s));
     G5_HIP(hipEventRecord(m.lev_begin[l], s));
-    const LaneEpi lepi{cnt + M_NUM + 1, g.offsets.p};
+    // statistics for the roots still choosing (a root that pulled keeps pulling)
+    const LaneEpi lepi{cnt + M_NUM + 1, g.offsets.p, lanes_mode ? td : live};
     if (lanes_mode && td && bu) {
       // the pushing roots over the CSR chunks holding one of their frontier
       // vertices, then the pull of the others, merging the pushed lanes
