@@ -964,7 +964,10 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
       }
       unsigned long long* head_words = head ? cnt + C_HEAD : nullptr;
       const bool pieces = blocked || (b.td_pieces > 0 && !probe_vis && nf * b.td_pieces <= kSegMax);
-      // with a head and row order, the expansion walks row suffixes
+      // with a head and row order, the expansion walks row suffixes (one
+      // segment per frontier row: the segment count field bounds nf)
+      if (head && !pieces && nf > kSegMax) head = false;
+      head_words = head ? cnt + C_HEAD : nullptr;
       const bool suffix = head && !pieces;
       const bool binned = b.bin_edges > 0 && E_known > kTdSmall && E_known >= (uint64_t)b.bin_edges;
       // Without binning no launch waits for E on the host: chunk rows and the
