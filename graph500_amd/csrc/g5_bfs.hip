@@ -855,6 +855,16 @@ void bfs_prepare(g5_ctx* ctx) {
   const size_t scan_n = (size_t)rows + 1;
   b.scan_tmp.alloc(scan_tmp_elems(scan_n));
   b.counters.alloc(C_NUM);
+  // the segment list of a head level: one segment per frontier row, and a
+  // growing top-down level has at most n_active / 2000 frontier vertices
+  // (one more switches to bottom-up), so no run allocates it mid-search
+  const size_t seg_n = (size_t)(g.n_active / 2000) + 2;
+  b.seg_pre.alloc(seg_n);
+  b.seg_base.alloc(seg_n);
+  b.seg_src.alloc(seg_n);
+  b.seg_start.alloc(seg_n);
+  b.seg_count.alloc(1);
+  b.seg_scan_tmp.alloc(scan_tmp_elems(seg_n));
   G5_HIP(hipHostMalloc((void**)&b.h_counters, C_NUM * sizeof(uint64_t), hipHostMallocDefault));
   G5_HIP(hipHostMalloc((void**)&b.h_pub, P_NUM * sizeof(uint64_t), hipHostMallocCoherent | hipHostMallocMapped));
   for (int i = 0; i < P_NUM; ++i) b.h_pub[i] = 0;
