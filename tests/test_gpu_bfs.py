@@ -123,7 +123,7 @@ def test_bfs_64_roots_bit_exact_s16(ctx, s16, oracle_mod):
         assert cnt == pr["nedge"]
 
 
-@pytest.mark.parametrize("scale,tuning", [(18, {}), (20, {}), (22, {}), (20, {"bin_edges": 1}),
+@pytest.mark.parametrize("scale,tuning", [(18, {}), (20, {}), (22, {}), (24, {}), (20, {"bin_edges": 1}),
                                           (22, {"bin_edges": 1 << 22}), (20, {"td_pieces": 0}),
                                           (20, {"td_pieces": 1}), (22, {"td_pieces": 7}),
                                           (22, {"td_pieces": 4096}), (20, {"block_shift": 0}),
@@ -163,7 +163,9 @@ def test_bfs_reference_fixtures_larger_scales(scale, tuning, oracle_mod):
             assert "".join("T" if d else "B" for d in st["direction"]) == pr["directions"]
             assert st["nq"] == pr["nq"]
             assert c.teps_edges() == pr["nedge"]
-            if i < 2:
+            ok, errs = c.validate()  # every root's parents, on the GPU
+            assert ok, (pr["root"], errs)
+            if i < 2 and scale < 24:  # the CPU validator too, where it takes seconds
                 edges = c.edges_download()
                 ok, cnt, msg = oracle_mod.validate(edges, info["max_used_vertex"] + 1, pr["root"],
                                                    c.get_pred())
