@@ -192,7 +192,7 @@ def test_batch_partial_and_odd_roots(oracle_mod, mode, rule):
 
 
 @pytest.mark.parametrize("scale,sweep,rule,beta", [(20, 0.25, 0, 50.0), (20, 2.0, 0, 50.0), (20, 0.0, 1, 50.0),
-                                                   (20, 0.0, 0, 0.0), (20, 0.0, 1, 0.0)])
+                                                   (20, 0.0, 0, 0.0), (20, 0.0, 1, 0.0), (24, 0.0, 1, 50.0)])
 def test_batch_matches_single_root_gpu(scale, sweep, rule, beta):
     """beta 0 lets small tail frontiers push again after the list-driven
     tail pulls (whose frontier words and nonzero bits the push then reads)."""
