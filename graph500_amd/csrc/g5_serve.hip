@@ -299,6 +299,7 @@ struct Engine {
   uint64_t last_fdeg = 0;
   std::deque<int64_t> waiting;
   int64_t levels_run = 0;
+  int64_t admit_events = 0;
 
   ~Engine() {
     if (h) (void)hipHostFree(h);
@@ -455,16 +456,23 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
   }
 }
 
-// Admits waiting queries into free lanes; returns the lanes admitted.
+// Admits waiting queries into free lanes; returns the lanes admitted.  An
+// idle graph admits at once.  A graph with searches in flight admits at this
+// level boundary once min(min_admit, free lanes, free answer arrays) queries
+// can start together (a cohort shares its expensive levels), or once the
+// oldest waiting query has waited max_wait_s: queries join the running
+// searches as lanes and answer memory free up.
 static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
   const uint64_t busy_lanes = e.active | e.cooling;
   if (e.waiting.empty() || busy_lanes == ~UINT64_C(0) || e.free_bufs.empty()) return 0;
-  if (e.active) {
-    // a busy graph waits for a cohort (or for the oldest query's patience)
-    const int64_t can = std::min<int64_t>(
-        {(int64_t)e.waiting.size(), (int64_t)(64 - __builtin_popcountll(busy_lanes)), (int64_t)e.free_bufs.size()});
+  const bool in_flight = e.active != 0;
+  if (in_flight) {
+    const int64_t free_lanes = 64 - __builtin_popcountll(busy_lanes);
+    const int64_t free_bufs = (int64_t)e.free_bufs.size();
+    const int64_t can = std::min<int64_t>({(int64_t)e.waiting.size(), free_lanes, free_bufs});
+    const int64_t want = std::min<int64_t>({(int64_t)srv->cfg.min_admit, free_lanes, free_bufs});
     const double waited = t - srv->req[e.waiting.front()].arrival;
-    if (can < srv->cfg.min_admit && waited < srv->cfg.max_wait_s) return 0;
+    if (can < want && waited < srv->cfg.max_wait_s) return 0;
   }
   ServeRoots rs;
   for (int r = 0; r < 64; ++r) {
@@ -503,6 +511,9 @@ static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
     G5_HIP(hipGetLastError());
     e.nzs_lanes[e.level & 1] |= admitted;
     e.active |= admitted;
+    const int64_t k = __builtin_popcountll(admitted);
+    (in_flight ? e.st.admitted_busy : e.st.admitted_idle) += k;
+    ++e.admit_events;
   }
   return admitted;
 }
@@ -630,8 +641,12 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
   int64_t next_arrival = 0, done = 0;
   const int64_t done0 = srv->done_count;
   const double t0 = now_s();
-  int64_t levels0 = 0;
-  for (auto& e : srv->eng) levels0 += e->levels_run;
+  int64_t levels0 = 0, busy0 = 0, events0 = 0;
+  for (auto& e : srv->eng) {
+    levels0 += e->levels_run;
+    busy0 += e->st.admitted_busy;
+    events0 += e->admit_events;
+  }
   int max_active = 0;
   while (done < nq) {
     const double t = now_s() - t0;
@@ -668,8 +683,12 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
     sv_take_teps(srv, *e);
   }
   const double wall = now_s() - t0;
-  int64_t edges = 0, levels1 = 0;
-  for (auto& e : srv->eng) levels1 += e->levels_run;
+  int64_t edges = 0, levels1 = 0, busy1 = 0, events1 = 0;
+  for (auto& e : srv->eng) {
+    levels1 += e->levels_run;
+    busy1 += e->st.admitted_busy;
+    events1 += e->admit_events;
+  }
   for (int64_t i = 0; i < nq; ++i) {
     const Request& q = srv->req[base + i];
     g5_query_result& r = res[i];
@@ -687,6 +706,9 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
     rep->edges = edges;
     rep->levels_launched = levels1 - levels0;
     rep->max_active_lanes = max_active;
+    rep->reserved = 0;
+    rep->admitted_busy = busy1 - busy0;
+    rep->admit_events = events1 - events0;
   }
 }
 
@@ -746,6 +768,8 @@ int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_conf
     delete srv;
     return rc;
   }
+  // the contexts refuse g5_construct and g5_destroy while this server exists
+  for (auto& e : srv->eng) ++e->ctx->servers;
   *out = srv;
   return G5_OK;
 }
@@ -753,7 +777,10 @@ int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_conf
 void g5_server_destroy(g5_server* srv) {
   if (!srv) return;
   (void)hipSetDevice(srv->device);
-  for (auto& e : srv->eng) (void)hipStreamSynchronize(e->s);
+  for (auto& e : srv->eng) {
+    (void)hipStreamSynchronize(e->s);
+    --e->ctx->servers;
+  }
   delete srv;
 }
 
