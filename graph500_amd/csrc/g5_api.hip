@@ -62,8 +62,12 @@ int g5_create(int device, g5_ctx** out) {
   return G5_OK;
 }
 
-void g5_destroy(g5_ctx* ctx) {
-  if (!ctx) return;
+int g5_destroy(g5_ctx* ctx) {
+  if (!ctx) return G5_OK;
+  if (ctx->servers) {
+    ctx->err = "g5_destroy: a server still serves this context's graph (g5_server_destroy first)";
+    return G5_ERR_STATE;
+  }
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->bfs.release();
@@ -74,6 +78,7 @@ void g5_destroy(g5_ctx* ctx) {
   ctx->acc.release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  return G5_OK;
 }
 
 const char* g5_last_error(const g5_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -211,6 +216,8 @@ int g5_generate_kronecker(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1
 
 int g5_construct(g5_ctx* ctx, int reorder_mode, int release_edges) {
   return guarded(ctx, [&] {
+    G5_CHECK_CODE(!ctx->servers, G5_ERR_STATE,
+                  "g5_construct: a server serves this context's graph (g5_server_destroy first)");
     set_device(ctx);
     ctx->bfs.release();
     ctx->ms.release();
@@ -399,12 +406,19 @@ int g5_validate_pred(g5_ctx* ctx, int64_t root, const int64_t* pred, int64_t* er
 int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* t) {
   return guarded(ctx, [&] {
     G5_CHECK(t && t->bin_edges >= 0 && t->td_pieces >= 0 && t->td_pieces <= 4096 &&
-             (t->block_shift == 0 || (t->block_shift >= 12 && t->block_shift <= 31)) && t->td_head >= 0,
+             (t->block_shift == 0 || (t->block_shift >= 12 && t->block_shift <= 31)) && t->td_head >= 0 &&
+             t->seg_max >= 0 && t->seg_max < (INT64_C(1) << 24),
              "g5_bfs_set_tuning: bad tuning");
     ctx->bfs.bin_edges = t->bin_edges;
     ctx->bfs.td_pieces = t->td_pieces;
     ctx->bfs.hub_shift = t->block_shift;
     ctx->bfs.td_head = t->td_head;
+    ctx->bfs.seg_max = t->seg_max;
+    if (ctx->bfs.prepared) {
+      // segment buffers and block index for the new tuning, outside any run
+      set_device(ctx);
+      bfs_tuning_buffers(ctx);
+    }
   });
 }
 
@@ -416,6 +430,7 @@ int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* t) {
     t->td_pieces = ctx->bfs.td_pieces;
     t->block_shift = ctx->bfs.hub_shift;
     t->td_head = ctx->bfs.td_head;
+    t->seg_max = ctx->bfs.seg_max;
   });
 }
 
@@ -459,6 +474,11 @@ int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* t) {
     ctx->ms.pull_grid = t->pull_grid;
     ctx->ms.sweep_below = t->sweep_below;
     ctx->ms.sweep_grid = t->sweep_grid;
+    if (ctx->ms.prepared && t->compact_below > 0) {
+      // the compact frontier's buffers, now rather than inside a batch
+      set_device(ctx);
+      ms_alloc_compact(ctx);
+    }
   });
 }
 

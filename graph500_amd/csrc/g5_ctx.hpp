@@ -73,6 +73,7 @@ struct BfsState {
   DevBuf<uint64_t> seg_scan_tmp;
   DevBuf<uint64_t> seg_start;     // first cols index of each row suffix
   int64_t td_head = 4;            // head threshold factor in 1/16 (0: no head; see k_td_head_size)
+  int64_t seg_max = 0;            // segments per segment-list level (0: the count field's limit)
   int64_t hub_shift = 0;          // target block = 2^hub_shift ids (0: no block index)
   int64_t hub_built_shift = 0;    // shift the index was built for (0: none)
   int64_t hub_pieces = 0;         // blocks + 1
@@ -242,6 +243,7 @@ struct g5_ctx {
   g5::BfsState bfs;
   g5::MsState ms;
   g5::DevBuf<uint64_t> acc;  // small reduction scratch
+  int servers = 0;           // g5_server objects that serve this context's graph
 };
 
 namespace g5 {
@@ -256,6 +258,7 @@ void launch_weighted(int device, const g5_packed_edge* in, int64_t start_index, 
                      uint64_t s1, uint64_t s2, g5_weighted_edge* out, hipStream_t stream);
 // g5_bfs.hip
 void bfs_prepare(g5_ctx* ctx);
+void bfs_tuning_buffers(g5_ctx* ctx);
 void bfs_run(g5_ctx* ctx, int64_t root_orig);
 void bfs_get_pred(g5_ctx* ctx, int64_t* host_pred);
 void bfs_get_levels(g5_ctx* ctx, uint8_t* host_levels);
@@ -266,6 +269,7 @@ void bfs_validate(g5_ctx* ctx, int64_t* errors_out);
 int64_t pred_validate(g5_ctx* ctx, int64_t root, const int64_t* host_pred, int64_t* errors_out);
 // g5_msbfs.hip
 void ms_prepare(g5_ctx* ctx);
+void ms_alloc_compact(g5_ctx* ctx);
 void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots);
 void ms_get_levels(g5_ctx* ctx, int lane, uint8_t* host_levels);
 void ms_teps_edges(g5_ctx* ctx, int64_t* out);

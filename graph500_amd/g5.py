@@ -30,7 +30,7 @@ ERRORS = {-1: "G5_ERR_ARG", -2: "G5_ERR_STATE", -3: "G5_ERR_HIP", -4: "G5_ERR_ME
           -5: "G5_ERR_LIMIT"}
 
 # Every entry point of include/g5.h (the export test checks this list against
-# the header and the library).
+# the header and the library); include/g5_measure.h adds MEASURE_EXPORTS.
 EXPORTS = [
     "g5_version", "g5_device_count", "g5_create", "g5_destroy", "g5_last_error",
     "g5_synchronize", "g5_edges_reserve", "g5_edges_append", "g5_edges_count",
@@ -45,6 +45,7 @@ EXPORTS = [
     "g5_batch_end", "g5_server_create", "g5_server_destroy", "g5_server_last_error", "g5_server_run",
     "g5_server_get_levels", "g5_server_release", "g5_server_graph_stats_get",
 ]
+MEASURE_EXPORTS = ["g5_measure_gather"]
 
 
 class G5Error(RuntimeError):
@@ -115,13 +116,13 @@ class QueryResult(ctypes.Structure):
 class ServerGraphStats(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int64), ("push_levels", ctypes.c_int64), ("pull_levels", ctypes.c_int64),
                 ("mixed_levels", ctypes.c_int64), ("lane_levels", ctypes.c_int64), ("busy_ms", ctypes.c_double),
-                ("completed", ctypes.c_int64)]
+                ("completed", ctypes.c_int64), ("admitted_busy", ctypes.c_int64), ("admitted_idle", ctypes.c_int64)]
 
 
 class ServerReport(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("completed", ctypes.c_int64), ("edges", ctypes.c_int64),
                 ("levels_launched", ctypes.c_int64), ("max_active_lanes", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("admitted_busy", ctypes.c_int64), ("admit_events", ctypes.c_int64)]
 
 
 QUERY_DTYPE = np.dtype([("graph", np.int32), ("keep", np.int32), ("root", np.int64), ("arrival_s", np.float64)])
@@ -153,7 +154,7 @@ def load_library(path=LIB_PATH):
         "g5_version": ([], ctypes.c_char_p),
         "g5_device_count": ([ctypes.POINTER(i32)], i32),
         "g5_create": ([i32, ctypes.POINTER(vp)], i32),
-        "g5_destroy": ([vp], None),
+        "g5_destroy": ([vp], i32),
         "g5_last_error": ([vp], ctypes.c_char_p),
         "g5_synchronize": ([vp], i32),
         "g5_edges_reserve": ([vp, i64], i32),
@@ -199,6 +200,8 @@ def load_library(path=LIB_PATH):
         "g5_server_get_levels": ([vp, i64, vp], i32),
         "g5_server_release": ([vp, i64], i32),
         "g5_server_graph_stats_get": ([vp, i32, ctypes.POINTER(ServerGraphStats)], i32),
+        # include/g5_measure.h
+        "g5_measure_gather": ([i32, i64, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -228,12 +231,17 @@ class G5Context:
         self._h = h
 
     def close(self):
+        """Frees the context; raises G5Error (and keeps it) while a G5Server
+        over it is still open."""
         if getattr(self, "_h", None):
-            self._lib.g5_destroy(self._h)
+            self._check(self._lib.g5_destroy(self._h))
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except G5Error:
+            pass
 
     def __enter__(self):
         return self
@@ -366,7 +374,9 @@ class G5Context:
         L = self.bfs_stats()["num_levels"]
         return per[1:L].tolist(), tot.value
 
-    def bfs_set_tuning(self, bin_edges=None, td_pieces=None, block_shift=None, td_head=None):
+    BFS_TUNING = ("bin_edges", "td_pieces", "block_shift", "td_head", "seg_max")
+
+    def bfs_set_tuning(self, bin_edges=None, td_pieces=None, block_shift=None, td_head=None, seg_max=None):
         """Single-root tuning; a knob left None keeps its value.  bin_edges:
         top-down levels with at least this many frontier edges are binned by
         target range (0 = never).  td_pieces: a growing bulk top-down level
@@ -376,24 +386,20 @@ class G5Context:
         td_head: a growing bulk top-down level with E frontier edges settles
         its targets of degree >= td_head / 16 * (nnz / E)^2 (a prefix of the
         degree-ordered rows) with a bottom-up probe and skips their edges
-        (0 = off; degree-ordered graphs only)."""
-        t = np.zeros(4, np.int64)
+        (0 = off; degree-ordered graphs only).  seg_max: most segments of a
+        segment-list level (0 = 2^24 - 1); a level beyond it is taken in row
+        order (tests lower it to reach that fallback on small graphs)."""
+        t = np.zeros(len(self.BFS_TUNING), np.int64)
         self._check(self._lib.g5_bfs_get_tuning(self._h, _ptr(t)))
-        if bin_edges is not None:
-            t[0] = bin_edges
-        if td_pieces is not None:
-            t[1] = td_pieces
-        if block_shift is not None:
-            t[2] = block_shift
-        if td_head is not None:
-            t[3] = td_head
+        for i, v in enumerate((bin_edges, td_pieces, block_shift, td_head, seg_max)):
+            if v is not None:
+                t[i] = v
         self._check(self._lib.g5_bfs_set_tuning(self._h, _ptr(t)))
 
     def bfs_tuning(self):
-        t = np.zeros(4, np.int64)
+        t = np.zeros(len(self.BFS_TUNING), np.int64)
         self._check(self._lib.g5_bfs_get_tuning(self._h, _ptr(t)))
-        return {"bin_edges": int(t[0]), "td_pieces": int(t[1]), "block_shift": int(t[2]),
-                "td_head": int(t[3])}
+        return {k: int(v) for k, v in zip(self.BFS_TUNING, t)}
 
     VALIDATION_RULES = ["root", "self_parent", "depth", "edge", "parent_edge", "range"]
 
@@ -506,12 +512,13 @@ class G5Server:
     """Serving loop over graphs resident in HBM (g5_server_*): continuous
     batching of (graph, root) BFS queries, up to 64 in flight per graph."""
 
-    def __init__(self, contexts, answer_buffers=128, alpha=64.0, pull_filter=0, min_admit=64, max_wait_s=0.3,
+    def __init__(self, contexts, answer_buffers=128, alpha=64.0, pull_filter=0, min_admit=8, max_wait_s=0.005,
                  lane_rule=3):
-        """Defaults: a busy graph admits new queries as a cohort of 64 or once
-        the oldest has waited 0.3 s; each search pulls while its frontier's
-        edges exceed 1/64 of the edges it has not reached, and pushes
-        otherwise."""
+        """Defaults: a graph with searches in flight admits waiting queries
+        at a level boundary once 8 of them (or as many as there are free
+        lanes and answer arrays) can start together, or once the oldest has
+        waited 5 ms; each search pulls while its frontier's edges exceed 1/64
+        of the edges it has not reached, and pushes otherwise."""
         self._lib = load_library()
         self._ctxs = list(contexts)  # the server uses their graphs and streams
         arr = (ctypes.c_void_p * len(self._ctxs))(*[c._h for c in self._ctxs])
@@ -551,7 +558,8 @@ class G5Server:
         self._check(self._lib.g5_server_run(self._h, _ptr(q), len(q), float(timeout_s), _ptr(res),
                                             ctypes.byref(rep)))
         return res, {"wall_s": rep.wall_s, "completed": rep.completed, "edges": rep.edges,
-                     "levels_launched": rep.levels_launched, "max_active_lanes": rep.max_active_lanes}
+                     "levels_launched": rep.levels_launched, "max_active_lanes": rep.max_active_lanes,
+                     "admitted_busy": rep.admitted_busy, "admit_events": rep.admit_events}
 
     def levels(self, request, n):
         out = np.zeros(n, np.uint8)

@@ -131,7 +131,10 @@ const char* g5_version(void);
 int g5_device_count(int* count);
 /* Creates a context on HIP device 'device' with its own stream. */
 int g5_create(int device, g5_ctx** out);
-void g5_destroy(g5_ctx* ctx);
+/* Frees the context and everything it holds.  Returns G5_ERR_STATE, and
+ * frees nothing, while a g5_server created over the context still exists
+ * (g5_server_destroy first); a NULL context is a no-op. */
+int g5_destroy(g5_ctx* ctx);
 /* Message of the last failing call on ctx (empty when none). */
 const char* g5_last_error(const g5_ctx* ctx);
 /* Waits for all work queued by ctx. */
@@ -181,7 +184,8 @@ int g5_generate_graph(g5_ctx* ctx, int scale, int edgefactor, uint64_t seed1, ui
 /* Builds the CSR from the edge list.  reorder_mode is VERTEX_REORDERING
  * (mpi/parameters.h:38-40): 2 = degree order (ties by id), 1 = vertices with
  * edges first, 0 = original order.  release_edges != 0 frees the device edge
- * list afterwards (EdgeListStorage::beginRead(true), graph_generator.hpp:98). */
+ * list afterwards (EdgeListStorage::beginRead(true), graph_generator.hpp:98).
+ * G5_ERR_STATE while a g5_server serves the context's graph. */
 int g5_construct(g5_ctx* ctx, int reorder_mode, int release_edges);
 int g5_graph_info_get(g5_ctx* ctx, g5_graph_info* out);
 /* Graph2DCSR::has_edge (mpi/graph_constructor.hpp:101-110): 1, 0 or < 0. */
@@ -238,12 +242,18 @@ int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
  *     td_head / 16 * (nnz / E)^2 (E = the level's frontier edges; a prefix of
  *     the degree-ordered rows) are settled bottom-up first and the expansion
  *     walks only the row suffixes beyond them (default 4; 0 = off; degree
- *     order only). */
+ *     order only).
+ *   seg_max: most segments one segment-list level may hold (0 = the count
+ *     field's limit, 2^24 - 1, the default; 1..2^24 - 1 lower it).  A level
+ *     whose frontier rows (suffix mode) or pieces would exceed it is taken
+ *     in plain row order instead.  Only tests lower it, to reach that path
+ *     on small graphs. */
 typedef struct g5_bfs_tuning {
   int64_t bin_edges;
   int64_t td_pieces;
   int64_t block_shift;
   int64_t td_head;
+  int64_t seg_max;
 } g5_bfs_tuning;
 int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* tuning);
 int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* tuning);
@@ -280,9 +290,12 @@ int g5_batch_teps_edges(g5_ctx* ctx, int64_t* counts);
 int g5_batch_stats_get(g5_ctx* ctx, g5_batch_stats* out);
 /* Algorithmic bytes of the last batch under the bit-packed pull model: per
  * level 24 n + sum over vertices lacking some root of (8 + 12 k), k = probes
- * until every lacking root is found; plus 1 byte per (root, reached vertex).
- * Under lane_rule 1 only the roots pulled at a level count as lacking, and
- * every vertex whose frontier word holds a pushed root adds 8 + 12 deg. */
+ * until every lacking root is found.  Under lane_rule 1 only the roots pulled
+ * at a level count as lacking, and every vertex whose frontier word holds a
+ * pushed root adds 8 + 12 deg.  The batch's output is its per-level frontier
+ * words (the write inside 24 n); a root's uint8 levels are unpacked from them
+ * by g5_batch_get_levels, outside the batch, so no per-(root, vertex) output
+ * byte is counted. */
 int g5_batch_model_bytes(g5_ctx* ctx, double* per_level, double* total);
 /* Per level of the last batch (G5_MAX_LEVELS entries each, index = level):
  * the roots that pushed and the roots costed as pulling in the byte model
@@ -310,14 +323,17 @@ typedef struct g5_server_config {
   double alpha;            /* lane_rule 1: a search pulls when its frontier has more than nnz/alpha
                               edges, else pushes; lane_rule 3: when its frontier's edges exceed
                               1/alpha of the edges of the vertices it has not reached */
-  int32_t min_admit;       /* a busy graph admits only when at least this many queries can start
-                              together (searches started together share their sweeps) ... */
+  int32_t min_admit;       /* a graph with searches in flight admits waiting queries at a level
+                              boundary once min(min_admit, free lanes, free answer arrays) of them
+                              can start together (searches started together share their sweeps;
+                              1 admits every query at the first boundary with a free lane) ... */
   int32_t lane_rule;       /* 0: each search switches direction by the reference rule on its own
                               frontier size (mpi/bfs.hpp:2799-2829); 1: by its frontier's edges;
                               2: all searches together, by the frontier union's edges; 3: by its
                               frontier's edges against its unreached edges (as g5_batch_tuning
                               lane_rule 1) */
-  double max_wait_s;       /* ... or when its oldest waiting query has waited this long */
+  double max_wait_s;       /* ... or once its oldest waiting query has waited this long.  An idle
+                              graph admits at once. */
 } g5_server_config;
 
 typedef struct g5_query {
@@ -343,6 +359,8 @@ typedef struct g5_server_report {
   int64_t levels_launched;  /* engine levels over all graphs */
   int32_t max_active_lanes; /* most lanes active at once in one graph */
   int32_t reserved;
+  int64_t admitted_busy;    /* queries admitted into a graph that had searches in flight */
+  int64_t admit_events;     /* level boundaries at which some query was admitted */
 } g5_server_report;
 
 /* Per-graph engine counters since the server was created. */
@@ -354,6 +372,8 @@ typedef struct g5_server_graph_stats {
   int64_t lane_levels;    /* sum over levels of the active lanes */
   double busy_ms;         /* device time of the levels (HIP events) */
   int64_t completed;      /* queries answered */
+  int64_t admitted_busy;  /* queries admitted while other searches of this graph were in flight */
+  int64_t admitted_idle;  /* queries admitted into an idle graph */
 } g5_server_graph_stats;
 
 int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_config* config, g5_server** out);

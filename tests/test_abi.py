@@ -9,9 +9,18 @@ import pytest
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_functions():
-    txt = open(os.path.join(REPO, "include", "g5.h")).read()
+def header_functions(name="g5.h"):
+    txt = open(os.path.join(REPO, "include", name)).read()
     return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int|void)\s+(g5_\w+)\s*\(", txt, re.M)))
+
+
+def all_header_functions():
+    """Every C entry point declared in any header under include/."""
+    names = set()
+    for f in os.listdir(os.path.join(REPO, "include")):
+        if f.endswith(".h"):
+            names |= set(header_functions(f))
+    return sorted(names)
 
 
 @pytest.fixture(scope="module")
@@ -29,8 +38,17 @@ def test_header_lists_every_binding():
 
 
 def test_library_exports_every_header_symbol(lib):
-    for name in header_functions():
+    from graph500_amd import g5
+    assert all_header_functions() == sorted(g5.EXPORTS + g5.MEASURE_EXPORTS)
+    for name in all_header_functions():
         assert hasattr(lib, name), name
+
+
+def test_measure_gather_rejects_bad_sizes(lib):
+    ms = ctypes.c_double(0)
+    chk = ctypes.c_uint64(0)
+    for nbytes, gathers in ((12, 1), (0, 1), (64, 0)):
+        assert lib.g5_measure_gather(0, nbytes, gathers, ctypes.byref(ms), ctypes.byref(chk)) == -1
 
 
 def test_version_and_device_query_without_gpu(lib):
@@ -41,6 +59,7 @@ def test_version_and_device_query_without_gpu(lib):
 
 
 def test_null_context_is_rejected(lib):
+    assert lib.g5_destroy(None) == 0
     assert lib.g5_run_bfs(None, 0) == -1
     assert lib.g5_construct(None, 2, 0) == -1
     assert lib.g5_last_error(None) == b"null context"
