@@ -5,7 +5,10 @@ switch thresholds of g5_run_bfs_batch by measurement: coordinate descent over
 each knob in turn (the others fixed at their best so far), every trial the
 device time of a whole batch (minimum of `reps` runs).  Every configuration
 computes the same levels (the parity tests run each level kind against the
-oracle); the tuner also checks that the traversed-edge counts never change.
+oracle); the tuner also refuses a trial whose traversed-edge counts or whose
+(root, vertex) pairs discovered at each level differ from the first run's:
+the edge counts fix each root's reached set, the per-level pair counts (which
+do not depend on the direction a level took) catch a level assigned wrongly.
 
 Results are cached in a JSON file keyed by the graph (estimated SCALE,
 active vertices, directed entries, reorder mode) and the library build
@@ -66,11 +69,14 @@ def _save(path, data):
 
 
 def _measure(ctx, roots, reps):
-    best = float("inf")
+    """Best device time of `reps` batches, and the last one's per-level pair counts."""
+    best, pairs = float("inf"), None
     for _ in range(reps):
         ctx.run_bfs_batch(roots)
-        best = min(best, ctx.batch_stats()["device_ms"])
-    return best
+        st = ctx.batch_stats()
+        best = min(best, st["device_ms"])
+        pairs = list(st.get("pairs", []))
+    return best, pairs
 
 
 def autotune(ctx, roots, reps=2, budget_s=60.0, cache=DEFAULT_CACHE, use_cache=True, log=None):
@@ -88,7 +94,7 @@ def autotune(ctx, roots, reps=2, budget_s=60.0, cache=DEFAULT_CACHE, use_cache=T
     ctx.run_bfs_batch(roots)  # warm-up (allocates per-level arrays)
     ref_edges = ctx.batch_teps_edges().tolist()
     cur = ctx.batch_tuning()
-    best_ms = _measure(ctx, roots, reps)
+    best_ms, ref_pairs = _measure(ctx, roots, reps)
     trials = 1
     for knob, values in SPACE:
         if knob == "persist_waves" and not cur["persistent"]:
@@ -100,10 +106,12 @@ def autotune(ctx, roots, reps=2, budget_s=60.0, cache=DEFAULT_CACHE, use_cache=T
                 continue
             trial = dict(cur, **setting)
             ctx.batch_set_tuning(**trial)
-            ms = _measure(ctx, roots, reps)
+            ms, pairs = _measure(ctx, roots, reps)
             trials += 1
             if ctx.batch_teps_edges().tolist() != ref_edges:
                 raise RuntimeError("autotune: traversed-edge counts changed under %s" % trial)
+            if pairs != ref_pairs:
+                raise RuntimeError("autotune: pairs discovered per level changed under %s" % trial)
             if log:
                 log("autotune %s=%s: %.3f ms (best %.3f)" % (knob, v, ms, best_ms))
             if ms < best_ms:

@@ -11,11 +11,12 @@ class FakeCtx:
     """Batch time is a fixed function of the tuning; optionally one knob
     value corrupts the edge counts."""
 
-    def __init__(self, bad=None):
+    def __init__(self, bad=None, bad_pairs=None):
         self.t = {"alpha": 16.0, "pull_grid": 8192, "sweep_below": 0.25, "sweep_grid": 4096, "persistent": 0,
                   "persist_levels": 16, "persist_grid": 0, "persist_waves": 8, "pull_filter": 0, "beta": 50.0,
                   "compact_below": 0.0, "filter_from": 0.01, "pull_unroll": 0, "lane_rule": 0}
         self.bad = bad
+        self.bad_pairs = bad_pairs
         self.runs = 0
 
     def graph_info(self):
@@ -39,7 +40,10 @@ class FakeCtx:
         a = abs(t["alpha"] - 128.0) / 64 - 1 if t["lane_rule"] else abs(t["alpha"] - 2.0) / 8
         ms = 10.0 + a + (0 if t["beta"] == 20.0 else 1) + t["pull_filter"] \
             + (2 if t["persistent"] else 0) + (0 if t["pull_grid"] == 16384 else 0.5)
-        return {"device_ms": ms}
+        pairs = [3, 40, 500, 7]
+        if self.bad_pairs and self.t.get(self.bad_pairs[0]) == self.bad_pairs[1]:
+            pairs = [3, 41, 499, 7]  # a vertex given the wrong level: same reached set
+        return {"device_ms": ms, "pairs": pairs}
 
     def batch_teps_edges(self):
         e = np.arange(4, dtype=np.int64)
@@ -73,10 +77,17 @@ def test_changed_results_are_refused(tmp_path):
         autotune.autotune(FakeCtx(bad=("alpha", 8.0)), [1], reps=1, cache=str(tmp_path / "t.json"))
 
 
+def test_wrong_levels_with_the_same_reached_set_are_refused(tmp_path):
+    """A trial that moves a vertex to another level keeps every traversed-edge
+    count; the per-level pair counts catch it."""
+    with pytest.raises(RuntimeError, match="per level"):
+        autotune.autotune(FakeCtx(bad_pairs=("pull_unroll", 3)), [1], reps=1, cache=str(tmp_path / "t.json"))
+
+
 def test_union_rule_alpha_is_kept_when_it_wins(tmp_path):
     class UnionWins(FakeCtx):
         def batch_stats(self):
-            ms = super().batch_stats()["device_ms"]
-            return {"device_ms": ms + (5.0 if self.t["lane_rule"] else 0.0)}
+            st = super().batch_stats()
+            return {"device_ms": st["device_ms"] + (5.0 if self.t["lane_rule"] else 0.0), "pairs": st["pairs"]}
     res = autotune.autotune(UnionWins(), [1], reps=1, cache=str(tmp_path / "t.json"))
     assert res["tuning"]["lane_rule"] == 0 and res["tuning"]["alpha"] == 2.0

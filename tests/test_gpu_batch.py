@@ -114,6 +114,32 @@ def test_batch_per_root_directions_match_reference_s16(s16ctx, alpha, filt):
     _per_root_run(s16ctx, alpha, filt)
 
 
+@pytest.mark.parametrize("rule", [0, 1])
+@pytest.mark.parametrize("unroll", [0, 1, 2, 3, 4])
+def test_batch_pull_unroll_matches_reference_s16(s16ctx, unroll, rule):
+    """Every probe unrolling of the per-level pull (k_ms_pull<4,2>, <8,2>,
+    <8,4>, <4,4>, <2,1>) gives the reference's levels: rule 0 with an alpha
+    that never pushes runs every level after the first as that pull; rule 1
+    runs it once every root pulls."""
+    g = gold("ref_s16_64roots.json")
+    roots = [p["root"] for p in g["per_root"]]
+    s16ctx.batch_set_tuning(lane_rule=rule, alpha=1e12 if rule == 0 else 128.0, pull_unroll=unroll, persistent=0,
+                            beta=50.0, sweep_below=0.0, pull_filter=0, compact_below=0.0)
+    say("batch s16 pull_unroll", unroll, "rule", rule)
+    try:
+        s16ctx.run_bfs_batch(roots)
+        st = s16ctx.batch_stats()
+        assert 0 in st["direction"], st["direction"]
+        teps = s16ctx.batch_teps_edges()
+        for i, pr in enumerate(g["per_root"]):
+            assert sha(s16ctx.batch_levels(i)) == pr["levels_sha256"], (unroll, rule, pr["root"])
+            assert teps[i] == pr["nedge"]
+        assert sum(st["pairs"]) + 64 == sum(sum(p["level_histogram"]) for p in g["per_root"])
+    finally:
+        s16ctx.batch_set_tuning(alpha=16.0, sweep_below=0.25, persistent=0, pull_filter=0, beta=50.0, lane_rule=0,
+                                pull_unroll=0)
+
+
 def test_batch_per_root_levels_mix_directions_s16(s16ctx):
     """Some level of the S16 batch runs pushes and pulls together at alpha 4."""
     d = _per_root_run(s16ctx, 4.0, 0)
@@ -212,6 +238,35 @@ def test_batch_matches_single_root_gpu(scale, sweep, rule, beta):
             assert teps[i] == pr["nedge"]
         c.prepare_bfs()
         for i in range(16, 64, 7):
+            c.run_bfs(int(roots[i]))
+            assert np.array_equal(c.get_levels(), c.batch_levels(i)), i
+            assert c.teps_edges() == teps[i]
+
+
+@pytest.mark.parametrize("unroll", [0, 4])
+@pytest.mark.parametrize("scale", [20, 22])
+def test_batch_tuned_combination_matches_reference(scale, unroll):
+    """The combination the autotuner picked for the SCALE 28 bench (per-root
+    rule, alpha 256, beta 200, filter_from 0.001, probe unrolling 4) and the
+    same with unrolling 0: the fixture roots' level hashes and edge counts,
+    and single-root runs of roots 16-63."""
+    import graph500_amd
+    g = gold("ref_s%d_16roots.json" % scale)
+    with graph500_amd.G5Context(0) as c:
+        c.generate_kronecker(scale)
+        c.construct(2, release_edges=True)
+        roots = c.find_roots(64)
+        c.batch_prepare()
+        c.batch_set_tuning(lane_rule=1, alpha=256.0, beta=200.0, filter_from=0.001, pull_unroll=unroll,
+                           pull_grid=8192, sweep_below=0.0, persistent=0, pull_filter=0, compact_below=0.0)
+        say("batch s%d tuned combination, unroll %d" % (scale, unroll))
+        c.run_bfs_batch(roots)
+        teps = c.batch_teps_edges()
+        for i, pr in enumerate(g["per_root"]):
+            assert sha(c.batch_levels(i)) == pr["levels_sha256"], (scale, unroll, pr["root"])
+            assert teps[i] == pr["nedge"]
+        c.prepare_bfs()
+        for i in range(16, 64, 5):
             c.run_bfs(int(roots[i]))
             assert np.array_equal(c.get_levels(), c.batch_levels(i)), i
             assert c.teps_edges() == teps[i]
