@@ -839,6 +839,33 @@ static void wait_published(BfsState& b, hipStream_t s) {
   for (int i = 0; i < C_NUM; ++i) b.h_counters[i] = h[i];
 }
 
+static void build_hub_index(g5_ctx* ctx);
+
+// Buffers of the segment-list levels, sized for the tuning so that no run
+// allocates: a growing top-down level has at most n_active / 2000 frontier
+// vertices (one more switches to bottom-up), so a head level's row suffixes
+// take at most that many segments and a piece or block level that many
+// times its pieces, within seg_max.  The block index of the first rows is
+// built here too.  (The binned form, off by default, sizes its pair buffer
+// by the level's edges inside the run.)
+void bfs_tuning_buffers(g5_ctx* ctx) {
+  Graph& g = ctx->graph;
+  BfsState& b = ctx->bfs;
+  if (b.hub_shift > 0 && b.hub_built_shift != b.hub_shift) build_hub_index(ctx);
+  const int64_t seg_max = b.seg_max > 0 ? std::min<int64_t>(b.seg_max, kSegMax) : kSegMax;
+  const int64_t nf_max = g.n_active / 2000 + 1;
+  int64_t pieces = 1;
+  if (b.td_pieces > 0) pieces = std::max<int64_t>(pieces, b.td_pieces);
+  if (b.hub_shift > 0) pieces = std::max<int64_t>(pieces, b.hub_pieces);
+  const size_t seg_n = (size_t)std::min<int64_t>(nf_max * pieces, seg_max) + 2;
+  b.seg_pre.alloc(seg_n);
+  b.seg_base.alloc(seg_n);
+  b.seg_src.alloc(seg_n);
+  b.seg_start.alloc(seg_n);
+  b.seg_count.alloc(1);
+  b.seg_scan_tmp.alloc(scan_tmp_elems(seg_n));
+}
+
 void bfs_prepare(g5_ctx* ctx) {
   Graph& g = ctx->graph;
   G5_CHECK(g.built, "g5_prepare_bfs: no graph (call g5_construct first)");
@@ -855,16 +882,7 @@ void bfs_prepare(g5_ctx* ctx) {
   const size_t scan_n = (size_t)rows + 1;
   b.scan_tmp.alloc(scan_tmp_elems(scan_n));
   b.counters.alloc(C_NUM);
-  // the segment list of a head level: one segment per frontier row, and a
-  // growing top-down level has at most n_active / 2000 frontier vertices
-  // (one more switches to bottom-up), so no run allocates it mid-search
-  const size_t seg_n = (size_t)(g.n_active / 2000) + 2;
-  b.seg_pre.alloc(seg_n);
-  b.seg_base.alloc(seg_n);
-  b.seg_src.alloc(seg_n);
-  b.seg_start.alloc(seg_n);
-  b.seg_count.alloc(1);
-  b.seg_scan_tmp.alloc(scan_tmp_elems(seg_n));
+  bfs_tuning_buffers(ctx);
   G5_HIP(hipHostMalloc((void**)&b.h_counters, C_NUM * sizeof(uint64_t), hipHostMallocDefault));
   G5_HIP(hipHostMalloc((void**)&b.h_pub, P_NUM * sizeof(uint64_t), hipHostMallocCoherent | hipHostMallocMapped));
   for (int i = 0; i < P_NUM; ++i) b.h_pub[i] = 0;
@@ -912,7 +930,7 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
   g5_bfs_stats& st = b.stats;
   st = g5_bfs_stats{};
 
-  if (b.hub_shift > 0 && b.hub_built_shift != b.hub_shift) build_hub_index(ctx);
+  if (b.hub_shift > 0 && b.hub_built_shift != b.hub_shift) bfs_tuning_buffers(ctx);
 
   G5_HIP(hipEventRecord(b.ev0, s));
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, b.level.p,
@@ -962,7 +980,8 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
       const bool known_bulk = e_next >= 0 && (uint64_t)e_next > kTdSmall;
       // while the search grows, a bulk level can take its rows block by
       // block (exact target blocks) or piece by piece (positional pieces)
-      const bool blocked = b.hub_shift > 0 && !probe_vis && nf * b.hub_pieces <= kSegMax;
+      const int64_t seg_max = b.seg_max > 0 ? std::min<int64_t>(b.seg_max, kSegMax) : kSegMax;
+      const bool blocked = b.hub_shift > 0 && !probe_vis && nf * b.hub_pieces <= seg_max;
       // a growing bulk level settles its highest-degree targets bottom-up
       // first (degree order only)
       bool head = b.td_head > 0 && !probe_vis && g.reorder_mode == 2;
@@ -973,10 +992,10 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
         if ((double)b.td_head / 16.0 * r * r > (double)g.max_degree) head = false;
       }
       unsigned long long* head_words = head ? cnt + C_HEAD : nullptr;
-      const bool pieces = blocked || (b.td_pieces > 0 && !probe_vis && nf * b.td_pieces <= kSegMax);
+      const bool pieces = blocked || (b.td_pieces > 0 && !probe_vis && nf * b.td_pieces <= seg_max);
       // with a head and row order, the expansion walks row suffixes (one
       // segment per frontier row: the segment count field bounds nf)
-      if (head && !pieces && nf > kSegMax) head = false;
+      if (head && !pieces && nf > seg_max) head = false;
       head_words = head ? cnt + C_HEAD : nullptr;
       const bool suffix = head && !pieces;
       const bool binned = b.bin_edges > 0 && E_known > kTdSmall && E_known >= (uint64_t)b.bin_edges;

@@ -130,14 +130,19 @@ def test_bfs_64_roots_bit_exact_s16(ctx, s16, oracle_mod):
                                           (20, {"block_shift": 17}), (22, {"block_shift": 12}),
                                           (20, {"td_head": 1}), (22, {"td_head": 16}), (22, {"td_head": 0}),
                                           (22, {"td_head": 1 << 30}), (22, {"td_head": 4, "block_shift": 14}),
-                                          (22, {"td_head": 4, "td_pieces": 64})])
+                                          (22, {"td_head": 4, "td_pieces": 64}),
+                                          (20, {"td_head": 4, "seg_max": 8}),
+                                          (22, {"td_head": 4, "td_pieces": 64, "seg_max": 1000}),
+                                          (22, {"td_head": 0, "block_shift": 14, "seg_max": 2000})])
 def test_bfs_reference_fixtures_larger_scales(scale, tuning, oracle_mod):
     """An empty tuning keeps the defaults; bin_edges 1 bins every bulk
     top-down level; td_pieces 0 takes bulk rows in row order, 1 through a
     one-piece-per-row segment list, 4096 is the largest piece count (levels
     whose segment list would not fit fall back to row order); block_shift 0
     turns target blocks off, 12 at SCALE 22 indexes only the first rows, so
-    both indexed and whole-row segments occur."""
+    both indexed and whole-row segments occur.  seg_max lowers the segment
+    count a segment-list level may hold, so that hub levels whose frontier
+    rows (suffix mode) or pieces exceed it take the row-order fallback."""
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
     say("scale", scale, "tuning", tuning)
@@ -258,6 +263,8 @@ def test_error_paths():
             c.bfs_set_tuning(td_pieces=4097)  # more pieces than allowed
         with pytest.raises(E):
             c.bfs_set_tuning(block_shift=5)  # blocks below 2^12 ids
+        with pytest.raises(E):
+            c.bfs_set_tuning(seg_max=1 << 24)  # beyond the segment count field
         c.generate_kronecker(8)
         with pytest.raises(E):
             c.construct(7)  # unknown reorder mode
