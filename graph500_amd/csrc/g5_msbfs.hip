@@ -460,11 +460,6 @@ __global__ void __launch_bounds__(kBlock) k_ms_teps(const uint64_t* __restrict__
   if (threadIdx.x < 65 && s_acc[threadIdx.x]) atomicAdd(acc + threadIdx.x, s_acc[threadIdx.x]);
 }
 
-struct PopCount {
-  const uint64_t* p;
-  __device__ uint64_t operator()(size_t i) const { return (uint64_t)__popcll(p[i]); }
-};
-
 // --- batched byte model (evaluated after the run from the per-level words) ---
 // level l: 24 n + sum over vertices with a needed pulled lane of (8 + 12 k),
 // k = probes until every needed lane is found (the row length if not), +
@@ -498,6 +493,17 @@ __global__ void __launch_bounds__(kBlock) k_ms_model(const uint64_t* const* __re
 // ---------------------------------------------------------------------------
 // Host
 // ---------------------------------------------------------------------------
+// Frontier levels preallocated by g5_batch_prepare (a SCALE 28 batch takes 9).
+constexpr int kMsPreLevels = 16;
+
+// The compact frontier copy of the pull (compact_below > 0).
+void ms_alloc_compact(g5_ctx* ctx) {
+  MsState& m = ctx->ms;
+  const int64_t rows = ctx->graph.n_rows;
+  m.pack.alloc(2 * ((rows + 63) / 64));
+  m.compact.alloc(rows);
+}
+
 void ms_prepare(g5_ctx* ctx) {
   Graph& g = ctx->graph;
   G5_CHECK(g.built, "g5_batch_prepare: no graph (call g5_construct first)");
@@ -521,6 +527,26 @@ void ms_prepare(g5_ctx* ctx) {
                        dim3(kBlock), 0, ctx->stream, g.offsets.p, rows, m.sweep_chunks, m.sweep_rows.p);
     G5_HIP(hipGetLastError());
   }
+  // Every buffer a level may need, sized here so that no batch allocates
+  // (the in-loop alloc calls below are then no-ops):
+  //   chunk_list / nz_sel  the mixed level's chunk list and selection bitmap
+  //   touch                a listed push level has fewer than rows / kTouchBelow frontier edges
+  //   miss / ulist         a list-driven tail level follows a level with fewer than
+  //                        rows / kListBelow unreached (root, vertex) pairs
+  //   fr[0, kMsPreLevels)  the frontier words of the first levels (deeper
+  //                        searches add levels as they reach them)
+  const int64_t words = (rows + 63) / 64;
+  if (m.sweep_chunks) m.chunk_list.alloc(m.sweep_chunks);
+  m.nz_sel.alloc(words);
+  m.touch.alloc((size_t)((double)rows / kTouchBelow) + 2);
+  m.miss.alloc(words);
+  m.ulist.alloc((size_t)((double)rows / kListBelow) + 2);
+  m.ulist_count.alloc(1);
+  for (int l = 0; l < kMsPreLevels; ++l) {
+    while ((int)m.fr.size() <= l) m.fr.emplace_back(new DevBuf<uint64_t>());
+    m.fr[l]->alloc(rows);
+  }
+  if (m.compact_below > 0) ms_alloc_compact(ctx);
   G5_HIP(hipHostMalloc((void**)&m.h_counters, (M_NUM + 1 + 128) * sizeof(uint64_t), hipHostMallocDefault));
   G5_HIP(hipEventCreate(&m.ev0));
   G5_HIP(hipEventCreate(&m.ev1));
@@ -1017,11 +1043,10 @@ void ms_model_bytes(g5_ctx* ctx, double* per_level, double* total) {
   ms_upload_level_ptrs(m, nlev, s);
   ctx->acc.alloc(65);
   const uint64_t lanes = m.nroots == 64 ? ~UINT64_C(0) : ((UINT64_C(1) << m.nroots) - 1);
-  // 1 B per (root, reached vertex)
+  // The batch's output is fr[l] (written at every level, inside the 24 n
+  // term); a root's uint8 levels are unpacked from it on request, outside
+  // the batch, so no per-(root, vertex) output byte is counted.
   double tot = 0;
-  for (int l = 0; l < nlev; ++l) tot += (double)(l == 0 ? 0 : m.stats.pairs[l]);
-  // the roots placed at level 0
-  tot += (double)device_reduce_sum(PopCount{m.fr[0]->p}, (size_t)g.n_rows, ctx->acc.p, s);
   for (int l = 0; l < G5_MAX_LEVELS; ++l)
     if (per_level) per_level[l] = 0;
   for (int l = 1; l < nlev; ++l) {

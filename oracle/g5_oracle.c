@@ -565,10 +565,6 @@ double or_msbfs_bytes(const or_csr* g, int lanes, const uint8_t* levels_new, int
     }
   uint64_t* front = (uint64_t*)calloc((size_t)na, sizeof(uint64_t));
   double total = 0;
-  int64_t reached = 0;
-  for (int r = 0; r < lanes; ++r)
-    for (int64_t v = 0; v < na; ++v) reached += levels_new[(size_t)r * n + v] != 0xFF;
-  total += (double)reached;
   /* Batch levels 1..maxl+1 (the last one discovers nothing and ends the batch). */
   int L = maxl + 1;
   for (int l = 1; l <= L; ++l) {

@@ -156,7 +156,11 @@ int or_bfs(const or_csr* g, int64_t root, uint8_t* level_by_orig, int64_t* pred_
 
 /* Batched algorithmic bytes for a 64-lane bit-packed pull sweep given the
  * per-lane levels (by NEW id, 0xFF unreached). levels is lanes x num_verts.
- * Returns total bytes; per-level bytes into per_level (may be NULL).
+ * Returns total bytes; per-level bytes into per_level (may be NULL).  The
+ * batch's output is its per-level frontier words (written at every level and
+ * counted in the 24 n term); the uint8 level array of a root is unpacked
+ * from them on request, outside the batch, so no per-(root, vertex) output
+ * byte is added.
  * pull_lanes / push_lanes (indexed by level, may be NULL): with per-root
  * directions only the pulled lanes count as needed, and a vertex whose
  * frontier word holds a pushed lane adds 8 + 12 deg (NULL: every lane pulls). */
