@@ -8,6 +8,15 @@
 #   oracle/_ref/refprobe   our dump driver (oracle/ref_probe.cc) linked against
 #                          the reference generator/constructor/BFS/validator,
 #                          VERBOSE build so per-level directions are printed
+#   oracle/_ref/runnable_gpu
+#                          the reference driver mpi/main.cc with g5::BfsOnGPU
+#                          (include/g5_bfs.hpp over graph500_amd/lib/libg5.so)
+#                          in place of BfsOnCPU: sed rewrites the class name at
+#                          its three uses on the way into the compiler (no
+#                          edited copy is written anywhere); everything else --
+#                          generation, find_roots, the timing, get_pred, the
+#                          reference's own validate_bfs_result and result
+#                          block -- is the reference's code, 64 roots, VR=2
 REF ?= /root/reference
 MPIBIN ?= /opt/conda/bin
 HERE := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
@@ -22,7 +31,10 @@ BASE := -fopenmp -g -Drestrict=__restrict__ -D__STDC_CONSTANT_MACROS -D__STDC_LI
 RUNNABLE_FLAGS := $(BASE) -DVERVOSE_MODE=0 -DVERTEX_REORDERING=2 -DREAL_BENCHMARK
 PROBE_FLAGS := $(BASE) -DVERVOSE_MODE=1 -DVERTEX_REORDERING=2
 
-all: $(OUT)/runnable $(OUT)/refprobe $(OUT)/refprobe64
+INCLUDE := $(abspath $(HERE)../include)
+LIBDIR := $(abspath $(HERE)../graph500_amd/lib)
+
+all: $(OUT)/runnable $(OUT)/refprobe $(OUT)/refprobe64 $(OUT)/runnable_gpu
 
 $(OUT)/.dir:
 	mkdir -p $(OUT) && touch $@
@@ -50,5 +62,23 @@ $(OUT)/refprobe: $(OUT)/ref_probe.o $(OUT)/low_level_func.o $(OUT)/splittable_mr
 
 $(OUT)/refprobe64: $(OUT)/ref_probe64.o $(OUT)/low_level_func.o $(OUT)/splittable_mrg.o
 	$(MPICXX) -fopenmp -o $@ $^ -lm -lnuma
+
+# The drop-in swap: BfsOnCPU -> g5::BfsOnGPU (mpi/main.cc:49, :58).
+SWAP := -e 's/BfsOnCPU::printInformation/g5::BfsOnGPU::printInformation/' \
+        -e 's/BfsOnCPU\* benchmark = new BfsOnCPU()/g5::BfsOnGPU* benchmark = new g5::BfsOnGPU()/'
+
+$(OUT)/main_gpu.o: $(REF)/mpi/main.cc $(INCLUDE)/g5_bfs.hpp $(INCLUDE)/g5.h $(OUT)/.dir
+	sed $(SWAP) $< | grep -c 'g5::BfsOnGPU' | grep -qx 2
+	sed $(SWAP) $< | $(MPICXX) -x c++ -c $(RUNNABLE_FLAGS) -I$(REF)/mpi -I$(INCLUDE) -include g5_bfs.hpp - -o $@
+
+# Linked by g++ itself with the MPI libraries named: the MPI wrapper would put
+# its own (older) libstdc++ first, which the ROCm runtime under libg5.so
+# cannot use; the system one comes first in the run path.
+MPILIB := $(abspath $(MPIBIN)/../lib)
+$(OUT)/runnable_gpu: $(OUT)/main_gpu.o $(OUT)/low_level_func.o $(OUT)/splittable_mrg.o $(LIBDIR)/libg5.so
+	g++ -fopenmp -o $@ $(OUT)/main_gpu.o $(OUT)/low_level_func.o $(OUT)/splittable_mrg.o \
+	  -L$(LIBDIR) -lg5 $(MPILIB)/libmpicxx.so $(MPILIB)/libmpi.so \
+	  -Wl,-rpath,'$$ORIGIN/../../graph500_amd/lib:/usr/lib/x86_64-linux-gnu:$(MPILIB)' \
+	  -Wl,-rpath-link,/opt/rocm/lib -lm -lnuma
 
 .PHONY: all
