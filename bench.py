@@ -41,6 +41,8 @@ single-root runs.
 Other modes:
   --copy-calibration      a plain 1 GiB device copy (for scaling rocprof
                           FETCH_SIZE/WRITE_SIZE counters)
+  --gather-calibration    2^28 random 8-byte reads over a 4 GiB array (what the
+                          same counters read per random 8-byte miss)
   --rocprof-summary DB    per-kernel time, or per-kernel counter averages, of a
                           rocprofv3 results database (the text kept in profiles/)
   --rocprof-dispatches DB KERNEL
@@ -78,10 +80,36 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_info():
+    """Host facts the CPU baseline is quoted with: the CPU model, nproc, and
+    the cores this process may use (its affinity mask)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0))}
+
+
+def baseline_threads():
+    """All cores of this process's share: its affinity mask, capped by
+    OMP_NUM_THREADS where the host sets one (a GPU box gives each GPU's jobs
+    a share of its cores and sets OMP_NUM_THREADS to it)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
 def cpu_baseline(threads, scale=20, timeout=150):
     """Reference runnable at a bounded SCALE; falls back to the oracle port."""
     exe = os.path.join(HERE, "oracle", "_ref", "runnable")
     mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
+    host = cpu_info()
     if os.path.exists(exe) and os.path.exists(mpirun):
         try:
             env = dict(os.environ, OMP_NUM_THREADS=str(threads))
@@ -91,18 +119,26 @@ def cpu_baseline(threads, scale=20, timeout=150):
                                env=env)
             m = re.search(r"harmonic_mean_TEPS:\s+([0-9.eE+-]+)", r.stdout)
             if r.returncode == 0 and m:
-                mt = re.search(r"mean_time:\s+([0-9.eE+-]+)", r.stdout)
-                return {
+                keys = {}
+                for k in ("mean_time", "median_TEPS", "construction_time", "graph_generation", "mean_validate"):
+                    mk = re.search(r"^%s:\s+([0-9.eE+-]+)" % k, r.stdout, re.M)
+                    keys[k] = float(mk.group(1)) if mk else None
+                out = {
                     "value": float(m.group(1)) / 1e9,
                     "unit": "GTEPS",
                     "cores": threads,
                     "kind": "reference",
                     "sample": ("reference runnable (VERTEX_REORDERING=2, 64 roots), mpirun -np 1, "
-                               "OMP_NUM_THREADS=%d, Kronecker SCALE %d ef16; harmonic-mean TEPS of "
-                               "its own timed run_bfs calls; mean_time %.4g s; wall %.1f s"
-                               % (threads, scale, float(mt.group(1)) if mt else float("nan"),
-                                  time.time() - t0)),
+                               "OMP_NUM_THREADS=%d (every core of this job's share), Kronecker SCALE %d ef16; "
+                               "harmonic-mean TEPS of its own timed run_bfs calls (mpi/main.cc:113-117); "
+                               "wall %.1f s including generation, construction and validation. SCALE %d keeps "
+                               "the sample within the bench's budget: the 1-process reference aborts beyond "
+                               "SCALE 24 and needs minutes there" % (threads, scale, time.time() - t0, scale)),
+                    "scale": scale,
                 }
+                out.update(keys)
+                out.update(host)
+                return out
             log("reference runnable failed rc=%s: %s" % (r.returncode, r.stderr[-400:]))
         except Exception as ex:  # noqa: BLE001
             log("reference runnable unavailable: %r" % (ex,))
@@ -117,8 +153,10 @@ def cpu_baseline(threads, scale=20, timeout=150):
         t0 = time.perf_counter()
         _, _, tr = c.bfs(int(r), want_pred=True)
         spe.append((time.perf_counter() - t0) / tr["teps_edges"])
-    return {"value": len(spe) / sum(spe) / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
-            "sample": "oracle BFS (single thread) over 16 roots at SCALE %d ef16" % scale}
+    out = {"value": len(spe) / sum(spe) / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+           "sample": "oracle BFS (single thread) over 16 roots at SCALE %d ef16" % scale, "scale": scale}
+    out.update(host)
+    return out
 
 
 def copy_calibration():
@@ -130,6 +168,28 @@ def copy_calibration():
     b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+
+
+def gather_calibration(array_bytes=4 << 30, gathers=1 << 28):
+    """One launch of k_measure_gather (libg5.so, include/g5_measure.h):
+    `gathers` random 8-byte reads over an array of `array_bytes`, far larger
+    than the last-level cache, so nearly every read misses; read under
+    rocprofv3 --pmc, it tells what the traffic counters report per random
+    8-byte miss."""
+    import ctypes
+    from graph500_amd import g5
+    lib = g5.load_library()
+    ms = ctypes.c_double(0)
+    chk = ctypes.c_uint64(0)
+    rc = lib.g5_measure_gather(0, ctypes.c_int64(array_bytes), ctypes.c_int64(gathers), ctypes.byref(ms),
+                               ctypes.byref(chk))
+    if rc != 0:
+        raise RuntimeError("g5_measure_gather failed: %d" % rc)
+    rate = gathers / (ms.value / 1e3)
+    print(json.dumps({"kernel": "k_measure_gather", "array_bytes": array_bytes, "gathers": gathers,
+                      "ms": round(ms.value, 4), "gathers_per_s": rate,
+                      "gbs_if_64B_per_gather": round(rate * 64 / 1e9, 1),
+                      "gbs_if_128B_per_gather": round(rate * 128 / 1e9, 1), "checksum": chk.value}))
 
 
 def rocprof_summary(db, top=30):
@@ -239,7 +299,14 @@ def single_root_leg(ctx, roots, warmup):
     }
 
 
-def serve_leg(scales, rates, max_queries, edgefactor, seed=1):
+# Serving policy: a graph with searches in flight admits at a level boundary
+# once min(min_admit, free lanes, free answer arrays) queries can start
+# together or the oldest has waited max_wait_s (continuous batching); each
+# search pulls while its frontier's edges exceed 1/alpha of its unreached edges.
+SERVE_POLICY = {"answer_buffers": 128, "min_admit": 8, "max_wait_s": 0.005, "lane_rule": 3, "alpha": 64.0}
+
+
+def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):
     """Serving loop over resident graphs: Poisson arrivals at each rate, each
     query on graph s with probability proportional to 2^-((27 - s) / 2) and a
     uniform random root with edges.  Per rate: latency p50/p99 (arrival to
@@ -264,7 +331,7 @@ def serve_leg(scales, rates, max_queries, edgefactor, seed=1):
         cand = rng.integers(0, n, size=min(n, 200000))
         has.append(np.array([v for v in cand[:20000] if c.has_edge(int(v))], np.int64))
     runs = []
-    policy = {"answer_buffers": 128, "min_admit": 64, "max_wait_s": 0.3, "lane_rule": 3, "alpha": 64.0}
+    policy = dict(SERVE_POLICY, **(policy or {}))
     with graph500_amd.G5Server(ctxs, **policy) as srv:
         for rate in rates:
             nq = int(min(max_queries, max(64, rate * 1.0)))
@@ -287,7 +354,8 @@ def serve_leg(scales, rates, max_queries, edgefactor, seed=1):
                    "p99_ms": round(float(np.percentile(lat, 99)), 3),
                    "served_qps": round(nq / rep["wall_s"], 1),
                    "gteps": round(rep["edges"] / rep["wall_s"] / 1e9, 2),
-                   "max_lanes": rep["max_active_lanes"], "sample_bit_exact": ok}
+                   "max_lanes": rep["max_active_lanes"], "admitted_busy": rep["admitted_busy"],
+                   "admit_events": rep["admit_events"], "sample_bit_exact": ok}
             log("serve %s" % run)
             runs.append(run)
     for c in ctxs:
@@ -295,7 +363,8 @@ def serve_leg(scales, rates, max_queries, edgefactor, seed=1):
     stated = next((r for r in runs if r["rate_qps"] == STATED_LOAD_QPS), None)
     return {"graphs_scale": list(scales), "policy": policy, "stated_load_qps": STATED_LOAD_QPS,
             "p50_ms_at_stated_load": stated["p50_ms"] if stated else None,
-            "p99_ms_at_stated_load": stated["p99_ms"] if stated else None, "runs": runs}
+            "p99_ms_at_stated_load": stated["p99_ms"] if stated else None,
+            "admitted_busy_at_stated_load": stated["admitted_busy"] if stated else None, "runs": runs}
 
 
 def main():
@@ -305,7 +374,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=28)
     ap.add_argument("--edgefactor", type=int, default=16)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core of this job's share")
     ap.add_argument("--cpu-scale", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single-root", action="store_true")
@@ -315,8 +384,12 @@ def main():
     ap.add_argument("--serve-rates", type=lambda x: [float(v) for v in x.split(",")],
                     default=[100.0, 300.0, 1000.0, 3000.0, 20000.0])
     ap.add_argument("--serve-max-queries", type=int, default=3000)
+    ap.add_argument("--serve-policy", type=json.loads, default=None,
+                    help="serving policy overrides as JSON, e.g. {\"min_admit\": 16}")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving leg (no SCALE 28 graph)")
     ap.add_argument("--autotune-budget", type=float, default=30.0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--gather-calibration", action="store_true")
     ap.add_argument("--rocprof-summary", metavar="DB")
     ap.add_argument("--rocprof-dispatches", nargs=2, metavar=("DB", "KERNEL"))
     ap.add_argument("--single-root-only", type=int, metavar="N")
@@ -333,10 +406,19 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return 0
+    if a.gather_calibration:
+        gather_calibration()
+        return 0
     if a.gpus != 1:
         print("bench.py measures one MI355X only: this build has no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         return 2
+
+    if a.serve_only:
+        print(json.dumps({"mode": "serve only", "serve": serve_leg(a.serve_scales, a.serve_rates,
+                                                                    a.serve_max_queries, a.edgefactor,
+                                                                    a.serve_policy)}))
+        return 0
 
     import graph500_amd
     ctx = graph500_amd.G5Context(0)
@@ -459,10 +541,10 @@ def main():
     ctx.batch_end()
     ctx.close()
     if not a.no_serve:
-        out["serve"] = serve_leg(a.serve_scales, a.serve_rates, a.serve_max_queries, a.edgefactor)
+        out["serve"] = serve_leg(a.serve_scales, a.serve_rates, a.serve_max_queries, a.edgefactor, a.serve_policy)
     if not a.no_cpu_baseline:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(a.cpu_threads, a.cpu_scale)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_threads or baseline_threads(), a.cpu_scale)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

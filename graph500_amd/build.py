@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 SOURCES = ["g5_gen.hip", "g5_prims.hip", "g5_csr.hip", "g5_bfs.hip", "g5_msbfs.hip", "g5_validate.hip",
-           "g5_serve.hip", "g5_api.hip"]
+           "g5_serve.hip", "g5_api.hip", "g5_measure.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",
