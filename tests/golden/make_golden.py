@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--ref", default="/root/reference")
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("scales", nargs="*", type=int, default=[16, 20])
+    ap.add_argument("--roots64", type=int, nargs="*", default=[16],
+                    help="SCALEs that also get a 64-root fixture (REAL_BENCHMARK root count)")
+    ap.add_argument("--only64", action="store_true", help="write only the 64-root fixtures")
     ap.add_argument("--spec2012", type=int, nargs="*", default=[],
                     help="SCALEs for spec-2012 fixtures (initiator 5500/100, binary-tree initial edges)")
     a = ap.parse_args()
@@ -100,7 +103,9 @@ def main():
               open(os.path.join(HERE, "mrg_table.json"), "w"), indent=1)
     for s in a.scales:
         for roots64 in (False, True):
-            if roots64 and s > 16:
+            if roots64 and s not in a.roots64:
+                continue
+            if not roots64 and a.only64:
                 continue
             d = run_probe(s, roots64, a.threads)
             res = summarize(s, d, 64 if roots64 else 16)

@@ -269,6 +269,49 @@ def test_statistics_formulas(oracle_mod):
     assert s["q1"] == 2 and s["q3"] == 4
 
 
+@pytest.mark.parametrize("scale", [8, 10])
+def test_single_root_bytes_brute_force(oracle_mod, scale):
+    """The single-root byte model of the oracle (the figure the GPU's
+    g5_bfs_model_bytes is tested equal to) against a restatement from the
+    definition, given the run's levels and schedule: a top-down level costs
+    sum over its frontier of (4 + 8 + 4 deg) plus 4 per discovered vertex; a
+    bottom-up level 3 n / 8 plus, per vertex with edges not visited before
+    it, 8 + 4 k with k the neighbours probed in ascending order up to and
+    including the first one visited before the level (the whole row if
+    none); every level adds 1 byte per discovered vertex, and the run 1 byte
+    for the root."""
+    O = oracle_mod
+    c = O.Csr(O.generate_edges(scale))
+    a = c.arrays()
+    off, cols, new2old = a["offsets"], a["cols"], a["new2old"]
+    n, na = c.num_verts, c.num_active
+    for r in c.find_roots(4):
+        lv_orig, _, tr = c.bfs(int(r), want_pred=False)
+        lv = lv_orig[new2old].astype(np.int64)  # by new id
+        lv[lv == 0xFF] = 1 << 20
+        total = 1.0
+        for L, d in enumerate(tr["direction"], start=1):
+            found = int((lv == L).sum())
+            if d:  # top-down
+                front = np.flatnonzero(lv[:na] == L - 1)
+                b = sum(12.0 + 4.0 * (off[u + 1] - off[u]) for u in front) + 4.0 * found
+            else:
+                b = 3.0 * n / 8.0
+                for v in range(na):
+                    if lv[v] < L:
+                        continue
+                    row = cols[off[v]:off[v + 1]]
+                    hit = np.flatnonzero(lv[row] < L)
+                    k = int(hit[0]) + 1 if len(hit) else len(row)
+                    b += 8.0 + 4.0 * k
+            b += found
+            assert tr["bytes"][L - 1] == pytest.approx(b), (int(r), L, d)
+            total += b
+        assert tr["total_bytes"] == pytest.approx(total)
+        # a bottom-up level probes each found vertex's row up to a visited neighbour only
+        assert any(d == 0 for d in tr["direction"]) or scale < 10
+
+
 @pytest.mark.parametrize("masked", [False, True])
 def test_msbfs_bytes_brute_force(oracle_mod, masked):
     """The batched byte model against a set-based restatement; masked: with
