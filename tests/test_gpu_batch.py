@@ -221,9 +221,11 @@ def test_batch_partial_and_odd_roots(oracle_mod, mode, rule):
                                                    (20, 0.0, 0, 0.0), (20, 0.0, 1, 0.0), (24, 0.0, 1, 50.0)])
 def test_batch_matches_single_root_gpu(scale, sweep, rule, beta):
     """beta 0 lets small tail frontiers push again after the list-driven
-    tail pulls (whose frontier words and nonzero bits the push then reads)."""
+    tail pulls (whose frontier words and nonzero bits the push then reads).
+    At SCALE 24 every one of the 64 lanes is checked against the reference's
+    64-root run."""
     import graph500_amd
-    g = gold("ref_s%d_16roots.json" % scale)
+    g = gold("ref_s24_64roots.json" if scale == 24 else "ref_s%d_16roots.json" % scale)
     with graph500_amd.G5Context(0) as c:
         c.generate_kronecker(scale)
         c.construct(2, release_edges=True)
@@ -233,11 +235,12 @@ def test_batch_matches_single_root_gpu(scale, sweep, rule, beta):
         say("batch s%d sweep %g lane_rule %d beta %g" % (scale, sweep, rule, beta))
         c.run_bfs_batch(roots)
         teps = c.batch_teps_edges()
+        assert roots.tolist()[:len(g["roots"])] == g["roots"]
         for i, pr in enumerate(g["per_root"]):
             assert sha(c.batch_levels(i)) == pr["levels_sha256"]
             assert teps[i] == pr["nedge"]
         c.prepare_bfs()
-        for i in range(16, 64, 7):
+        for i in range(len(g["per_root"]), 64, 7):
             c.run_bfs(int(roots[i]))
             assert np.array_equal(c.get_levels(), c.batch_levels(i)), i
             assert c.teps_edges() == teps[i]

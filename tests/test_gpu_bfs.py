@@ -178,6 +178,32 @@ def test_bfs_reference_fixtures_larger_scales(scale, tuning, oracle_mod):
                 assert cnt == pr["nedge"]
 
 
+def test_bfs_reference_64_roots_s24():
+    """SCALE 24 with the 64 roots of the reference's REAL_BENCHMARK run
+    (tests/golden/ref_s24_64roots.json): roots, level hashes, schedules,
+    frontier sizes and traversed-edge counts of every root; parents
+    validated on the GPU."""
+    import graph500_amd
+    g = gold("ref_s24_64roots.json")
+    say("scale 24, 64 reference roots")
+    with graph500_amd.G5Context(0) as c:
+        c.generate_kronecker(24, 16, 2, 3)
+        c.construct(2, release_edges=True)
+        info = c.graph_info()
+        assert info["num_active"] == g["num_active"] and info["nnz"] == g["nnz"]
+        assert c.find_roots(64).tolist() == g["roots"]
+        c.prepare_bfs()
+        for pr in g["per_root"]:
+            c.run_bfs(pr["root"])
+            assert sha(c.get_levels()) == pr["levels_sha256"], pr["root"]
+            st = c.bfs_stats()
+            assert "".join("T" if d else "B" for d in st["direction"]) == pr["directions"], pr["root"]
+            assert st["nq"] == pr["nq"]
+            assert c.teps_edges() == pr["nedge"] == 268432547
+            ok, errs = c.validate()
+            assert ok, (pr["root"], errs)
+
+
 def _run_small(oracle_mod, v0, v1, roots, modes=(2, 1, 0)):
     import graph500_amd
     e = oracle_mod.pack_edges(v0, v1)
