@@ -403,6 +403,19 @@ int g5_validate_pred(g5_ctx* ctx, int64_t root, const int64_t* pred, int64_t* er
   });
 }
 
+int g5_validate_pred_noalloc(g5_ctx* ctx, int64_t root, const int64_t* pred, int64_t* errors, int* valid,
+                             int64_t* edge_visit_count) {
+  return guarded(ctx, [&] {
+    G5_CHECK(pred && errors && valid, "g5_validate_pred_noalloc: null argument");
+    set_device(ctx);
+    const int64_t visited = pred_validate_noalloc(ctx, root, pred, errors);
+    *valid = 1;
+    for (int i = 0; i < 6; ++i)
+      if (errors[i]) *valid = 0;
+    if (edge_visit_count) *edge_visit_count = visited;
+  });
+}
+
 int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* t) {
   return guarded(ctx, [&] {
     G5_CHECK(t && t->bin_edges >= 0 && t->td_pieces >= 0 && t->td_pieces <= 4096 &&

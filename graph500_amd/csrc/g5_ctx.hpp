@@ -267,6 +267,7 @@ void bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
 // g5_validate.hip
 void bfs_validate(g5_ctx* ctx, int64_t* errors_out);
 int64_t pred_validate(g5_ctx* ctx, int64_t root, const int64_t* host_pred, int64_t* errors_out);
+int64_t pred_validate_noalloc(g5_ctx* ctx, int64_t root, const int64_t* host_pred, int64_t* errors_out);
 // g5_msbfs.hip
 void ms_prepare(g5_ctx* ctx);
 void ms_alloc_compact(g5_ctx* ctx);

@@ -18,6 +18,8 @@
 #include "g5_ctx.hpp"
 #include "g5_frontier.hpp"
 
+#include <mutex>
+
 namespace g5 {
 
 enum { V_ROOT = 0, V_SELF, V_DEPTH, V_EDGE, V_PARENT_EDGE, V_RANGE, V_NUM = 8 };
@@ -131,14 +133,13 @@ __global__ void __launch_bounds__(kBlock) k_val_edges(const uint8_t* __restrict_
   block_add(bad, err + V_EDGE);
 }
 
+// err: V_NUM device counters (cleared here when clear is set).
 static void run_checks(g5_ctx* ctx, const uint8_t* level, const uint32_t* parent, int64_t root_orig,
-                       bool clear, int64_t* errors_out) {
+                       bool clear, unsigned long long* err, int64_t* errors_out) {
   Graph& g = ctx->graph;
   hipStream_t s = ctx->stream;
-  ctx->acc.alloc(65);
-  unsigned long long* err = (unsigned long long*)ctx->acc.p;
   const int64_t rows = g.n_rows;
-  if (clear) G5_HIP(hipMemsetAsync(ctx->acc.p, 0, V_NUM * 8, s));
+  if (clear) G5_HIP(hipMemsetAsync(err, 0, V_NUM * 8, s));
   // a root that is no CSR row (no edges, modes 1 and 2) reaches nothing else
   uint32_t root_new = 0;
   G5_HIP(hipMemcpyAsync(&root_new, g.old2new.p + root_orig, 4, hipMemcpyDeviceToHost, s));
@@ -151,7 +152,7 @@ static void run_checks(g5_ctx* ctx, const uint8_t* level, const uint32_t* parent
                      g.offsets.p, g.cols.p, rows, err);
   G5_HIP(hipGetLastError());
   uint64_t h[V_NUM];
-  G5_HIP(hipMemcpyAsync(h, ctx->acc.p, V_NUM * 8, hipMemcpyDeviceToHost, s));
+  G5_HIP(hipMemcpyAsync(h, err, V_NUM * 8, hipMemcpyDeviceToHost, s));
   G5_HIP(hipStreamSynchronize(s));
   for (int i = 0; i < 6; ++i) errors_out[i] = (int64_t)h[i];
 }
@@ -159,32 +160,103 @@ static void run_checks(g5_ctx* ctx, const uint8_t* level, const uint32_t* parent
 void bfs_validate(g5_ctx* ctx, int64_t* errors_out) {
   BfsState& b = ctx->bfs;
   G5_CHECK(b.have_result, "g5_validate: no BFS result (call g5_run_bfs first)");
-  run_checks(ctx, b.level.p, b.parent.p, b.root_orig, true, errors_out);
+  ctx->acc.alloc(65);
+  run_checks(ctx, b.level.p, b.parent.p, b.root_orig, true, (unsigned long long*)ctx->acc.p, errors_out);
+}
+
+// The checks of a caller's pred array (host memory, pred_size() entries) on
+// device buffers the caller of this function provides: dpred (n words, the
+// pred copy), lvl / par (n_rows each) and acc (V_NUM + 1 words: the error
+// counters, then the reduction word).  Returns the traversed-edge count of
+// the reached set.
+static int64_t pred_validate_on(g5_ctx* ctx, int64_t root, const int64_t* host_pred, int64_t* dpred, uint8_t* lvl,
+                                uint32_t* par, uint64_t* acc, int64_t* errors_out) {
+  Graph& g = ctx->graph;
+  hipStream_t s = ctx->stream;
+  const int64_t rows = g.n_rows;
+  G5_HIP(hipMemcpyAsync(dpred, host_pred, g.n * 8, hipMemcpyHostToDevice, s));
+  G5_HIP(hipMemsetAsync(acc, 0, (V_NUM + 1) * 8, s));
+  hipLaunchKernelGGL(k_val_from_pred, dim3(grid_of(g.n, kBlock, 8192)), dim3(kBlock), 0, s, dpred, g.n,
+                     g.old2new.p, rows, root, lvl, par, (unsigned long long*)acc);
+  G5_HIP(hipGetLastError());
+  run_checks(ctx, lvl, par, root, false, (unsigned long long*)acc, errors_out);
+  const uint64_t sum = device_reduce_sum(ReachedWeight{lvl, g.weight.p}, (size_t)rows, acc + V_NUM, s);
+  return (int64_t)(sum / 2);
+}
+
+static void pred_validate_args(g5_ctx* ctx, int64_t root, const char* who) {
+  Graph& g = ctx->graph;
+  G5_CHECK(g.built, std::string(who) + ": no graph (call g5_construct first)");
+  G5_CHECK(root >= 0 && root < g.n, std::string(who) + ": root out of range");
 }
 
 // Validates a caller's pred array (host memory, pred_size() entries) and
 // returns the traversed-edge count of its reached set.
 int64_t pred_validate(g5_ctx* ctx, int64_t root, const int64_t* host_pred, int64_t* errors_out) {
+  pred_validate_args(ctx, root, "g5_validate_pred");
   Graph& g = ctx->graph;
-  G5_CHECK(g.built, "g5_validate_pred: no graph (call g5_construct first)");
-  G5_CHECK(root >= 0 && root < g.n, "g5_validate_pred: root out of range");
-  hipStream_t s = ctx->stream;
-  const int64_t rows = g.n_rows;
   DevBuf<int64_t> dpred;
   DevBuf<uint8_t> lvl;
   DevBuf<uint32_t> par;
   dpred.alloc(g.n);
-  lvl.alloc(rows);
-  par.alloc(rows);
-  G5_HIP(hipMemcpyAsync(dpred.p, host_pred, g.n * 8, hipMemcpyHostToDevice, s));
-  ctx->acc.alloc(65);
-  G5_HIP(hipMemsetAsync(ctx->acc.p, 0, V_NUM * 8, s));
-  hipLaunchKernelGGL(k_val_from_pred, dim3(grid_of(g.n, kBlock, 8192)), dim3(kBlock), 0, s, dpred.p, g.n,
-                     g.old2new.p, rows, root, lvl.p, par.p, (unsigned long long*)ctx->acc.p);
-  G5_HIP(hipGetLastError());
-  run_checks(ctx, lvl.p, par.p, root, false, errors_out);
-  const uint64_t sum = device_reduce_sum(ReachedWeight{lvl.p, g.weight.p}, (size_t)rows, ctx->acc.p, s);
-  return (int64_t)(sum / 2);
+  lvl.alloc(g.n_rows);
+  par.alloc(g.n_rows);
+  ctx->acc.alloc(V_NUM + 1);
+  return pred_validate_on(ctx, root, host_pred, dpred.p, lvl.p, par.p, ctx->acc.p, errors_out);
+}
+
+// ---------------------------------------------------------------------------
+// The same checks without any allocation (g5_validate_pred_noalloc): every
+// device buffer above is carved from `scratch`, one file-scope array per GPU
+// that stays resident until the process ends.  Its host-side name is no
+// device address: the address each device holds it at comes from
+// hipGetSymbolAddress on the first call on that device (with the device
+// current) and is kept per device.  Calls that overlap on one GPU share it
+// and race; the counters are cleared by every call, and the level and parent
+// words are written for every row before they are read.
+// ---------------------------------------------------------------------------
+constexpr size_t kValScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kValScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_dev[kMaxDevices];  // device address of scratch, per device
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Bytes of scratch a graph of n ids and `rows` rows needs.
+static size_t noalloc_need(int64_t n, int64_t rows) {
+  return align256((V_NUM + 1) * 8) + align256((size_t)n * 8) + align256((size_t)rows) + align256((size_t)rows * 4);
+}
+
+static unsigned char* scratch_on(int device) {
+  G5_CHECK(device >= 0 && device < kMaxDevices, "g5_validate_pred_noalloc: device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    G5_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    G5_CHECK(p, "g5_validate_pred_noalloc: no device address for the scratch array");
+    g_scratch_dev[device] = (unsigned char*)p;
+  }
+  return g_scratch_dev[device];
+}
+
+int64_t pred_validate_noalloc(g5_ctx* ctx, int64_t root, const int64_t* host_pred, int64_t* errors_out) {
+  pred_validate_args(ctx, root, "g5_validate_pred_noalloc");
+  Graph& g = ctx->graph;
+  const size_t need = noalloc_need(g.n, g.n_rows);
+  G5_CHECK_CODE(need <= kValScratchBytes, G5_ERR_LIMIT,
+                "g5_validate_pred_noalloc: the graph needs " + std::to_string(need) + " bytes of scratch, more than " +
+                    std::to_string(kValScratchBytes) + " (at most about 2^22 vertex ids); use g5_validate_pred");
+  unsigned char* base = scratch_on(ctx->device);  // the caller made ctx->device current
+  uint64_t* acc = (uint64_t*)base;
+  size_t off = align256((V_NUM + 1) * 8);
+  int64_t* dpred = (int64_t*)(base + off);
+  off += align256((size_t)g.n * 8);
+  uint8_t* lvl = base + off;
+  off += align256((size_t)g.n_rows);
+  uint32_t* par = (uint32_t*)(base + off);
+  return pred_validate_on(ctx, root, host_pred, dpred, lvl, par, acc, errors_out);
 }
 
 }  // namespace g5

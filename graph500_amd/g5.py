@@ -38,7 +38,7 @@ EXPORTS = [
     "g5_generate_graph", "g5_construct",
     "g5_graph_info_get", "g5_has_edge", "g5_pred_size", "g5_find_roots", "g5_graph_download",
     "g5_prepare_bfs", "g5_run_bfs", "g5_get_pred", "g5_get_levels", "g5_teps_edges",
-    "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_bfs_set_tuning", "g5_bfs_get_tuning", "g5_validate", "g5_validate_pred", "g5_end_bfs",
+    "g5_bfs_stats_get", "g5_bfs_model_bytes", "g5_bfs_set_tuning", "g5_bfs_get_tuning", "g5_validate", "g5_validate_pred", "g5_validate_pred_noalloc", "g5_end_bfs",
     "g5_batch_prepare", "g5_batch_set_tuning", "g5_batch_get_tuning", "g5_run_bfs_batch",
     "g5_batch_get_levels", "g5_batch_teps_edges", "g5_batch_stats_get", "g5_batch_model_bytes",
     "g5_batch_level_lanes",
@@ -182,6 +182,7 @@ def load_library(path=LIB_PATH):
         "g5_bfs_set_tuning": ([vp, vp], i32),
         "g5_bfs_get_tuning": ([vp, vp], i32),
         "g5_validate_pred": ([vp, i64, vp, vp, ctypes.POINTER(i32), ctypes.POINTER(i64)], i32),
+        "g5_validate_pred_noalloc": ([vp, i64, vp, vp, ctypes.POINTER(i32), ctypes.POINTER(i64)], i32),
         "g5_end_bfs": ([vp], i32),
         "g5_batch_prepare": ([vp], i32),
         "g5_batch_set_tuning": ([vp, ctypes.POINTER(BatchTuning)], i32),
@@ -422,6 +423,19 @@ class G5Context:
         visited = ctypes.c_int64(0)
         self._check(self._lib.g5_validate_pred(self._h, int(root), _ptr(pred), _ptr(err), ctypes.byref(ok),
                                                ctypes.byref(visited)))
+        return bool(ok.value), dict(zip(self.VALIDATION_RULES, err.tolist())), int(visited.value)
+
+    def validate_pred_noalloc(self, root, pred):
+        """validate_pred through g5_validate_pred_noalloc (no device allocation;
+        graphs of up to about 2^22 vertex ids)."""
+        pred = np.ascontiguousarray(pred, dtype=np.int64)
+        if pred.shape != (self.pred_size(),):
+            raise ValueError("pred must hold pred_size() = %d entries" % self.pred_size())
+        err = np.zeros(6, np.int64)
+        ok = ctypes.c_int(0)
+        visited = ctypes.c_int64(0)
+        self._check(self._lib.g5_validate_pred_noalloc(self._h, int(root), _ptr(pred), _ptr(err), ctypes.byref(ok),
+                                                       ctypes.byref(visited)))
         return bool(ok.value), dict(zip(self.VALIDATION_RULES, err.tolist())), int(visited.value)
 
     def end_bfs(self):

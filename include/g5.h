@@ -271,6 +271,17 @@ int g5_validate(g5_ctx* ctx, int64_t* errors, int* valid);
  * count of its reached set in *edge_visit_count (may be NULL). */
 int g5_validate_pred(g5_ctx* ctx, int64_t root, const int64_t* pred, int64_t* errors, int* valid,
                      int64_t* edge_visit_count);
+/* g5_validate_pred without any device allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): its staging, level, parent and counter buffers live in one
+ * 64 MiB array per GPU that the library holds until the process ends, so it
+ * serves graphs of up to about 2^22 vertex ids (pred_size() * 8 + 5 bytes
+ * per row, plus alignment, within 64 MiB); larger graphs get G5_ERR_LIMIT.
+ * Same checks, outputs and errors as g5_validate_pred.  Calls that overlap on
+ * one GPU share that array and race: serialize them. */
+int g5_validate_pred_noalloc(g5_ctx* ctx, int64_t root, const int64_t* pred, int64_t* errors, int* valid,
+                             int64_t* edge_visit_count);
 /* end_bfs (mpi/bfs.hpp:126-128): frees BFS state. */
 int g5_end_bfs(g5_ctx* ctx);
 
