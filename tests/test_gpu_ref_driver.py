@@ -48,7 +48,9 @@ def test_reference_main_over_bfs_on_gpu(scale):
     assert int(kv["SCALE"]) == scale
     assert float(kv["min_nedge"]) == float(kv["max_nedge"]) == PF_NEDGE[scale]
     assert float(kv["harmonic_mean_TEPS"]) > 0
-    # one validated count per root, from the reference's own validator
-    counts = [int(x) for x in re.findall(r"Number of traversed edges is (\d+)", out)]
+    # one validated count per root, from the reference's own validator (its
+    # progress lines go to stderr, the result block to stdout)
+    log = out + r.stderr
+    counts = [int(x) for x in re.findall(r"Number of traversed edges is (\d+)", log)]
     assert counts == [PF_NEDGE[scale]] * 64
-    assert len(re.findall(r"Validate time for BFS \d+ is", out)) == 64
+    assert len(re.findall(r"Validate time for BFS \d+ is", log)) == 64

@@ -264,6 +264,64 @@ def test_reference_roots_served_in_a_mix():
             c.close()
 
 
+PF_NEDGE = {18: 4194250, 22: 67108130, 25: 536865258, 27: 2147462776}  # mpi/utils.hpp:3664-3708
+
+
+def test_serve_config5_graphs_resident():
+    """The serving configuration's graphs (SCALE 18, 22, 25 and 27, edgefactor
+    16) resident together; 400 Poisson queries at 1,000 per second, graph
+    picked with probability proportional to 2^-((27 - SCALE) / 2), uniform
+    roots with edges.  Six kept answers per graph equal the single-root level
+    array of their root, whose parents pass every validator rule on the GPU;
+    the traversed-edge count equals the single-root run's, which is
+    pf_nedge[SCALE] for a root in the giant component."""
+    import graph500_amd
+    scales = (18, 22, 25, 27)
+    ctxs = []
+    try:
+        for sc in scales:
+            say("serve config graph S%d" % sc)
+            c = graph500_amd.G5Context(0)
+            c.generate_kronecker(sc, 16, 2, 3)
+            c.construct(2, release_edges=True)
+            c.prepare_bfs()
+            ctxs.append(c)
+        w = np.array([2.0 ** (-(27 - sc) / 2.0) for sc in scales])
+        w /= w.sum()
+        rng = np.random.default_rng(27)
+        nq = 400
+        graph = rng.choice(len(scales), size=nq, p=w)
+        arrival = np.cumsum(rng.exponential(1e-3, size=nq))
+        keep = set()
+        for gi in range(len(scales)):
+            idx = np.flatnonzero(graph == gi)
+            keep |= set(idx[:: max(1, len(idx) // 6)][:6].tolist())
+        specs = [(int(graph[i]), _roots_with_edges(ctxs[graph[i]], 1, rng)[0], float(arrival[i]),
+                  1 if i in keep else 0) for i in range(nq)]
+        with graph500_amd.G5Server(ctxs) as srv:
+            say("serve config: %d queries, %d kept" % (nq, len(keep)))
+            res, rep = srv.run(_queries(specs), timeout_s=120)
+            assert (res["status"] >= 2).all()
+            say("serve config: admitted into busy graphs %d, admit events %d" % (rep["admitted_busy"],
+                                                                                 rep["admit_events"]))
+            for i in sorted(keep):
+                gi, r = specs[i][0], specs[i][1]
+                c = ctxs[gi]
+                lv = srv.levels(res["request"][i], c.pred_size())
+                c.run_bfs(r)
+                ok, errs = c.validate()
+                assert ok, (scales[gi], r, errs)
+                assert np.array_equal(lv, c.get_levels()), (scales[gi], r)
+                e = c.teps_edges()
+                assert res["edges"][i] == e, (scales[gi], r)
+                if e > PF_NEDGE[scales[gi]] // 2:  # the giant component
+                    assert e == PF_NEDGE[scales[gi]], (scales[gi], r)
+                srv.release(res["request"][i])
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 def test_serve_errors(graphs):
     import graph500_amd
     ctxs, _ = graphs
