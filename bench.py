@@ -299,11 +299,12 @@ def single_root_leg(ctx, roots, warmup):
     }
 
 
-# Serving policy: a graph with searches in flight admits at a level boundary
-# once min(min_admit, free lanes, free answer arrays) queries can start
-# together or the oldest has waited max_wait_s (continuous batching); each
-# search pulls while its frontier's edges exceed 1/alpha of its unreached edges.
-SERVE_POLICY = {"answer_buffers": 128, "min_admit": 8, "max_wait_s": 0.005, "lane_rule": 3, "alpha": 64.0}
+# Serving policy (continuous batching): a graph with searches in flight admits
+# waiting queries into its free lanes at a level boundary while every search
+# in flight still pushes, once min(min_admit, free answer arrays) queries can
+# start together, or once the oldest has waited max_wait_s; each search pulls
+# while its frontier's edges exceed 1/alpha of its unreached edges.
+SERVE_POLICY = {"answer_buffers": 128, "min_admit": 64, "max_wait_s": 0.3, "lane_rule": 3, "alpha": 64.0}
 
 
 def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):

@@ -29,8 +29,10 @@ DEFAULT_CACHE = os.environ.get("G5_TUNING_CACHE",
 # "a,b" takes pairs: the direction rule and its alpha are tried together, as
 # alpha means a different threshold under each rule.
 SPACE = [
+    ("mixed_push", [0, 1]),
     ("lane_rule,alpha", [(0, 1.0), (0, 2.0), (0, 4.0), (0, 8.0), (0, 16.0), (0, 64.0),
-                         (1, 32.0), (1, 64.0), (1, 96.0), (1, 128.0), (1, 192.0), (1, 256.0)]),
+                         (1, 16.0), (1, 24.0), (1, 32.0), (1, 48.0), (1, 64.0), (1, 96.0), (1, 128.0), (1, 192.0),
+                         (1, 256.0)]),
     ("persistent", [0, 1]),
     ("sweep_below", [0.0, 0.25]),
     ("beta", [0.0, 20.0, 50.0, 200.0]),
@@ -38,7 +40,7 @@ SPACE = [
     ("filter_from", [0.001, 0.01, 0.05]),
     ("compact_below", [0.0, 0.25, 0.5]),
     ("pull_grid", [4096, 8192, 16384]),
-    ("pull_unroll", [0, 1, 2, 3, 4]),
+    ("pull_unroll", [0, 1, 2, 3, 4, 5, 6, 7]),
     ("persist_waves", [4, 6, 8]),
 ]
 

@@ -470,8 +470,9 @@ int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* t) {
                  t->persist_grid <= (1 << 20) &&
                  (t->persist_waves == 4 || t->persist_waves == 6 || t->persist_waves == 8) &&
                  t->pull_filter >= 0 && t->pull_filter <= 2 && t->beta >= 0 && t->compact_below >= 0 &&
-                 t->filter_from >= 0 && t->filter_from <= 1 && t->pull_unroll >= 0 && t->pull_unroll <= 4 &&
-                 (t->lane_rule == 0 || t->lane_rule == 1),
+                 t->filter_from >= 0 && t->filter_from <= 1 && t->pull_unroll >= 0 && t->pull_unroll <= 7 &&
+                 (t->lane_rule == 0 || t->lane_rule == 1) && (t->mixed_push == 0 || t->mixed_push == 1) &&
+                 t->reserved == 0,
              "g5_batch_set_tuning: bad tuning");
     ctx->ms.persistent = t->persistent;
     ctx->ms.persist_levels = t->persist_levels;
@@ -483,6 +484,7 @@ int g5_batch_set_tuning(g5_ctx* ctx, const g5_batch_tuning* t) {
     ctx->ms.filter_from = t->filter_from;
     ctx->ms.pull_unroll = t->pull_unroll;
     ctx->ms.lane_rule = t->lane_rule;
+    ctx->ms.mixed_push = t->mixed_push;
     ctx->ms.alpha = t->alpha;
     ctx->ms.pull_grid = t->pull_grid;
     ctx->ms.sweep_below = t->sweep_below;
@@ -512,6 +514,8 @@ int g5_batch_get_tuning(g5_ctx* ctx, g5_batch_tuning* t) {
     t->filter_from = ctx->ms.filter_from;
     t->pull_unroll = ctx->ms.pull_unroll;
     t->lane_rule = ctx->ms.lane_rule;
+    t->mixed_push = ctx->ms.mixed_push;
+    t->reserved = 0;
   });
 }
 

@@ -141,8 +141,9 @@ struct MsState {
   int pull_filter = 0;         // pull tests the nonzero bitmap before a frontier word: 0 never, 1 always,
                                // 2 for vertices from filter_from x rows on
   double filter_from = 0.01;
-  int pull_unroll = 0;         // probe unrolling of the per-level pull (0..4, see g5_msbfs.hip)
+  int pull_unroll = 0;         // probe unrolling of the per-level pulls (0..7, see g5_msbfs.hip)
   int lane_rule = 1;           // 0: one direction per level for every root; 1: per root (per-level launches)
+  int mixed_push = 1;          // push of a mixed level: 0 over CSR chunks, 1 over a queue of the frontier vertices
   uint64_t level_push[G5_MAX_LEVELS] = {};  // per level: roots pushed, roots pulled (per-root rule)
   uint64_t level_pull[G5_MAX_LEVELS] = {};
   bool level_masks = false;    // the last run used the per-root rule

@@ -29,6 +29,19 @@ struct FrontierDegree {
   }
 };
 
+// The same with the frontier's length read on the device (a queue whose
+// count a scan wrote there); the scan may run over an upper bound.
+struct FrontierDegreeN {
+  const uint32_t* queue;
+  const uint64_t* offsets;
+  const uint64_t* nf;
+  __device__ uint64_t operator()(size_t i) const {
+    if (i >= *nf) return 0;
+    const uint32_t u = queue[i];
+    return offsets[u + 1] - offsets[u];
+  }
+};
+
 __device__ __forceinline__ bool bit_set(const uint64_t* __restrict__ bm, uint32_t v) {
   return (bm[v >> 6] >> (v & 63)) & 1;
 }

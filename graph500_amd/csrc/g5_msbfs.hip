@@ -78,9 +78,12 @@ __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ chunk_rows,
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ cols,
-                                                    const uint64_t* __restrict__ front, uint64_t* next) {
-  // front[u] goes to every target; lanes the target has seen are dropped
-  // when k_ms_finish folds next into seen
+                                                    const uint64_t* __restrict__ front, uint64_t* next,
+                                                    uint64_t lanes) {
+  // front[u] (its lanes in `lanes`) goes to every target; lanes the target
+  // has seen are dropped when k_ms_finish (or the merging pull) folds next
+  // into seen.  prefix[nf] is the edge count: nf may be an upper bound whose
+  // tail entries have degree 0.
   __shared__ ChunkRows cr;
   const uint64_t E = prefix[nf];
   for (uint64_t chunk = blockIdx.x; chunk * kEpb < E; chunk += gridDim.x) {
@@ -88,7 +91,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__
     const uint64_t e1 = (e0 + kEpb < E) ? e0 + kEpb : E;
     const int cnt = stage_chunk(chunk, chunk_rows, queue, prefix, offsets, cr);
     walk_strided(e0, e1, cnt, cr, cols, [&](uint32_t w, uint32_t u) {
-      const uint64_t m = front[u];
+      const uint64_t m = front[u] & lanes;
       if (m & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
         atomicOr((unsigned long long*)(next + w), (unsigned long long)m);
     });
@@ -196,6 +199,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_finish_lanes(uint64_t* __restrict
 
 // Pull for the lanes in live_in; with merge, next already holds this level's
 // pushed lanes, which are folded in.
+template <int kU, int kW>
 __global__ void __launch_bounds__(kBlock) k_ms_pull_lanes(uint64_t* __restrict__ seen, const uint64_t* __restrict__ front,
                                                           const uint64_t* __restrict__ nz_front, uint64_t* __restrict__ next,
                                                           uint64_t* __restrict__ nz_next,
@@ -206,8 +210,8 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull_lanes(uint64_t* __restrict__
                                                           unsigned long long* __restrict__ c, LaneEpi epi_in) {
   const LaneEpi epi = epi_in;  // a copy in registers: the epilogue accumulates per thread
   const FrontView fv{front, nz_front, nullptr, nullptr, filter ? kFrontFilter : kFrontPlain, 0};
-  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x,
-               ~UINT64_C(0), epi, merge != 0);
+  ms_pull_body<LaneEpi, kU, kW>(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x,
+                                gridDim.x, ~UINT64_C(0), epi, merge != 0);
 }
 
 // One bit per vertex whose frontier word has one of `lanes`.
@@ -629,21 +633,61 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     // statistics for the roots still choosing (a root that pulled keeps pulling)
     const LaneEpi lepi{cnt + M_NUM + 1, g.offsets.p, lanes_mode ? td : live};
     if (lanes_mode && td && bu) {
-      // the pushing roots over the CSR chunks holding one of their frontier
-      // vertices, then the pull of the others, merging the pushed lanes
-      m.chunk_list.alloc(m.sweep_chunks);
+      // the pushing roots first, then the pull of the others, merging the
+      // pushed lanes
       m.nz_sel.alloc(words);
       hipLaunchKernelGGL(k_ms_select, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, front, rows, td,
                          m.nz_sel.p);
       G5_HIP(hipMemsetAsync(next, 0, rows * 8, s));
-      hipLaunchKernelGGL(k_ms_collect, dim3(grid_of((int64_t)m.sweep_chunks, kBlock, 4096)), dim3(kBlock), 0, s,
-                         m.sweep_rows.p, m.sweep_chunks, m.nz_sel.p, m.chunk_list.p, d_list_count);
-      hipLaunchKernelGGL(k_ms_push_list, dim3(grid_of((int64_t)m.sweep_chunks, 1, 4096)), dim3(kBlock), 0, s,
-                         m.chunk_list.p, (const unsigned*)d_list_count, m.sweep_rows.p, (uint64_t)g.nnz, g.offsets.p,
-                         g.cols.p, front, m.nz_sel.p, next, td);
+      if (m.mixed_push == 1) {
+        // the pushing roots' frontier vertices as a queue (the count stays on
+        // the device), their rows as one edge-balanced virtual edge list; the
+        // host sizes the scans and grids by the per-root counts of the
+        // previous level (the union is at most their sum)
+        double nf_ub = 0, fd_ub = 0;
+        for (uint64_t x = td; x; x &= x - 1) {
+          nf_ub += (double)h_lane[__builtin_ctzll(x)];
+          fd_ub += lane_fd[__builtin_ctzll(x)];
+        }
+        const int64_t nf = (int64_t)std::min<double>(nf_ub, (double)rows);
+        const uint64_t E_ub = (uint64_t)std::min<double>(fd_ub, (double)g.nnz);
+        m.ulist_count.alloc(1);
+        device_scan(BitsCountN{m.nz_sel.p, words}, BitsEmitN{m.nz_sel.p, words, m.queue.p, m.ulist_count.p},
+                    (size_t)words + 1, m.scan_tmp.p, s);
+        device_scan(FrontierDegreeN{m.queue.p, g.offsets.p, m.ulist_count.p}, StoreU64{m.prefix.p}, (size_t)nf + 1,
+                    m.scan_tmp.p, s);
+        const uint64_t nchunks = (E_ub + kEpb - 1) / kEpb;
+        G5_CHECK(2 * nchunks <= m.chunk_rows.n, "g5_run_bfs_batch: chunk table too small");
+        if (nchunks) {
+          hipLaunchKernelGGL(k_chunk_rows<kEpb>, dim3(grid_of((int64_t)(2 * nchunks), kBlock, 65536)), dim3(kBlock),
+                             0, s, m.prefix.p, nf, nchunks, m.chunk_rows.p);
+          hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)E_ub, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p, nf,
+                             m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, next, td);
+        }
+      } else {
+        // over the CSR chunks holding one of the pushing roots' frontier vertices
+        m.chunk_list.alloc(m.sweep_chunks);
+        hipLaunchKernelGGL(k_ms_collect, dim3(grid_of((int64_t)m.sweep_chunks, kBlock, 4096)), dim3(kBlock), 0, s,
+                           m.sweep_rows.p, m.sweep_chunks, m.nz_sel.p, m.chunk_list.p, d_list_count);
+        hipLaunchKernelGGL(k_ms_push_list, dim3(grid_of((int64_t)m.sweep_chunks, 1, 4096)), dim3(kBlock), 0, s,
+                           m.chunk_list.p, (const unsigned*)d_list_count, m.sweep_rows.p, (uint64_t)g.nnz,
+                           g.offsets.p, g.cols.p, front, m.nz_sel.p, next, td);
+      }
       G5_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_ms_pull_lanes, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s,
-                         m.seen.p, front, nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, bu,
+      // probe unrolling as in the plain pull (pull_unroll)
+      decltype(&k_ms_pull_lanes<4, 2>) kpl = k_ms_pull_lanes<4, 2>;
+      switch (m.pull_unroll) {
+        case 1: kpl = k_ms_pull_lanes<8, 2>; break;
+        case 2: kpl = k_ms_pull_lanes<8, 4>; break;
+        case 3: kpl = k_ms_pull_lanes<4, 4>; break;
+        case 4: kpl = k_ms_pull_lanes<2, 1>; break;
+        case 5: kpl = k_ms_pull_lanes<1, 1>; break;
+        case 6: kpl = k_ms_pull_lanes<2, 2>; break;
+        case 7: kpl = k_ms_pull_lanes<1, 2>; break;
+        default: break;
+      }
+      hipLaunchKernelGGL(kpl, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s, m.seen.p, front,
+                         nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, bu,
                          m.pull_filter == 1 ? 1 : 0, 1, cnt, lepi);
     } else if (push) {
       device_scan(UnionCount{nz_front}, UnionEmit{nz_front, m.queue.p}, (size_t)words, m.scan_tmp.p, s);
@@ -672,7 +716,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
                            cnt, lepi, lane_stats ? 1 : 0);
       } else {
         hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)fdeg, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p,
-                           nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, next);
+                           nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, next, ~UINT64_C(0));
       }
       G5_HIP(hipGetLastError());
       if (touch) {
@@ -709,6 +753,9 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
         case 2: kp = k_ms_pull<8, 4>; break;
         case 3: kp = k_ms_pull<4, 4>; break;
         case 4: kp = k_ms_pull<2, 1>; break;
+        case 5: kp = k_ms_pull<1, 1>; break;
+        case 6: kp = k_ms_pull<2, 2>; break;
+        case 7: kp = k_ms_pull<1, 2>; break;
         default: break;
       }
       // Tail levels: once fewer than rows / kListBelow (root, vertex) pairs

@@ -458,21 +458,28 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
 
 // Admits waiting queries into free lanes; returns the lanes admitted.  An
 // idle graph admits at once.  A graph with searches in flight admits at this
-// level boundary once min(min_admit, free lanes, free answer arrays) queries
-// can start together (a cohort shares its expensive levels), or once the
-// oldest waiting query has waited max_wait_s: queries join the running
-// searches as lanes and answer memory free up.
+// level boundary, into the lanes and answer arrays that are free,
+//   * while every search in flight still pushes: their frontiers are small,
+//     and the newcomers reach their own expensive (pulling) levels together
+//     with them;
+//   * once min(min_admit, free answer arrays) queries can start together (a
+//     cohort shares its expensive levels);
+//   * once the oldest waiting query has waited max_wait_s.
+// Admitting into a graph whose searches already pull gives it searches at
+// different depths, and every later level then pulls for some of them.
 static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
   const uint64_t busy_lanes = e.active | e.cooling;
   if (e.waiting.empty() || busy_lanes == ~UINT64_C(0) || e.free_bufs.empty()) return 0;
   const bool in_flight = e.active != 0;
   if (in_flight) {
+    bool young = true;
+    for (uint64_t x = e.active; x && young; x &= x - 1) young = e.lane_fwd[__builtin_ctzll(x)];
     const int64_t free_lanes = 64 - __builtin_popcountll(busy_lanes);
     const int64_t free_bufs = (int64_t)e.free_bufs.size();
     const int64_t can = std::min<int64_t>({(int64_t)e.waiting.size(), free_lanes, free_bufs});
-    const int64_t want = std::min<int64_t>({(int64_t)srv->cfg.min_admit, free_lanes, free_bufs});
+    const int64_t want = std::min<int64_t>((int64_t)srv->cfg.min_admit, free_bufs);
     const double waited = t - srv->req[e.waiting.front()].arrival;
-    if (can < want && waited < srv->cfg.max_wait_s) return 0;
+    if (!young && can < want && waited < srv->cfg.max_wait_s) return 0;
   }
   ServeRoots rs;
   for (int r = 0; r < 64; ++r) {

@@ -135,7 +135,8 @@ class BatchTuning(ctypes.Structure):
                 ("sweep_grid", ctypes.c_int64), ("persistent", ctypes.c_int32), ("persist_levels", ctypes.c_int32),
                 ("persist_grid", ctypes.c_int64), ("persist_waves", ctypes.c_int32), ("pull_filter", ctypes.c_int32),
                 ("beta", ctypes.c_double), ("filter_from", ctypes.c_double), ("pull_unroll", ctypes.c_int32),
-                ("lane_rule", ctypes.c_int32), ("compact_below", ctypes.c_double)]
+                ("lane_rule", ctypes.c_int32), ("compact_below", ctypes.c_double), ("mixed_push", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 _lib = None
@@ -447,7 +448,7 @@ class G5Context:
 
     TUNING_FIELDS = ("alpha", "pull_grid", "sweep_below", "sweep_grid", "persistent", "persist_levels",
                      "persist_grid", "persist_waves", "pull_filter", "beta", "filter_from", "pull_unroll",
-                     "lane_rule", "compact_below")
+                     "lane_rule", "compact_below", "mixed_push")
 
     def batch_set_tuning(self, **kw):
         """Sets any of TUNING_FIELDS; the others keep their values."""
@@ -526,13 +527,14 @@ class G5Server:
     """Serving loop over graphs resident in HBM (g5_server_*): continuous
     batching of (graph, root) BFS queries, up to 64 in flight per graph."""
 
-    def __init__(self, contexts, answer_buffers=128, alpha=64.0, pull_filter=0, min_admit=8, max_wait_s=0.005,
+    def __init__(self, contexts, answer_buffers=128, alpha=64.0, pull_filter=0, min_admit=64, max_wait_s=0.3,
                  lane_rule=3):
         """Defaults: a graph with searches in flight admits waiting queries
-        at a level boundary once 8 of them (or as many as there are free
-        lanes and answer arrays) can start together, or once the oldest has
-        waited 5 ms; each search pulls while its frontier's edges exceed 1/64
-        of the edges it has not reached, and pushes otherwise."""
+        into its free lanes at a level boundary while every search in flight
+        still pushes (newcomers join a young cohort), once 64 of them can
+        start together, or once the oldest has waited 0.3 s; each search
+        pulls while its frontier's edges exceed 1/64 of the edges it has not
+        reached, and pushes otherwise."""
         self._lib = load_library()
         self._ctxs = list(contexts)  # the server uses their graphs and streams
         arr = (ctypes.c_void_p * len(self._ctxs))(*[c._h for c in self._ctxs])

@@ -94,7 +94,7 @@ typedef struct g5_batch_stats {
 /* Tuning of the batched BFS (set by the autotuner, graph500_amd/autotune.py;
  * defaults: lane_rule 1, alpha 128, pull_grid 8192, sweep_below 0, sweep_grid
  * 4096, persistent 0, persist_levels 16, persist_grid 0, persist_waves 4,
- * pull_filter 0, beta 50, filter_from 0.01, pull_unroll 0, compact_below 0). */
+ * pull_filter 0, beta 50, filter_from 0.01, pull_unroll 0, compact_below 0, mixed_push 1). */
 typedef struct g5_batch_tuning {
   double alpha;        /* push while the frontier union has fewer than nnz/alpha edges */
   int64_t pull_grid;   /* workgroups of the vertex-parallel pull (256 threads each) */
@@ -114,8 +114,9 @@ typedef struct g5_batch_tuning {
   double beta;             /* pull (never push) once fewer than 1/beta of the (root, vertex) pairs
                               are unreached; 0 disables */
   double filter_from;      /* pull_filter 2 threshold, a fraction of the rows */
-  int32_t pull_unroll;     /* probes in flight per lane in the per-level pull: 0 = 4 per short-row step
-                              and 2 per long-row step, 1 = 8/2, 2 = 8/4, 3 = 4/4, 4 = 2/1 */
+  int32_t pull_unroll;     /* probes in flight per lane in the per-level pulls: 0 = 4 per short-row step
+                              and 2 per long-row step, 1 = 8/2, 2 = 8/4, 3 = 4/4, 4 = 2/1, 5 = 1/1,
+                              6 = 2/2, 7 = 1/2 */
   int32_t lane_rule;       /* 0: one direction per level for all roots (the rule on alpha above);
                               1: a direction per root and level (per-level launches, reorder modes
                               1 and 2): a root pulls when its frontier's edges exceed 1/alpha of the
@@ -124,6 +125,11 @@ typedef struct g5_batch_tuning {
   double compact_below;    /* per-level pulls (persistent = 0) read frontier words from a compact
                               copy holding only the frontier's words when the frontier spans fewer
                               than compact_below x rows vertices; 0 disables */
+  int32_t mixed_push;      /* (lane_rule 1) the push of a level where some roots push and the others
+                              pull: 0 = over the CSR chunks holding one of the pushing roots'
+                              frontier vertices; 1 = over those vertices as a queue, edge-balanced
+                              chunks of their rows only */
+  int32_t reserved;        /* must be 0 */
 } g5_batch_tuning;
 
 /* ---- context ---- */
@@ -334,10 +340,11 @@ typedef struct g5_server_config {
   double alpha;            /* lane_rule 1: a search pulls when its frontier has more than nnz/alpha
                               edges, else pushes; lane_rule 3: when its frontier's edges exceed
                               1/alpha of the edges of the vertices it has not reached */
-  int32_t min_admit;       /* a graph with searches in flight admits waiting queries at a level
-                              boundary once min(min_admit, free lanes, free answer arrays) of them
-                              can start together (searches started together share their sweeps;
-                              1 admits every query at the first boundary with a free lane) ... */
+  int32_t min_admit;       /* a graph with searches in flight admits waiting queries into its free
+                              lanes at a level boundary while every search in flight still pushes,
+                              or once min(min_admit, free answer arrays) of them can start together
+                              (searches started together share their pulling levels; 1 admits every
+                              query at the first boundary with a free lane) ... */
   int32_t lane_rule;       /* 0: each search switches direction by the reference rule on its own
                               frontier size (mpi/bfs.hpp:2799-2829); 1: by its frontier's edges;
                               2: all searches together, by the frontier union's edges; 3: by its

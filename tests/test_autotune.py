@@ -14,7 +14,8 @@ class FakeCtx:
     def __init__(self, bad=None, bad_pairs=None):
         self.t = {"alpha": 16.0, "pull_grid": 8192, "sweep_below": 0.25, "sweep_grid": 4096, "persistent": 0,
                   "persist_levels": 16, "persist_grid": 0, "persist_waves": 8, "pull_filter": 0, "beta": 50.0,
-                  "compact_below": 0.0, "filter_from": 0.01, "pull_unroll": 0, "lane_rule": 0}
+                  "compact_below": 0.0, "filter_from": 0.01, "pull_unroll": 0, "lane_rule": 0,
+                  "mixed_push": 1}
         self.bad = bad
         self.bad_pairs = bad_pairs
         self.runs = 0

@@ -83,11 +83,12 @@ def test_batch_compact_frontier_matches_reference_s16(s16ctx, compact):
     s16ctx.batch_set_tuning(alpha=16.0, compact_below=0.0, beta=50.0)
 
 
-def _per_root_run(s16ctx, alpha, filt):
+def _per_root_run(s16ctx, alpha, filt, mixed_push=1, unroll=0):
     g = gold("ref_s16_64roots.json")
     roots = [p["root"] for p in g["per_root"]]
-    s16ctx.batch_set_tuning(alpha=alpha, persistent=0, lane_rule=1, pull_filter=filt, beta=0.0, sweep_below=0.0)
-    say("batch s16 per-root rule alpha", alpha, "filter", filt)
+    s16ctx.batch_set_tuning(alpha=alpha, persistent=0, lane_rule=1, pull_filter=filt, beta=0.0, sweep_below=0.0,
+                            mixed_push=mixed_push, pull_unroll=unroll)
+    say("batch s16 per-root rule alpha", alpha, "filter", filt, "mixed_push", mixed_push, "unroll", unroll)
     try:
         s16ctx.run_bfs_batch(roots)
         st = s16ctx.batch_stats()
@@ -102,23 +103,37 @@ def _per_root_run(s16ctx, alpha, filt):
         assert sum(st["pairs"]) + 64 == sum(sum(p["level_histogram"]) for p in g["per_root"])
         return st["direction"]
     finally:
-        s16ctx.batch_set_tuning(alpha=16.0, sweep_below=0.25, persistent=0, pull_filter=0, beta=50.0, lane_rule=0)
+        s16ctx.batch_set_tuning(alpha=16.0, sweep_below=0.25, persistent=0, pull_filter=0, beta=50.0, lane_rule=0,
+                                mixed_push=1, pull_unroll=0)
 
 
+@pytest.mark.parametrize("mixed_push", [0, 1])
 @pytest.mark.parametrize("alpha,filt", [(4.0, 0), (16.0, 1), (1e-9, 0), (1e12, 0), (0.05, 0)])
-def test_batch_per_root_directions_match_reference_s16(s16ctx, alpha, filt):
+def test_batch_per_root_directions_match_reference_s16(s16ctx, alpha, filt, mixed_push):
     """lane_rule 1: each root pulls or pushes by its own frontier; a level
-    with both runs the push of some roots and the pull of the others.  1e-9
-    pushes every root at every level, 1e12 pulls every root after its first
-    level."""
-    _per_root_run(s16ctx, alpha, filt)
+    with both runs the push of some roots (over the CSR chunks holding their
+    frontier, or over their frontier as a queue) and the pull of the others.
+    1e-9 pushes every root at every level, 1e12 pulls every root after its
+    first level."""
+    _per_root_run(s16ctx, alpha, filt, mixed_push)
+
+
+@pytest.mark.parametrize("mixed_push", [0, 1])
+@pytest.mark.parametrize("unroll", [0, 1, 2, 3, 4, 5, 6, 7])
+def test_batch_mixed_level_pull_unroll_s16(s16ctx, unroll, mixed_push):
+    """Every probe unrolling of the merging pull of a mixed level
+    (k_ms_pull_lanes<4,2>, <8,2>, <8,4>, <4,4>, <2,1>, <1,1>, <2,2>, <1,2>)
+    under both push forms:
+    alpha 4 gives the S16 batch mixed levels."""
+    d = _per_root_run(s16ctx, 4.0, 0, mixed_push, unroll)
+    assert 3 in d, d
 
 
 @pytest.mark.parametrize("rule", [0, 1])
-@pytest.mark.parametrize("unroll", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("unroll", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_batch_pull_unroll_matches_reference_s16(s16ctx, unroll, rule):
     """Every probe unrolling of the per-level pull (k_ms_pull<4,2>, <8,2>,
-    <8,4>, <4,4>, <2,1>) gives the reference's levels: rule 0 with an alpha
+    <8,4>, <4,4>, <2,1>, <1,1>, <2,2>, <1,2>) gives the reference's levels: rule 0 with an alpha
     that never pushes runs every level after the first as that pull; rule 1
     runs it once every root pulls."""
     g = gold("ref_s16_64roots.json")
@@ -246,13 +261,14 @@ def test_batch_matches_single_root_gpu(scale, sweep, rule, beta):
             assert c.teps_edges() == teps[i]
 
 
-@pytest.mark.parametrize("unroll", [0, 4])
+@pytest.mark.parametrize("unroll,alpha,mixed_push", [(0, 256.0, 0), (4, 256.0, 0), (4, 256.0, 1), (0, 32.0, 1),
+                                                      (4, 48.0, 1)])
 @pytest.mark.parametrize("scale", [20, 22])
-def test_batch_tuned_combination_matches_reference(scale, unroll):
-    """The combination the autotuner picked for the SCALE 28 bench (per-root
-    rule, alpha 256, beta 200, filter_from 0.001, probe unrolling 4) and the
-    same with unrolling 0: the fixture roots' level hashes and edge counts,
-    and single-root runs of roots 16-63."""
+def test_batch_tuned_combination_matches_reference(scale, unroll, alpha, mixed_push):
+    """Combinations the autotuner picks at SCALE 28 (per-root rule, beta 200,
+    filter_from 0.001; alpha 256 with the chunk push of mixed levels, alpha
+    32-48 with the queue push; probe unrolling 4 or 0): the fixture roots'
+    level hashes and edge counts, and single-root runs of roots 16-63."""
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
     with graph500_amd.G5Context(0) as c:
@@ -260,9 +276,10 @@ def test_batch_tuned_combination_matches_reference(scale, unroll):
         c.construct(2, release_edges=True)
         roots = c.find_roots(64)
         c.batch_prepare()
-        c.batch_set_tuning(lane_rule=1, alpha=256.0, beta=200.0, filter_from=0.001, pull_unroll=unroll,
-                           pull_grid=8192, sweep_below=0.0, persistent=0, pull_filter=0, compact_below=0.0)
-        say("batch s%d tuned combination, unroll %d" % (scale, unroll))
+        c.batch_set_tuning(lane_rule=1, alpha=alpha, beta=200.0, filter_from=0.001, pull_unroll=unroll,
+                           pull_grid=8192, sweep_below=0.0, persistent=0, pull_filter=0, compact_below=0.0,
+                           mixed_push=mixed_push)
+        say("batch s%d tuned combination, unroll %d alpha %g mixed_push %d" % (scale, unroll, alpha, mixed_push))
         c.run_bfs_batch(roots)
         teps = c.batch_teps_edges()
         for i, pr in enumerate(g["per_root"]):

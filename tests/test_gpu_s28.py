@@ -27,11 +27,17 @@ SCALE = 28
 PF_NEDGE_28 = 4294927670  # mpi/utils.hpp:3693
 
 # a batch tuning that the autotuner picked at this shape (per-root rule,
-# alpha 256, beta 200, probe unrolling 4 = k_ms_pull<2, 1>)
+# alpha 256, beta 200, probe unrolling 4 = k_ms_pull<2, 1>, mixed levels
+# pushed over CSR chunks), the same rule pushing more roots with the queue
+# push of mixed levels, and the defaults
 TUNED = {"lane_rule": 1, "alpha": 256.0, "beta": 200.0, "pull_unroll": 4, "filter_from": 0.001,
-         "pull_grid": 8192, "sweep_below": 0.0, "persistent": 0, "pull_filter": 0, "compact_below": 0.0}
+         "pull_grid": 8192, "sweep_below": 0.0, "persistent": 0, "pull_filter": 0, "compact_below": 0.0,
+         "mixed_push": 0}
+QUEUE = dict(TUNED, alpha=48.0, mixed_push=1)
 DEFAULT = {"lane_rule": 1, "alpha": 128.0, "beta": 50.0, "pull_unroll": 0, "filter_from": 0.01,
-           "pull_grid": 8192, "sweep_below": 0.0, "persistent": 0, "pull_filter": 0, "compact_below": 0.0}
+           "pull_grid": 8192, "sweep_below": 0.0, "persistent": 0, "pull_filter": 0, "compact_below": 0.0,
+           "mixed_push": 1}
+TUNINGS = {"tuned": TUNED, "queue": QUEUE, "default": DEFAULT}
 
 
 def say(*a):
@@ -93,7 +99,7 @@ def test_single_root_all_64_validate_and_count_pf_nedge(s28):
     assert len(s28["single"]) == 64
 
 
-@pytest.mark.parametrize("name", ["tuned", "default"])
+@pytest.mark.parametrize("name", ["tuned", "queue", "default"])
 def test_batch_lanes_equal_single_root(s28, name):
     """One 64-root batch; lane i's levels equal root i's single-root levels and
     every lane counts pf_nedge[28] traversed edges."""
@@ -102,7 +108,7 @@ def test_batch_lanes_equal_single_root(s28, name):
         pytest.skip("needs the single-root results of the test above")
     c.batch_prepare()
     try:
-        c.batch_set_tuning(**(TUNED if name == "tuned" else DEFAULT))
+        c.batch_set_tuning(**TUNINGS[name])
         say("s28 batch", name)
         c.run_bfs_batch(s28["roots"])
         st = c.batch_stats()
