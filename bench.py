@@ -71,9 +71,12 @@ BATCH_KERNELS = {
     0: ("k_ms_pull", "k_ms_pull (batched vertex-parallel pull)"),
     1: ("k_ms_push", "k_ms_push + k_ms_finish (batched push)"),
     2: ("k_ms_sweep", "k_ms_sweep + k_ms_finish (batched edge-parallel pull sweep)"),
-    3: ("k_ms_mixed", "k_ms_select + k_ms_collect + k_ms_push_list + k_ms_pull_lanes (batched level with "
-                      "per-root directions: the pushing roots, then the pulling roots)"),
+    3: ("k_ms_mixed", "k_ms_select + queue scans + k_chunk_rows + k_ms_push + k_ms_pull_lanes (batched level "
+                      "with per-root directions: the pushing roots' frontier as a queue, then the pulling roots)"),
 }
+# the mixed level's kernels when its push goes over CSR chunks (mixed_push 0)
+MIXED_CHUNK_PUSH = ("k_ms_select + k_ms_collect + k_ms_push_list + k_ms_pull_lanes (batched level with per-root "
+                    "directions: the pushing roots over CSR chunks, then the pulling roots)")
 
 
 def log(*a):
@@ -485,6 +488,7 @@ def main():
     dom_gbs = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     per_launch = dom_bytes / max(dom_launches, 1)
     traffic = traffic_from_profile(BATCH_KERNELS[dom][0], a.scale)
+    ctx_tuning = ctx.batch_tuning()
     out = {
         "metric": METRIC,
         "value": round(gteps, 3),
@@ -511,7 +515,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": BATCH_KERNELS[dom][1],
+            "kernel": (MIXED_CHUNK_PUSH if dom == 3 and ctx_tuning.get("mixed_push") == 0
+                       else BATCH_KERNELS[dom][1]),
             "achieved": round(dom_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -529,7 +534,7 @@ def main():
             "roofline_frac_whole_batch": round((tot / (sum(dev) / len(dev))) / (HBM_PEAK_GBS * 1e9), 4),
             "levels": "".join("UPSM"[d] for d in st["direction"]),
             "level_ms": [round(x, 3) for x in st["level_ms"]],
-            "tuning": ctx.batch_tuning(),
+            "tuning": ctx_tuning,
             "autotune": ({"trials": tuned["trials"], "seconds": tuned["seconds"]} if tuned else None),
             "timed_region_s": round(t_all, 3),
         },

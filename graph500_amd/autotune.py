@@ -29,6 +29,7 @@ DEFAULT_CACHE = os.environ.get("G5_TUNING_CACHE",
 # "a,b" takes pairs: the direction rule and its alpha are tried together, as
 # alpha means a different threshold under each rule.
 SPACE = [
+    ("pull_unroll", [0, 1, 2, 3, 4, 5, 6, 7]),
     ("mixed_push", [0, 1]),
     ("lane_rule,alpha", [(0, 1.0), (0, 2.0), (0, 4.0), (0, 8.0), (0, 16.0), (0, 64.0),
                          (1, 16.0), (1, 24.0), (1, 32.0), (1, 48.0), (1, 64.0), (1, 96.0), (1, 128.0), (1, 192.0),
@@ -40,9 +41,11 @@ SPACE = [
     ("filter_from", [0.001, 0.01, 0.05]),
     ("compact_below", [0.0, 0.25, 0.5]),
     ("pull_grid", [4096, 8192, 16384]),
-    ("pull_unroll", [0, 1, 2, 3, 4, 5, 6, 7]),
     ("persist_waves", [4, 6, 8]),
 ]
+# a second pass over the knobs that depend most on each other (the probe
+# unrolling moves the best direction threshold and back)
+REFINE = ["pull_unroll", "lane_rule,alpha"]
 
 
 def graph_key(ctx):
@@ -98,7 +101,8 @@ def autotune(ctx, roots, reps=2, budget_s=60.0, cache=DEFAULT_CACHE, use_cache=T
     cur = ctx.batch_tuning()
     best_ms, ref_pairs = _measure(ctx, roots, reps)
     trials = 1
-    for knob, values in SPACE:
+    space = dict(SPACE)
+    for knob, values in SPACE + [(k, space[k]) for k in REFINE]:
         if knob == "persist_waves" and not cur["persistent"]:
             continue
         names = knob.split(",")

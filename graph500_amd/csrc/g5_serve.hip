@@ -123,8 +123,11 @@ __global__ void __launch_bounds__(kBlock) k_sv_pull(uint64_t* __restrict__ seen,
                                                     int merge, ServeEpi epi) {
   const FrontView fv{front, nz_front, nullptr, nullptr, filter ? kFrontFilter : kFrontPlain, 0};
   const ServeLocal el{epi};
-  ms_pull_body(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x, gridDim.x, keep, el,
-               merge != 0);
+  // two probes per step on a short row, one per lane and step on a long row:
+  // the batch's fastest pull unrolling at SCALE 28 (fewer probes past the
+  // point where a vertex's searches are all found)
+  ms_pull_body<ServeLocal, 2, 1>(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x,
+                                 gridDim.x, keep, el, merge != 0);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sv_finish(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
