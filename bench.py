@@ -173,26 +173,30 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
 
 
-def gather_calibration(array_bytes=4 << 30, gathers=1 << 28):
-    """One launch of k_measure_gather (libg5.so, include/g5_measure.h):
-    `gathers` random 8-byte reads over an array of `array_bytes`, far larger
-    than the last-level cache, so nearly every read misses; read under
-    rocprofv3 --pmc, it tells what the traffic counters report per random
-    8-byte miss."""
+def gather_calibration(sizes=(4 << 30,), gathers=1 << 28):
+    """One launch of k_measure_gather (libg5.so, include/g5_measure.h) per
+    array size: `gathers` random 8-byte reads over an array of that many
+    bytes.  Over an array far larger than the last-level cache nearly every
+    read misses; read under rocprofv3 --pmc, it tells what the traffic
+    counters report per random 8-byte miss.  Over a range of sizes it gives
+    the random-read rate of each level of the memory system (L2-resident,
+    last-level-cache-resident, HBM)."""
     import ctypes
     from graph500_amd import g5
     lib = g5.load_library()
-    ms = ctypes.c_double(0)
-    chk = ctypes.c_uint64(0)
-    rc = lib.g5_measure_gather(0, ctypes.c_int64(array_bytes), ctypes.c_int64(gathers), ctypes.byref(ms),
-                               ctypes.byref(chk))
-    if rc != 0:
-        raise RuntimeError("g5_measure_gather failed: %d" % rc)
-    rate = gathers / (ms.value / 1e3)
-    print(json.dumps({"kernel": "k_measure_gather", "array_bytes": array_bytes, "gathers": gathers,
-                      "ms": round(ms.value, 4), "gathers_per_s": rate,
-                      "gbs_if_64B_per_gather": round(rate * 64 / 1e9, 1),
-                      "gbs_if_128B_per_gather": round(rate * 128 / 1e9, 1), "checksum": chk.value}))
+    for array_bytes in sizes:
+        ms = ctypes.c_double(0)
+        chk = ctypes.c_uint64(0)
+        rc = lib.g5_measure_gather(0, ctypes.c_int64(array_bytes), ctypes.c_int64(gathers), ctypes.byref(ms),
+                                   ctypes.byref(chk))
+        if rc != 0:
+            raise RuntimeError("g5_measure_gather failed: %d" % rc)
+        rate = gathers / (ms.value / 1e3)
+        print(json.dumps({"kernel": "k_measure_gather", "array_bytes": array_bytes, "gathers": gathers,
+                          "ms": round(ms.value, 4), "gathers_per_s": rate,
+                          "gbs_if_64B_per_gather": round(rate * 64 / 1e9, 1),
+                          "gbs_if_128B_per_gather": round(rate * 128 / 1e9, 1), "checksum": chk.value}),
+              flush=True)
 
 
 def rocprof_summary(db, top=30):
@@ -247,10 +251,16 @@ def rocprof_dispatches(db, pattern):
 def traffic_from_profile(kernel, scale):
     """HBM bytes per launch of `kernel` from the committed rocprof PMC summary
     (profiles/<kernel>_s<scale>_traffic.txt, line 'traffic_bytes_per_launch: X')."""
+    return profile_figure(kernel, scale, "traffic_bytes_per_launch")
+
+
+def profile_figure(kernel, scale, key):
+    """A 'key: value' figure of the committed profile summary of `kernel` at
+    `scale` (profiles/<kernel>_s<scale>_traffic.txt), or None."""
     path = os.path.join(HERE, "profiles", "%s_s%d_traffic.txt" % (kernel, scale))
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(r"^%s:\s*([0-9.eE+]+)" % re.escape(key), open(path).read(), re.M)
     return float(m.group(1)) if m else None
 
 
@@ -394,6 +404,8 @@ def main():
     ap.add_argument("--autotune-budget", type=float, default=30.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--gather-calibration", action="store_true")
+    ap.add_argument("--gather-bytes", type=lambda x: [int(v) for v in x.split(",")], default=[4 << 30],
+                    help="array sizes of --gather-calibration, bytes, comma-separated")
     ap.add_argument("--rocprof-summary", metavar="DB")
     ap.add_argument("--rocprof-dispatches", nargs=2, metavar=("DB", "KERNEL"))
     ap.add_argument("--single-root-only", type=int, metavar="N")
@@ -411,7 +423,7 @@ def main():
         copy_calibration()
         return 0
     if a.gather_calibration:
-        gather_calibration()
+        gather_calibration(a.gather_bytes)
         return 0
     if a.gpus != 1:
         print("bench.py measures one MI355X only: this build has no multi-GPU path "
@@ -488,6 +500,8 @@ def main():
     dom_gbs = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     per_launch = dom_bytes / max(dom_launches, 1)
     traffic = traffic_from_profile(BATCH_KERNELS[dom][0], a.scale)
+    requests = profile_figure(BATCH_KERNELS[dom][0], a.scale, "memory_requests_per_launch")
+    gather_rate = profile_figure(BATCH_KERNELS[dom][0], a.scale, "random_read_rate_g_per_s")
     ctx_tuning = ctx.batch_tuning()
     out = {
         "metric": METRIC,
@@ -525,6 +539,13 @@ def main():
             "launches": dom_launches,
             "algorithmic_bytes_per_launch": round(per_launch),
             "avg_launch_ms": round(dom_ms / max(dom_launches, 1), 4),
+            # the same kernels as memory requests (rocprof TCC_EA0_RDREQ + WRREQ
+            # per launch) against the random-read rate of the memory system
+            # (k_measure_gather over an array larger than the last-level cache)
+            "memory_requests_per_launch": requests,
+            "request_rate_g_per_s": (round(requests / (dom_ms / max(dom_launches, 1) / 1e3) / 1e9, 1)
+                                     if requests and dom_ms > 0 else None),
+            "random_read_rate_g_per_s": gather_rate,
         },
         "batch": {
             "gteps": round(gteps, 3),
