@@ -25,7 +25,8 @@
 //              reads them for the alpha/beta rule.
 //   queues     a top-down frontier that comes from a bitmap is rebuilt by an
 //              exclusive scan of per-word popcounts (ascending vertex order).
-//   bottom-up  k_bu: one wave per 64-vertex bitmap word, one lane per vertex.
+//   bottom-up  k_bu: one wave per two 64-vertex bitmap words, one lane per
+//              vertex of each (both words' first probes in flight together).
 //              Unvisited lanes first probe their first (highest-degree)
 //              neighbour from the coalesced first_nbr array; misses with a
 //              short row scan it lane-sequentially, long rows are scanned by
@@ -68,7 +69,9 @@ __global__ void k_bfs_root(const uint32_t* __restrict__ old2new, int64_t root_or
                            uint64_t* __restrict__ vis, uint32_t* __restrict__ queue,
                            unsigned long long* __restrict__ counters) {
   const uint32_t r = old2new[root_orig];
-  counters[C_AUX] = r;
+  // the first level's counters start at zero; k_publish clears them after
+  // every later level
+  for (int i = 0; i < C_NUM; ++i) counters[i] = 0;
   if ((int64_t)r < rows) {
     level[r] = 0;
     parent[r] = r;
@@ -398,30 +401,22 @@ __global__ void __launch_bounds__(kBlock) k_td_expand_segs(
   }
 }
 
-// First neighbour of v, in ascending (highest degree first) order, whose bit
-// is set in bm: the bottom-up probe.  Every lane of the wave calls it; lanes
-// with active=false only help.  The first neighbour comes from the coalesced
-// first_nbr array; rows up to kShortRow are then scanned by their lane, longer
-// rows by the whole wave 64 entries at a time with a ballot for the first hit.
-__device__ __forceinline__ bool first_marked_nbr(bool active, uint32_t v, const uint64_t* __restrict__ bm,
-                                                 const uint32_t* __restrict__ first_nbr,
-                                                 const uint64_t* __restrict__ offsets,
-                                                 const uint32_t* __restrict__ cols, uint32_t& p) {
+// Row part of the bottom-up probe: the first entry of [offsets[v] + 1,
+// offsets[v + 1]) (the first neighbour is probed apart, from the coalesced
+// first_nbr array) whose bit is set in bm.  Every lane of the wave calls it;
+// lanes with active=false only help.  Rows up to kShortRow are scanned by
+// their lane, longer rows by the whole wave 64 entries at a time with a ballot
+// for the first hit.
+__device__ __forceinline__ bool row_marked_nbr(bool active, uint32_t v, const uint64_t* __restrict__ bm,
+                                               const uint64_t* __restrict__ offsets,
+                                               const uint32_t* __restrict__ cols, uint32_t& p) {
   const int lane = threadIdx.x & 63;
   bool found = false, need = false;
   uint64_t rb = 0, re = 0;
   if (active) {
-    const uint32_t f = first_nbr[v];
-    if (f != kNoNbr) {
-      if (bit_set(bm, f)) {
-        found = true;
-        p = f;
-      } else {
-        rb = offsets[v] + 1;
-        re = offsets[v + 1];
-        need = rb < re;
-      }
-    }
+    rb = offsets[v] + 1;
+    re = offsets[v + 1];
+    need = rb < re;
   }
   if (need && re - rb <= (uint64_t)kShortRow) {
     // four independent probes per step (entries past the row repeat the
@@ -466,6 +461,20 @@ __device__ __forceinline__ bool first_marked_nbr(bool active, uint32_t v, const 
     }
   }
   return found;
+}
+
+// First neighbour of v, in ascending (highest degree first) order, whose bit
+// is set in bm: the bottom-up probe (first_nbr, then the row).  Every lane of
+// the wave calls it; lanes with active=false only help.
+__device__ __forceinline__ bool first_marked_nbr(bool active, uint32_t v, const uint64_t* __restrict__ bm,
+                                                 const uint32_t* __restrict__ first_nbr,
+                                                 const uint64_t* __restrict__ offsets,
+                                                 const uint32_t* __restrict__ cols, uint32_t& p) {
+  const uint32_t f = active ? first_nbr[v] : kNoNbr;
+  const bool hit = f != kNoNbr && bit_set(bm, f);
+  if (hit) p = f;
+  const bool more = row_marked_nbr(f != kNoNbr && !hit, v, bm, offsets, cols, p);
+  return hit || more;
 }
 
 // Head of a growing bulk top-down level: the rows [0, 64 * head_words) (the
@@ -661,6 +670,9 @@ __global__ void __launch_bounds__(kBlock) k_td_finish(const uint8_t* __restrict_
   block_add(total, counters + C_FOUND);
 }
 
+// Bottom-up level: two 64-vertex bitmap words per wave and step (one lane per
+// vertex of each), so both words' first-neighbour probes are in flight
+// together; a pair whose vertices are all visited is skipped whole.
 __global__ void __launch_bounds__(kBlock) k_bu(
     const uint64_t* __restrict__ vis, uint64_t* __restrict__ vis_next,
     const uint32_t* __restrict__ first_nbr, const uint64_t* __restrict__ offsets,
@@ -669,22 +681,43 @@ __global__ void __launch_bounds__(kBlock) k_bu(
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
   uint64_t found_total = 0;
-  for (int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < words; w += nwaves) {
-    const uint64_t cur = vis[w];
-    if (cur == ~UINT64_C(0)) {
-      if (lane == 0) vis_next[w] = cur;
+  for (int64_t w0 = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 2; w0 < words; w0 += nwaves * 2) {
+    const int64_t w1 = w0 + 1;
+    const bool has1 = w1 < words;
+    const uint64_t c0 = vis[w0];
+    const uint64_t c1 = has1 ? vis[w1] : ~UINT64_C(0);  // bits past the last row read as visited
+    if (c0 == ~UINT64_C(0) && c1 == ~UINT64_C(0)) {
+      if (lane == 0) {
+        vis_next[w0] = c0;
+        if (has1) vis_next[w1] = c1;
+      }
       continue;
     }
-    const uint32_t v = (uint32_t)(w * 64 + lane);
-    uint32_t p = 0;
-    const bool found = first_marked_nbr(!((cur >> lane) & 1), v, vis, first_nbr, offsets, cols, p);
-    const uint64_t fm = __ballot(found);
-    if (lane == 0) vis_next[w] = cur | fm;
-    if (found) {
-      level[v] = (uint8_t)lvl;
-      parent[v] = p;
+    const uint32_t v0 = (uint32_t)(w0 * 64 + lane), v1 = (uint32_t)(w1 * 64 + lane);
+    const bool a0 = !((c0 >> lane) & 1), a1 = !((c1 >> lane) & 1);
+    const uint32_t f0 = a0 ? first_nbr[v0] : kNoNbr;
+    const uint32_t f1 = a1 ? first_nbr[v1] : kNoNbr;
+    const bool h0 = f0 != kNoNbr && bit_set(vis, f0);
+    const bool h1 = f1 != kNoNbr && bit_set(vis, f1);
+    uint32_t p0 = f0, p1 = f1;
+    // every lane makes both calls (the long-row scan needs the whole wave)
+    const bool r0 = row_marked_nbr(f0 != kNoNbr && !h0, v0, vis, offsets, cols, p0);
+    const bool r1 = row_marked_nbr(f1 != kNoNbr && !h1, v1, vis, offsets, cols, p1);
+    const bool g0 = h0 || r0, g1 = h1 || r1;
+    const uint64_t fm0 = __ballot(g0), fm1 = __ballot(g1);
+    if (lane == 0) {
+      vis_next[w0] = c0 | fm0;
+      if (has1) vis_next[w1] = c1 | fm1;
     }
-    found_total += __popcll(fm);  // identical in every lane
+    if (g0) {
+      level[v0] = (uint8_t)lvl;
+      parent[v0] = p0;
+    }
+    if (g1) {
+      level[v1] = (uint8_t)lvl;
+      parent[v1] = p1;
+    }
+    found_total += __popcll(fm0) + __popcll(fm1);  // identical in every lane
   }
   block_add(lane == 0 ? found_total : 0, counters + C_FOUND);
 }
@@ -816,9 +849,12 @@ __global__ void k_make_levels(const uint32_t* __restrict__ old2new, const uint8_
 // wait for a copy and a stream synchronisation.
 enum { P_SEQ = C_NUM, P_NUM = C_NUM + 1 };
 
-__global__ void k_publish(const unsigned long long* __restrict__ counters, const uint64_t* __restrict__ prefix,
+__global__ void k_publish(unsigned long long* __restrict__ counters, const uint64_t* __restrict__ prefix,
                           int64_t nf, int forward, uint64_t* pub, uint64_t seq) {
-  for (int i = 0; i < C_NUM; ++i) pub[i] = counters[i];
+  for (int i = 0; i < C_NUM; ++i) {
+    pub[i] = counters[i];
+    counters[i] = 0;  // ready for the next level, with no fill launched before it
+  }
   if (forward) pub[1] = prefix[nf];
   __threadfence_system();
   pub[P_SEQ] = seq;
@@ -961,7 +997,6 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     ++lvl;
     G5_CHECK(lvl < 255, "g5_run_bfs: more than 254 levels (uint8 level array)");
     const bool forward = next_forward;
-    G5_HIP(hipMemsetAsync(b.counters.p, 0, C_NUM * sizeof(uint64_t), s));
     G5_HIP(hipEventRecord(b.lev_begin[lvl], s));
     if (forward) {
       device_scan(FrontierDegree{b.queue[qi].p, g.offsets.p, nf}, StoreU64{b.prefix.p},
