@@ -364,8 +364,14 @@ def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):
                 srv.release(res["request"][i])
                 ctxs[g].run_bfs(r)
                 ok &= bool(np.array_equal(lv, ctxs[g].get_levels())) and int(res["edges"][i]) == ctxs[g].teps_edges()
+            per_graph = {}
+            for gi, sc in enumerate(scales):
+                lg = lat[q["graph"] == gi]
+                if len(lg):
+                    per_graph["S%d" % sc] = {"queries": int(len(lg)), "p50_ms": round(float(np.percentile(lg, 50)), 3),
+                                             "p99_ms": round(float(np.percentile(lg, 99)), 3)}
             run = {"rate_qps": rate, "queries": nq, "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                   "p99_ms": round(float(np.percentile(lat, 99)), 3), "per_graph": per_graph,
                    "served_qps": round(nq / rep["wall_s"], 1),
                    "gteps": round(rep["edges"] / rep["wall_s"] / 1e9, 2),
                    "max_lanes": rep["max_active_lanes"], "admitted_busy": rep["admitted_busy"],

@@ -108,6 +108,22 @@ struct ServeLocal {
   }
 };
 
+// A level's counters into the engine's pinned host block ([0, M_NUM) the
+// level counters, [80, 208) the per-lane counts and degree sums), then, after
+// a system-scope fence, the level's sequence number into slot kSvSeq: the
+// host learns that the level finished by polling host memory, with no copy
+// commands on the stream.
+constexpr int kSvSeq = 208;
+__global__ void k_sv_publish(const unsigned long long* __restrict__ c, unsigned long long* __restrict__ h,
+                             uint64_t seq) {
+  const int t = threadIdx.x;
+  if (t < M_NUM) h[t] = c[t];
+  if (t < 128) h[80 + t] = c[M_NUM + 1 + t];
+  __threadfence_system();
+  __syncthreads();
+  if (t == 0) h[kSvSeq] = seq;
+}
+
 struct ServeRoots {
   int64_t root[64];  // original id of the query admitted into each lane, -1 = none
   uint8_t* ans[64];
@@ -278,8 +294,10 @@ struct Engine {
   DevBuf<uint32_t> chunk_rows, chunk_list;
   DevBuf<unsigned long long> cnt;    // M_NUM counters, chunk count, 64 + 64 per-lane counts and degree sums
   DevBuf<unsigned long long> tacc;   // per-lane tuple-weight sums of a retirement
-  unsigned long long* h = nullptr;   // pinned: [0, M_NUM) counters, [16, 80) weight sums, [80, 208) lane counts
-  hipEvent_t ev = nullptr;
+  unsigned long long* h = nullptr;   // pinned, coherent: [0, M_NUM) counters, [16, 80) weight sums,
+                                     // [80, 208) lane counts, [kSvSeq] the last published level
+  unsigned long long* dh = nullptr;  // the same block as the device sees it
+  uint64_t pub_seq = 0;              // sequence number of the level in flight
   hipEvent_t ev_start = nullptr, ev_end = nullptr;  // timing of the level in flight
   g5_server_graph_stats st{};
   std::vector<std::unique_ptr<DevBuf<uint8_t>>> bufs;
@@ -306,7 +324,6 @@ struct Engine {
 
   ~Engine() {
     if (h) (void)hipHostFree(h);
-    if (ev) (void)hipEventDestroy(ev);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_end) (void)hipEventDestroy(ev_end);
   }
@@ -347,8 +364,9 @@ static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
   e.chunk_list.alloc(e.nchunks + 1);
   e.cnt.alloc(M_NUM + 1 + 128);
   e.tacc.alloc(64);
-  G5_HIP(hipHostMalloc((void**)&e.h, 224 * sizeof(unsigned long long), hipHostMallocDefault));
-  G5_HIP(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
+  G5_HIP(hipHostMalloc((void**)&e.h, 224 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
+  for (int i = 0; i < 224; ++i) e.h[i] = 0;
+  G5_HIP(hipHostGetDevicePointer((void**)&e.dh, e.h, 0));
   G5_HIP(hipEventCreate(&e.ev_start));
   G5_HIP(hipEventCreate(&e.ev_end));
   hipStream_t s = e.s;
@@ -596,10 +614,9 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
                        nz_next, g.offsets.p, e.rows, c, keep, epi);
   }
   G5_HIP(hipGetLastError());
-  G5_HIP(hipMemcpyAsync(e.h, c, M_NUM * 8, hipMemcpyDeviceToHost, s));
-  G5_HIP(hipMemcpyAsync(e.h + 80, c + M_NUM + 1, 128 * 8, hipMemcpyDeviceToHost, s));
   G5_HIP(hipEventRecord(e.ev_end, s));
-  G5_HIP(hipEventRecord(e.ev, s));
+  hipLaunchKernelGGL(k_sv_publish, dim3(1), dim3(kBlock), 0, s, c, e.dh, ++e.pub_seq);
+  G5_HIP(hipGetLastError());
   e.level = l;
   e.busy = true;
   ++e.levels_run;
@@ -608,11 +625,19 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
 // One non-blocking step of an engine; returns true if it made progress.
 static bool sv_step(g5_server* srv, Engine& e, double t) {
   if (e.busy) {
-    const hipError_t q = hipEventQuery(e.ev);
-    if (q == hipErrorNotReady) return false;
-    G5_HIP(q);
+    volatile unsigned long long* seq = e.h + kSvSeq;
+    if (*seq != e.pub_seq) {
+      // not published yet; a stream that has finished without publishing
+      // failed
+      const hipError_t q = hipStreamQuery(e.s);
+      if (q == hipErrorNotReady) return false;
+      G5_HIP(q);
+      G5_CHECK(*seq == e.pub_seq, "serving: a level's counters were not published");
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
     e.busy = false;
     float lms = 0;
+    G5_HIP(hipEventSynchronize(e.ev_end));  // recorded before the publish: already complete
     if (hipEventElapsedTime(&lms, e.ev_start, e.ev_end) == hipSuccess) e.st.busy_ms += lms;
     sv_complete_level(srv, e, t);
   }
