@@ -420,13 +420,15 @@ int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* t) {
   return guarded(ctx, [&] {
     G5_CHECK(t && t->bin_edges >= 0 && t->td_pieces >= 0 && t->td_pieces <= 4096 &&
              (t->block_shift == 0 || (t->block_shift >= 12 && t->block_shift <= 31)) && t->td_head >= 0 &&
-             t->seg_max >= 0 && t->seg_max < (INT64_C(1) << 24),
+             t->seg_max >= 0 && t->seg_max < (INT64_C(1) << 24) && t->small_edges >= 0 &&
+             t->small_edges <= (INT64_C(1) << 20),
              "g5_bfs_set_tuning: bad tuning");
     ctx->bfs.bin_edges = t->bin_edges;
     ctx->bfs.td_pieces = t->td_pieces;
     ctx->bfs.hub_shift = t->block_shift;
     ctx->bfs.td_head = t->td_head;
     ctx->bfs.seg_max = t->seg_max;
+    ctx->bfs.small_edges = t->small_edges;
     if (ctx->bfs.prepared) {
       // segment buffers and block index for the new tuning, outside any run
       set_device(ctx);
@@ -444,6 +446,7 @@ int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* t) {
     t->block_shift = ctx->bfs.hub_shift;
     t->td_head = ctx->bfs.td_head;
     t->seg_max = ctx->bfs.seg_max;
+    t->small_edges = ctx->bfs.small_edges;
   });
 }
 

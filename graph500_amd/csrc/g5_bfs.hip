@@ -843,36 +843,148 @@ __global__ void k_make_levels(const uint32_t* __restrict__ old2new, const uint8_
   }
 }
 
-// Level end: the counters the host decides the next level with (and the
-// level's frontier edge count after a top-down level) are written straight
-// into coherent host memory, the sequence number last, so the host need not
-// wait for a copy and a stream synchronisation.
-enum { P_SEQ = C_NUM, P_NUM = C_NUM + 1 };
+// --- fused small top-down levels ---------------------------------------------
+// Consecutive top-down levels whose frontiers are small (at most kFuseNf
+// vertices with at most max_edges edges) run inside one launch of one
+// workgroup: per level, the frontier's degree prefix in LDS, its edges taken
+// kFuseThreads at a time (a lane finds its row by binary search in LDS), the
+// visited bitmap test-and-set (plain-load filter, then atomicOr) and the next
+// queue; between levels the workgroup applies the reference's switch rule
+// (mpi/bfs.hpp:2799-2829: a growing top-down level that finds more vertices
+// than the previous one and more than n_active/2000 hands over to bottom-up),
+// so these levels cost no host round trip and no further launches.  The
+// launch stops when a level finds nothing, before a level that pulls, before
+// a frontier too large for it, or at max_levels.  Each level's vertex count
+// and, at the end, the launch's state are published to coherent host memory,
+// every value tagged with the launch's sequence number.
+constexpr int kFuseThreads = 1024;
+constexpr int kFuseNf = 4096;
+constexpr int kFuseMaxLevels = 64;
+enum { F_LEVELS = kFuseMaxLevels, F_STOP, F_QI, F_NF, F_E, F_NUM };
+enum { FUSE_EMPTY = 0, FUSE_PULL = 1, FUSE_LARGE = 2, FUSE_LIMIT = 3 };
 
-__global__ void k_publish(unsigned long long* __restrict__ counters, const uint64_t* __restrict__ prefix,
-                          int64_t nf, int forward, uint64_t* pub, uint64_t seq) {
-  for (int i = 0; i < C_NUM; ++i) {
-    pub[i] = counters[i];
-    counters[i] = 0;  // ready for the next level, with no fill launched before it
+__global__ void __launch_bounds__(kFuseThreads) k_td_fused(
+    uint32_t* __restrict__ q0, uint32_t* __restrict__ q1, int qi0, int64_t nf0,
+    const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ cols, uint64_t* vis,
+    uint8_t* __restrict__ level, uint32_t* __restrict__ parent, uint32_t lvl0, int max_levels, int64_t max_edges,
+    double na, int64_t prev_nq0, int growing, Published* out, uint64_t seq) {
+  __shared__ uint64_t s_pre[kFuseNf + 1];
+  __shared__ uint32_t s_src[kFuseNf];
+  __shared__ uint64_t s_wave[kFuseThreads / 64];
+  __shared__ uint32_t s_cnt;
+  constexpr int kPer = kFuseNf / kFuseThreads;  // frontier entries per thread in the scan
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  int qi = qi0;
+  int64_t nf = nf0, prev_nq = prev_nq0;
+  uint64_t stop = FUSE_LIMIT, E = 0;
+  int k = 0;
+  while (k < max_levels) {
+    const uint32_t* cur = qi ? q1 : q0;
+    uint32_t* nxt = qi ? q0 : q1;
+    if (nf > kFuseNf) {
+      stop = FUSE_LARGE;
+      E = 0;  // not known here
+      break;
+    }
+    // degrees of the frontier, exclusive prefix in LDS (kPer per thread)
+    uint64_t d[kPer], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int64_t i = (int64_t)t * kPer + j;
+      d[j] = 0;
+      if (i < nf) {
+        const uint32_t u = cur[i];
+        s_src[i] = u;
+        d[j] = offsets[u + 1] - offsets[u];
+      }
+      sum += d[j];
+    }
+    uint64_t inc = sum;  // inclusive scan of the per-thread sums within the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t x = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += x;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    if (t == 0) s_cnt = 0;
+    __syncthreads();
+    uint64_t base = 0;
+    for (int w = 0; w < wave; ++w) base += s_wave[w];
+    uint64_t run = base + inc - sum;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int64_t i = (int64_t)t * kPer + j;
+      if (i < nf) s_pre[i] = run;
+      run += d[j];
+    }
+    if (t == kFuseThreads - 1) s_pre[nf] = run;
+    __syncthreads();
+    E = s_pre[nf];
+    if ((int64_t)E > max_edges) {
+      stop = FUSE_LARGE;
+      break;
+    }
+    const uint32_t lvl = lvl0 + (uint32_t)k;
+    for (uint64_t e = t; e < E; e += kFuseThreads) {
+      int lo = 0, hi = (int)nf - 1;  // last row starting at or before e
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[mid] <= e) lo = mid; else hi = mid - 1;
+      }
+      const uint32_t u = s_src[lo];
+      const uint32_t w = cols[offsets[u] + (e - s_pre[lo])];
+      const uint64_t bit = UINT64_C(1) << (w & 63);
+      if (vis[w >> 6] & bit) continue;
+      const uint64_t old = atomicOr((unsigned long long*)(vis + (w >> 6)), (unsigned long long)bit);
+      if (old & bit) continue;
+      level[w] = (uint8_t)lvl;
+      parent[w] = u;
+      nxt[atomicAdd(&s_cnt, 1u)] = w;
+    }
+    __syncthreads();
+    const int64_t nq = s_cnt;
+    if (t == 0) publish(out + k, (uint64_t)nq, seq);
+    const bool pull = growing && nq > prev_nq && (double)nq > na / 2000.0;
+    prev_nq = nq;
+    nf = nq;
+    qi ^= 1;
+    ++k;
+    __syncthreads();  // every lane has read s_cnt and s_pre before the next level writes them
+    if (nq == 0) {
+      stop = FUSE_EMPTY;
+      break;
+    }
+    if (pull) {
+      stop = FUSE_PULL;
+      break;
+    }
   }
-  if (forward) pub[1] = prefix[nf];
-  __threadfence_system();
-  pub[P_SEQ] = seq;
+  if (t == 0) {
+    publish(out + F_LEVELS, (uint64_t)k, seq);
+    publish(out + F_STOP, stop, seq);
+    publish(out + F_QI, (uint64_t)qi, seq);
+    publish(out + F_NF, (uint64_t)nf, seq);
+    publish(out + F_E, E, seq);
+  }
 }
 
-// Spins until the level's counters are published; a stream that finished or
-// failed without publishing ends the wait with an error.
-static void wait_published(BfsState& b, hipStream_t s) {
-  volatile uint64_t* h = b.h_pub;
-  for (uint64_t spins = 1; h[P_SEQ] != b.pub_seq; ++spins) {
-    if (spins % 1024) continue;
-    const hipError_t e = hipStreamQuery(s);
-    if (e == hipErrorNotReady) continue;
-    G5_HIP(e);
-    G5_CHECK(h[P_SEQ] == b.pub_seq, "g5_run_bfs: level counters were not published");
+// Level end: the counters the host decides the next level with (and the
+// level's frontier edge count after a top-down level) are published straight
+// into coherent host memory, each tagged with the level's sequence number, so
+// the host need not wait for a copy and a stream synchronisation.
+__global__ void k_publish(unsigned long long* __restrict__ counters, const uint64_t* __restrict__ prefix,
+                          int64_t nf, int forward, Published* pub, uint64_t seq) {
+  for (int i = 0; i < C_NUM; ++i) {
+    const uint64_t v = (forward && i == 1) ? prefix[nf] : (uint64_t)counters[i];
+    counters[i] = 0;  // ready for the next level, with no fill launched before it
+    publish(pub + i, v, seq);
   }
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
-  for (int i = 0; i < C_NUM; ++i) b.h_counters[i] = h[i];
+}
+
+// Waits for the level's counters (see PublishBlock::wait).
+static void wait_published(BfsState& b, hipStream_t s) {
+  b.pub.wait(0, C_NUM, b.pub_seq, s, "g5_run_bfs: level counters were not published");
+  for (int i = 0; i < C_NUM; ++i) b.h_counters[i] = b.pub.value(i);
 }
 
 static void build_hub_index(g5_ctx* ctx);
@@ -920,10 +1032,10 @@ void bfs_prepare(g5_ctx* ctx) {
   b.counters.alloc(C_NUM);
   bfs_tuning_buffers(ctx);
   G5_HIP(hipHostMalloc((void**)&b.h_counters, C_NUM * sizeof(uint64_t), hipHostMallocDefault));
-  G5_HIP(hipHostMalloc((void**)&b.h_pub, P_NUM * sizeof(uint64_t), hipHostMallocCoherent | hipHostMallocMapped));
-  for (int i = 0; i < P_NUM; ++i) b.h_pub[i] = 0;
-  G5_HIP(hipHostGetDevicePointer((void**)&b.d_pub, b.h_pub, 0));
+  b.pub.alloc(C_NUM);
   b.pub_seq = 0;
+  b.fuse.alloc(F_NUM);
+  b.fuse_seq = 0;
   G5_HIP(hipEventCreate(&b.ev0));
   G5_HIP(hipEventCreate(&b.ev1));
   b.lev_begin.assign(G5_MAX_LEVELS, nullptr);
@@ -993,11 +1105,61 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     st.direction[1] = 1;
     st.nq[1] = 0;
   }
+  // a fused launch that found the next frontier too large hands that level
+  // to the per-level path
+  bool fuse_next = true;
   while (root_active) {
     ++lvl;
     G5_CHECK(lvl < 255, "g5_run_bfs: more than 254 levels (uint8 level array)");
     const bool forward = next_forward;
     G5_HIP(hipEventRecord(b.lev_begin[lvl], s));
+    if (forward && fuse_next && b.small_edges > 0 && nf <= kFuseNf &&
+        (e_next < 0 || e_next <= b.small_edges)) {
+      // small top-down levels, as many as fit, in one single-workgroup launch
+      const int max_levels = std::min(kFuseMaxLevels, 254 - lvl);
+      ++b.fuse_seq;
+      hipLaunchKernelGGL(k_td_fused, dim3(1), dim3(kFuseThreads), 0, s, b.queue[0].p, b.queue[1].p, qi, nf,
+                         g.offsets.p, g.cols.p, b.vis[cur].p, b.level.p, b.parent.p, (uint32_t)lvl, max_levels,
+                         b.small_edges, na, prev_nq, growing ? 1 : 0, b.fuse.d, b.fuse_seq);
+      G5_HIP(hipGetLastError());
+      G5_HIP(hipEventRecord(b.lev_end[lvl], s));
+      const char* unpublished = "g5_run_bfs: fused levels were not published";
+      b.fuse.wait(F_LEVELS, F_NUM, b.fuse_seq, s, unpublished);
+      const int k = (int)b.fuse.value(F_LEVELS);
+      b.fuse.wait(0, k, b.fuse_seq, s, unpublished);
+      const uint64_t stop = b.fuse.value(F_STOP);
+      if (k == 0) {
+        // the first frontier was already too large: this level runs per level
+        fuse_next = false;
+        e_next = (int64_t)b.fuse.value(F_E);  // its edge count, when the launch summed it (0: not known)
+        if (e_next == 0) e_next = -1;
+        --lvl;
+        continue;
+      }
+      for (int j = 0; j < k; ++j) {
+        const int l = lvl + j;
+        const int64_t nq = (int64_t)b.fuse.value(j);
+        st.direction[l] = 1;
+        st.nq[l] = nq;
+        st.num_levels = l + 1;
+        visited += nq;
+        if (j > 0) {
+          // the launch's time is the first fused level's; the others show 0
+          G5_HIP(hipEventRecord(b.lev_begin[l], s));
+          G5_HIP(hipEventRecord(b.lev_end[l], s));
+        }
+      }
+      prev_nq = (int64_t)b.fuse.value(k - 1);
+      qi = (int)b.fuse.value(F_QI);
+      nf = (int64_t)b.fuse.value(F_NF);
+      lvl += k - 1;
+      if (stop == FUSE_EMPTY) break;
+      next_forward = stop != FUSE_PULL;
+      fuse_next = stop != FUSE_LARGE;
+      e_next = stop == FUSE_LARGE && b.fuse.value(F_E) > 0 ? (int64_t)b.fuse.value(F_E) : -1;
+      continue;
+    }
+    fuse_next = true;
     if (forward) {
       device_scan(FrontierDegree{b.queue[qi].p, g.offsets.p, nf}, StoreU64{b.prefix.p},
                   (size_t)nf + 1, b.scan_tmp.p, s);
@@ -1127,7 +1289,7 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(b.lev_end[lvl], s));
     ++b.pub_seq;
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, cnt, b.prefix.p, nf, forward ? 1 : 0, b.d_pub,
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, cnt, b.prefix.p, nf, forward ? 1 : 0, b.pub.d,
                        b.pub_seq);
     G5_HIP(hipGetLastError());
     wait_published(b, s);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "g5_internal.hpp"
+#include "g5_publish.hpp"
 
 #include <memory>
 
@@ -57,9 +58,11 @@ struct BfsState {
   DevBuf<uint64_t> scan_tmp;
   DevBuf<uint64_t> counters;      // device counters (see g5_bfs.hip)
   uint64_t* h_counters = nullptr; // pinned host mirror
-  uint64_t* h_pub = nullptr;      // coherent host memory the level counters are published to
-  uint64_t* d_pub = nullptr;      // its device address
+  PublishBlock pub;               // the level counters, published by k_publish
   uint64_t pub_seq = 0;
+  PublishBlock fuse;              // the fused small levels' state (g5_bfs.hip)
+  uint64_t fuse_seq = 0;
+  int64_t small_edges = 32768;    // small top-down levels up to this many edges run fused (0: never)
   DevBuf<int64_t> pred_out;       // n, built on demand by get_pred
   DevBuf<uint8_t> level_out;      // n, built on demand by get_levels
   int64_t bin_edges = 0;  // top-down levels with at least this many edges are binned (0: never)
@@ -105,8 +108,8 @@ struct BfsState {
     hub_built_shift = 0;
     if (h_counters) (void)hipHostFree(h_counters);
     h_counters = nullptr;
-    if (h_pub) (void)hipHostFree(h_pub);
-    h_pub = d_pub = nullptr;
+    pub.release();
+    fuse.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     ev0 = ev1 = nullptr;

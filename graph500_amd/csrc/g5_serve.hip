@@ -108,20 +108,20 @@ struct ServeLocal {
   }
 };
 
-// A level's counters into the engine's pinned host block ([0, M_NUM) the
-// level counters, [80, 208) the per-lane counts and degree sums), then, after
-// a system-scope fence, the level's sequence number into slot kSvSeq: the
-// host learns that the level finished by polling host memory, with no copy
+// A level's counters into the engine's coherent host block ([0, M_NUM) the
+// level counters, [M_NUM, M_NUM + 128) the per-lane counts and degree sums,
+// [kSvTeps, kSvTeps + 64) the per-lane weight sums of the last retirement),
+// each tagged with the level's sequence number: the host learns that the
+// level finished, and its counters, by polling host memory, with no copy
 // commands on the stream.
-constexpr int kSvSeq = 208;
-__global__ void k_sv_publish(const unsigned long long* __restrict__ c, unsigned long long* __restrict__ h,
-                             uint64_t seq) {
+constexpr int kSvTeps = M_NUM + 128;
+constexpr int kSvPub = kSvTeps + 64;
+__global__ void k_sv_publish(const unsigned long long* __restrict__ c, const unsigned long long* __restrict__ tacc,
+                             Published* __restrict__ p, uint64_t seq) {
   const int t = threadIdx.x;
-  if (t < M_NUM) h[t] = c[t];
-  if (t < 128) h[80 + t] = c[M_NUM + 1 + t];
-  __threadfence_system();
-  __syncthreads();
-  if (t == 0) h[kSvSeq] = seq;
+  if (t < M_NUM) publish(p + t, c[t], seq);
+  if (t < 128) publish(p + M_NUM + t, c[M_NUM + 1 + t], seq);
+  if (t < 64) publish(p + kSvTeps + t, tacc[t], seq);
 }
 
 struct ServeRoots {
@@ -294,9 +294,8 @@ struct Engine {
   DevBuf<uint32_t> chunk_rows, chunk_list;
   DevBuf<unsigned long long> cnt;    // M_NUM counters, chunk count, 64 + 64 per-lane counts and degree sums
   DevBuf<unsigned long long> tacc;   // per-lane tuple-weight sums of a retirement
-  unsigned long long* h = nullptr;   // pinned, coherent: [0, M_NUM) counters, [16, 80) weight sums,
-                                     // [80, 208) lane counts, [kSvSeq] the last published level
-  unsigned long long* dh = nullptr;  // the same block as the device sees it
+  unsigned long long* h = nullptr;   // pinned: the per-lane weight sums of a retirement, copied at the end
+  PublishBlock pub;                  // the level counters and per-lane counts (k_sv_publish)
   uint64_t pub_seq = 0;              // sequence number of the level in flight
   hipEvent_t ev_start = nullptr, ev_end = nullptr;  // timing of the level in flight
   g5_server_graph_stats st{};
@@ -364,9 +363,9 @@ static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
   e.chunk_list.alloc(e.nchunks + 1);
   e.cnt.alloc(M_NUM + 1 + 128);
   e.tacc.alloc(64);
-  G5_HIP(hipHostMalloc((void**)&e.h, 224 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
-  for (int i = 0; i < 224; ++i) e.h[i] = 0;
-  G5_HIP(hipHostGetDevicePointer((void**)&e.dh, e.h, 0));
+  G5_HIP(hipMemsetAsync(e.tacc.p, 0, 64 * 8, e.s));
+  G5_HIP(hipHostMalloc((void**)&e.h, 64 * sizeof(unsigned long long), hipHostMallocDefault));
+  e.pub.alloc(kSvPub);
   G5_HIP(hipEventCreate(&e.ev_start));
   G5_HIP(hipEventCreate(&e.ev_end));
   hipStream_t s = e.s;
@@ -392,16 +391,22 @@ static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
   G5_HIP(hipStreamSynchronize(s));
 }
 
-// Consumes the traversed-edge sums of the last retirement (stream order puts
-// them before the level whose completion is being processed).
-static void sv_take_teps(g5_server* srv, Engine& e) {
+// Consumes the traversed-edge sums of the last retirement: from the level
+// whose completion is being processed (its publish carries them; stream order
+// puts the retirement's k_sv_teps before that level), or, when no level
+// followed the retirement, from a copy once the stream has drained.
+static void sv_take_teps(g5_server* srv, Engine& e, bool from_level) {
   if (!e.teps_pending) return;
+  if (!from_level) {
+    G5_HIP(hipMemcpyAsync(e.h, e.tacc.p, 64 * 8, hipMemcpyDeviceToHost, e.s));
+    G5_HIP(hipStreamSynchronize(e.s));
+  }
   uint64_t m = e.teps_mask;
   while (m) {
     const int r = __builtin_ctzll(m);
     m &= m - 1;
     Request& q = srv->req[e.teps_req[r]];
-    q.edges = (int64_t)(e.h[16 + r] / 2);
+    q.edges = (int64_t)((from_level ? e.pub.value(kSvTeps + r) : e.h[r]) / 2);
   }
   e.teps_pending = false;
   e.teps_mask = 0;
@@ -409,9 +414,9 @@ static void sv_take_teps(g5_server* srv, Engine& e) {
 
 // Processes a finished level; returns true if anything changed.
 static void sv_complete_level(g5_server* srv, Engine& e, double t) {
-  sv_take_teps(srv, e);
-  const uint64_t live = e.h[M_LIVE];
-  e.last_fdeg = e.h[M_FDEG];
+  sv_take_teps(srv, e, true);
+  const uint64_t live = e.pub.value(M_LIVE);
+  e.last_fdeg = e.pub.value(M_FDEG);
   e.cooling = 0;  // cleared from seen by the level just finished
   const uint64_t finished = e.active & ~live;
   // each search's own direction for its next level
@@ -420,8 +425,8 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
   while (m) {
     const int r = __builtin_ctzll(m);
     m &= m - 1;
-    const int64_t nq = (int64_t)e.h[80 + r];
-    const double fdeg = (double)e.h[144 + r];
+    const int64_t nq = (int64_t)e.pub.value(M_NUM + r);
+    const double fdeg = (double)e.pub.value(M_NUM + 64 + r);
     e.lane_visited[r] += nq;
     e.lane_vdeg[r] += fdeg;
     const int64_t unvisited = (int64_t)na - e.lane_visited[r];
@@ -451,7 +456,6 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
     hipLaunchKernelGGL(k_sv_teps, dim3(grid_of(e.rows, kBlock, 2048)), dim3(kBlock), 0, e.s, e.seen.p,
                        e.ctx->graph.weight.p, e.rows, finished, e.tacc.p);
     G5_HIP(hipGetLastError());
-    G5_HIP(hipMemcpyAsync(e.h + 16, e.tacc.p, 64 * 8, hipMemcpyDeviceToHost, e.s));
     e.teps_pending = true;
     e.teps_mask = finished;
     uint64_t m = finished;
@@ -615,7 +619,7 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
   }
   G5_HIP(hipGetLastError());
   G5_HIP(hipEventRecord(e.ev_end, s));
-  hipLaunchKernelGGL(k_sv_publish, dim3(1), dim3(kBlock), 0, s, c, e.dh, ++e.pub_seq);
+  hipLaunchKernelGGL(k_sv_publish, dim3(1), dim3(kBlock), 0, s, c, e.tacc.p, e.pub.d, ++e.pub_seq);
   G5_HIP(hipGetLastError());
   e.level = l;
   e.busy = true;
@@ -625,16 +629,14 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
 // One non-blocking step of an engine; returns true if it made progress.
 static bool sv_step(g5_server* srv, Engine& e, double t) {
   if (e.busy) {
-    volatile unsigned long long* seq = e.h + kSvSeq;
-    if (*seq != e.pub_seq) {
+    if (!e.pub.ready(0, kSvPub, e.pub_seq)) {
       // not published yet; a stream that has finished without publishing
       // failed
       const hipError_t q = hipStreamQuery(e.s);
       if (q == hipErrorNotReady) return false;
       G5_HIP(q);
-      G5_CHECK(*seq == e.pub_seq, "serving: a level's counters were not published");
+      G5_CHECK(e.pub.ready(0, kSvPub, e.pub_seq), "serving: a level's counters were not published");
     }
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
     e.busy = false;
     float lms = 0;
     G5_HIP(hipEventSynchronize(e.ev_end));  // recorded before the publish: already complete
@@ -715,7 +717,7 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
       sv_complete_level(srv, *e, now_s() - t0);
       G5_HIP(hipStreamSynchronize(e->s));
     }
-    sv_take_teps(srv, *e);
+    sv_take_teps(srv, *e, false);
   }
   const double wall = now_s() - t0;
   int64_t edges = 0, levels1 = 0, busy1 = 0, events1 = 0;

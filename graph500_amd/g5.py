@@ -376,9 +376,10 @@ class G5Context:
         L = self.bfs_stats()["num_levels"]
         return per[1:L].tolist(), tot.value
 
-    BFS_TUNING = ("bin_edges", "td_pieces", "block_shift", "td_head", "seg_max")
+    BFS_TUNING = ("bin_edges", "td_pieces", "block_shift", "td_head", "seg_max", "small_edges")
 
-    def bfs_set_tuning(self, bin_edges=None, td_pieces=None, block_shift=None, td_head=None, seg_max=None):
+    def bfs_set_tuning(self, bin_edges=None, td_pieces=None, block_shift=None, td_head=None, seg_max=None,
+                       small_edges=None):
         """Single-root tuning; a knob left None keeps its value.  bin_edges:
         top-down levels with at least this many frontier edges are binned by
         target range (0 = never).  td_pieces: a growing bulk top-down level
@@ -390,10 +391,13 @@ class G5Context:
         degree-ordered rows) with a bottom-up probe and skips their edges
         (0 = off; degree-ordered graphs only).  seg_max: most segments of a
         segment-list level (0 = 2^24 - 1); a level beyond it is taken in row
-        order (tests lower it to reach that fallback on small graphs)."""
+        order (tests lower it to reach that fallback on small graphs).
+        small_edges: consecutive top-down levels with at most 4096 frontier
+        vertices and this many frontier edges run in one launch that applies
+        the direction rule on the device (0 = off)."""
         t = np.zeros(len(self.BFS_TUNING), np.int64)
         self._check(self._lib.g5_bfs_get_tuning(self._h, _ptr(t)))
-        for i, v in enumerate((bin_edges, td_pieces, block_shift, td_head, seg_max)):
+        for i, v in enumerate((bin_edges, td_pieces, block_shift, td_head, seg_max, small_edges)):
             if v is not None:
                 t[i] = v
         self._check(self._lib.g5_bfs_set_tuning(self._h, _ptr(t)))

@@ -234,8 +234,8 @@ int g5_bfs_stats_get(g5_ctx* ctx, g5_bfs_stats* out);
  * G5_MAX_LEVELS entries, index = level. */
 int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
 /* Single-root tuning (no counterpart in the reference; levels do not depend
- * on it, parents may).  All knobs act on bulk top-down levels (more than 2^20
- * frontier edges).
+ * on it, parents may).  All knobs but small_edges act on bulk top-down levels
+ * (more than 2^20 frontier edges); small_edges acts on the small ones.
  *   bin_edges: levels with at least this many frontier edges first group
  *     their (target, source) pairs by target range (0 = never, the default:
  *     twice as slow as the plain level at SCALE 26-28).
@@ -253,13 +253,18 @@ int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
  *     field's limit, 2^24 - 1, the default; 1..2^24 - 1 lower it).  A level
  *     whose frontier rows (suffix mode) or pieces would exceed it is taken
  *     in plain row order instead.  Only tests lower it, to reach that path
- *     on small graphs. */
+ *     on small graphs.
+ *   small_edges: consecutive top-down levels whose frontiers have at most
+ *     4096 vertices and at most small_edges edges run in one single-workgroup
+ *     launch that applies the direction rule itself, with no host round trip
+ *     between them (default 32768; 0 = off; at most 2^20). */
 typedef struct g5_bfs_tuning {
   int64_t bin_edges;
   int64_t td_pieces;
   int64_t block_shift;
   int64_t td_head;
   int64_t seg_max;
+  int64_t small_edges;
 } g5_bfs_tuning;
 int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* tuning);
 int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* tuning);

@@ -133,7 +133,9 @@ def test_bfs_64_roots_bit_exact_s16(ctx, s16, oracle_mod):
                                           (22, {"td_head": 4, "td_pieces": 64}),
                                           (20, {"td_head": 4, "seg_max": 8}),
                                           (22, {"td_head": 4, "td_pieces": 64, "seg_max": 1000}),
-                                          (22, {"td_head": 0, "block_shift": 14, "seg_max": 2000})])
+                                          (22, {"td_head": 0, "block_shift": 14, "seg_max": 2000}),
+                                          (20, {"small_edges": 0}), (22, {"small_edges": 1 << 20}),
+                                          (20, {"small_edges": 64}), (24, {"small_edges": 0})])
 def test_bfs_reference_fixtures_larger_scales(scale, tuning, oracle_mod):
     """An empty tuning keeps the defaults; bin_edges 1 bins every bulk
     top-down level; td_pieces 0 takes bulk rows in row order, 1 through a
@@ -142,7 +144,11 @@ def test_bfs_reference_fixtures_larger_scales(scale, tuning, oracle_mod):
     turns target blocks off, 12 at SCALE 22 indexes only the first rows, so
     both indexed and whole-row segments occur.  seg_max lowers the segment
     count a segment-list level may hold, so that hub levels whose frontier
-    rows (suffix mode) or pieces exceed it take the row-order fallback."""
+    rows (suffix mode) or pieces exceed it take the row-order fallback.
+    small_edges 0 runs every level through the per-level path, 2^20 fuses
+    every top-down level with at most 4096 frontier vertices, 64 makes the
+    fused launch stop at most frontiers (too many edges), often before its
+    first level."""
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
     say("scale", scale, "tuning", tuning)
@@ -291,6 +297,10 @@ def test_error_paths():
             c.bfs_set_tuning(block_shift=5)  # blocks below 2^12 ids
         with pytest.raises(E):
             c.bfs_set_tuning(seg_max=1 << 24)  # beyond the segment count field
+        with pytest.raises(E):
+            c.bfs_set_tuning(small_edges=(1 << 20) + 1)  # beyond the largest fused level
+        with pytest.raises(E):
+            c.bfs_set_tuning(small_edges=-1)
         c.generate_kronecker(8)
         with pytest.raises(E):
             c.construct(7)  # unknown reorder mode
