@@ -421,7 +421,7 @@ int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* t) {
     G5_CHECK(t && t->bin_edges >= 0 && t->td_pieces >= 0 && t->td_pieces <= 4096 &&
              (t->block_shift == 0 || (t->block_shift >= 12 && t->block_shift <= 31)) && t->td_head >= 0 &&
              t->seg_max >= 0 && t->seg_max < (INT64_C(1) << 24) && t->small_edges >= 0 &&
-             t->small_edges <= (INT64_C(1) << 20),
+             t->small_edges <= (INT64_C(1) << 20) && t->tail_pull >= 0 && t->tail_pull <= 64,
              "g5_bfs_set_tuning: bad tuning");
     ctx->bfs.bin_edges = t->bin_edges;
     ctx->bfs.td_pieces = t->td_pieces;
@@ -429,6 +429,7 @@ int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* t) {
     ctx->bfs.td_head = t->td_head;
     ctx->bfs.seg_max = t->seg_max;
     ctx->bfs.small_edges = t->small_edges;
+    ctx->bfs.tail_pull = t->tail_pull;
     if (ctx->bfs.prepared) {
       // segment buffers and block index for the new tuning, outside any run
       set_device(ctx);
@@ -447,6 +448,7 @@ int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* t) {
     t->td_head = ctx->bfs.td_head;
     t->seg_max = ctx->bfs.seg_max;
     t->small_edges = ctx->bfs.small_edges;
+    t->tail_pull = ctx->bfs.tail_pull;
   });
 }
 

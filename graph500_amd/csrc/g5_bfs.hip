@@ -20,9 +20,15 @@
 //              While the search grows, a hub level first settles its
 //              highest-degree targets bottom-up (k_td_head) and expands only
 //              the row suffixes beyond them (k_td_expand_segs).
+//              After the bottom-up phase, a top-down level whose frontier is
+//              large against the unvisited vertices is found by the bottom-up
+//              kernel (same vertices, far fewer probes); consecutive small
+//              top-down levels run fused in one single-workgroup launch
+//              (k_td_fused) that applies the switch rule itself.
 //   host loop  one round trip per level: a one-lane kernel publishes the
-//              level's counters to coherent host memory, where the host
-//              reads them for the alpha/beta rule.
+//              level's counters to coherent host memory, each value tagged
+//              with the level's sequence number, where the host reads them
+//              for the alpha/beta rule.
 //   queues     a top-down frontier that comes from a bitmap is rebuilt by an
 //              exclusive scan of per-word popcounts (ascending vertex order).
 //   bottom-up  k_bu: one wave per two 64-vertex bitmap words, one lane per
@@ -670,54 +676,65 @@ __global__ void __launch_bounds__(kBlock) k_td_finish(const uint8_t* __restrict_
   block_add(total, counters + C_FOUND);
 }
 
-// Bottom-up level: two 64-vertex bitmap words per wave and step (one lane per
-// vertex of each), so both words' first-neighbour probes are in flight
-// together; a pair whose vertices are all visited is skipped whole.
+// Bottom-up level.  A wave reads `span` bitmap words at a time (one
+// coalesced load, one word per lane, span a power of two from 2 to 64),
+// copies the words whose vertices are all visited to the next bitmap (one
+// coalesced store) and walks the others two at a time, one lane per vertex
+// of each, so both words' first-neighbour probes are in flight together.
+// The host sizes span so that the grid still has a step per wave or more;
+// late levels, where almost every word is full, then cost little more than
+// a pass over the bitmap.
 __global__ void __launch_bounds__(kBlock) k_bu(
     const uint64_t* __restrict__ vis, uint64_t* __restrict__ vis_next,
     const uint32_t* __restrict__ first_nbr, const uint64_t* __restrict__ offsets,
     const uint32_t* __restrict__ cols, uint8_t* __restrict__ level, uint32_t* __restrict__ parent,
-    int64_t words, uint32_t lvl, unsigned long long* __restrict__ counters) {
+    int64_t words, int span, uint32_t lvl, unsigned long long* __restrict__ counters) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
   uint64_t found_total = 0;
-  for (int64_t w0 = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 2; w0 < words; w0 += nwaves * 2) {
-    const int64_t w1 = w0 + 1;
-    const bool has1 = w1 < words;
-    const uint64_t c0 = vis[w0];
-    const uint64_t c1 = has1 ? vis[w1] : ~UINT64_C(0);  // bits past the last row read as visited
-    if (c0 == ~UINT64_C(0) && c1 == ~UINT64_C(0)) {
+  for (int64_t base = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * span; base < words;
+       base += nwaves * span) {
+    const int64_t wl = base + lane;
+    const bool mine = lane < span && wl < words;
+    const uint64_t cl = mine ? vis[wl] : ~UINT64_C(0);  // lanes beyond the span or the end read as visited
+    uint64_t open = __ballot(cl != ~UINT64_C(0));        // words with an unvisited vertex
+    if (mine && cl == ~UINT64_C(0)) vis_next[wl] = cl;
+    while (open) {
+      // the next two open words (identical in every lane)
+      const int j0 = __builtin_ctzll(open);
+      open &= open - 1;
+      const bool has1 = open != 0;
+      const int j1 = has1 ? __builtin_ctzll(open) : j0;
+      if (has1) open &= open - 1;
+      const uint64_t c0 = __shfl(cl, j0, 64);
+      const uint64_t c1 = has1 ? __shfl(cl, j1, 64) : ~UINT64_C(0);
+      const int64_t w0 = base + j0, w1 = base + j1;
+      const uint32_t v0 = (uint32_t)(w0 * 64 + lane), v1 = (uint32_t)(w1 * 64 + lane);
+      const bool a0 = !((c0 >> lane) & 1), a1 = !((c1 >> lane) & 1);
+      const uint32_t f0 = a0 ? first_nbr[v0] : kNoNbr;
+      const uint32_t f1 = a1 ? first_nbr[v1] : kNoNbr;
+      const bool h0 = f0 != kNoNbr && bit_set(vis, f0);
+      const bool h1 = f1 != kNoNbr && bit_set(vis, f1);
+      uint32_t p0 = f0, p1 = f1;
+      // every lane makes both calls (the long-row scan needs the whole wave)
+      const bool r0 = row_marked_nbr(f0 != kNoNbr && !h0, v0, vis, offsets, cols, p0);
+      const bool r1 = row_marked_nbr(f1 != kNoNbr && !h1, v1, vis, offsets, cols, p1);
+      const bool g0 = h0 || r0, g1 = h1 || r1;
+      const uint64_t fm0 = __ballot(g0), fm1 = __ballot(g1);
       if (lane == 0) {
-        vis_next[w0] = c0;
-        if (has1) vis_next[w1] = c1;
+        vis_next[w0] = c0 | fm0;
+        if (has1) vis_next[w1] = c1 | fm1;
       }
-      continue;
+      if (g0) {
+        level[v0] = (uint8_t)lvl;
+        parent[v0] = p0;
+      }
+      if (g1) {
+        level[v1] = (uint8_t)lvl;
+        parent[v1] = p1;
+      }
+      found_total += __popcll(fm0) + __popcll(fm1);  // identical in every lane
     }
-    const uint32_t v0 = (uint32_t)(w0 * 64 + lane), v1 = (uint32_t)(w1 * 64 + lane);
-    const bool a0 = !((c0 >> lane) & 1), a1 = !((c1 >> lane) & 1);
-    const uint32_t f0 = a0 ? first_nbr[v0] : kNoNbr;
-    const uint32_t f1 = a1 ? first_nbr[v1] : kNoNbr;
-    const bool h0 = f0 != kNoNbr && bit_set(vis, f0);
-    const bool h1 = f1 != kNoNbr && bit_set(vis, f1);
-    uint32_t p0 = f0, p1 = f1;
-    // every lane makes both calls (the long-row scan needs the whole wave)
-    const bool r0 = row_marked_nbr(f0 != kNoNbr && !h0, v0, vis, offsets, cols, p0);
-    const bool r1 = row_marked_nbr(f1 != kNoNbr && !h1, v1, vis, offsets, cols, p1);
-    const bool g0 = h0 || r0, g1 = h1 || r1;
-    const uint64_t fm0 = __ballot(g0), fm1 = __ballot(g1);
-    if (lane == 0) {
-      vis_next[w0] = c0 | fm0;
-      if (has1) vis_next[w1] = c1 | fm1;
-    }
-    if (g0) {
-      level[v0] = (uint8_t)lvl;
-      parent[v0] = p0;
-    }
-    if (g1) {
-      level[v1] = (uint8_t)lvl;
-      parent[v1] = p1;
-    }
-    found_total += __popcll(fm0) + __popcll(fm1);  // identical in every lane
   }
   block_add(lane == 0 ? found_total : 0, counters + C_FOUND);
 }
@@ -1160,7 +1177,16 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
       continue;
     }
     fuse_next = true;
-    if (forward) {
+    // A top-down level after the bottom-up phase whose frontier holds more
+    // than 1/tail_pull as many vertices as remain unvisited is found by the
+    // bottom-up kernel: an unvisited vertex with a visited neighbour is on
+    // this level (an older neighbour would have reached it before), so both
+    // kernels find the same vertices, and this one probes from the few
+    // unvisited ones instead of expanding the frontier.  The level keeps its
+    // top-down place in the schedule.
+    const bool pull_td = forward && !growing && b.tail_pull > 0 && b.tail_pull * nf > g.n_active - visited;
+    const bool td = forward && !pull_td;  // the top-down kernels run
+    if (td) {
       device_scan(FrontierDegree{b.queue[qi].p, g.offsets.p, nf}, StoreU64{b.prefix.p},
                   (size_t)nf + 1, b.scan_tmp.p, s);
       if (b.bin_edges > 0) {
@@ -1282,21 +1308,24 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
                            s, b.level.p, rows, b.words, (uint32_t)lvl, b.vis[cur].p, b.vis[cur ^ 1].p, cnt,
                            b.prefix.p, nf);
     } else {
-      hipLaunchKernelGGL(k_bu, dim3(grid_of(b.words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
+      // words per wave step: as many as keep at least 32768 wave steps
+      int span = 2;
+      while (span < 64 && b.words / (2 * span) >= 32768) span *= 2;
+      hipLaunchKernelGGL(k_bu, dim3(grid_of((b.words + span - 1) / span, kBlock / 64, 8192)), dim3(kBlock), 0, s,
                          b.vis[cur].p, b.vis[cur ^ 1].p, g.first_nbr.p, g.offsets.p, g.cols.p,
-                         b.level.p, b.parent.p, b.words, (uint32_t)lvl, cnt);
+                         b.level.p, b.parent.p, b.words, span, (uint32_t)lvl, cnt);
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(b.lev_end[lvl], s));
     ++b.pub_seq;
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, cnt, b.prefix.p, nf, forward ? 1 : 0, b.pub.d,
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, cnt, b.prefix.p, nf, td ? 1 : 0, b.pub.d,
                        b.pub_seq);
     G5_HIP(hipGetLastError());
     wait_published(b, s);
     const int64_t nq = (int64_t)b.h_counters[0];
-    const bool bulk = forward && b.h_counters[1] > kTdSmall;
+    const bool bulk = td && b.h_counters[1] > kTdSmall;
     // a small top-down level summed its winners' degrees
-    e_next = (forward && !bulk) ? (int64_t)b.h_counters[C_NEXT_E] : -1;
+    e_next = (td && !bulk) ? (int64_t)b.h_counters[C_NEXT_E] : -1;
     st.direction[lvl] = forward ? 1 : 0;
     st.nq[lvl] = nq;
     st.num_levels = lvl + 1;
@@ -1315,15 +1344,15 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     prev_nq = nq;
     // The next top-down level needs this level's vertices as a queue.  A small
     // top-down level appended them to queue[qi^1]; a bulk one left them in
-    // vis[cur^1]; a bottom-up level left them as vis[cur^1] & ~vis[cur].
-    if (forward && !bulk) {
+    // vis[cur^1]; the bottom-up kernel left them as vis[cur^1] & ~vis[cur].
+    if (td && !bulk) {
       qi ^= 1;
     } else if (next_forward) {
-      const uint64_t* before = forward ? nullptr : b.vis[cur].p;
+      const uint64_t* before = td ? nullptr : b.vis[cur].p;
       device_scan(FrontierBits{b.vis[cur ^ 1].p, before},
                   FrontierEmit{b.vis[cur ^ 1].p, before, b.queue[qi].p}, (size_t)b.words, b.scan_tmp.p, s);
     }
-    if (!forward) cur ^= 1;
+    if (!td) cur ^= 1;
     nf = nq;
   }
   G5_HIP(hipEventRecord(b.ev1, s));

@@ -376,10 +376,10 @@ class G5Context:
         L = self.bfs_stats()["num_levels"]
         return per[1:L].tolist(), tot.value
 
-    BFS_TUNING = ("bin_edges", "td_pieces", "block_shift", "td_head", "seg_max", "small_edges")
+    BFS_TUNING = ("bin_edges", "td_pieces", "block_shift", "td_head", "seg_max", "small_edges", "tail_pull")
 
     def bfs_set_tuning(self, bin_edges=None, td_pieces=None, block_shift=None, td_head=None, seg_max=None,
-                       small_edges=None):
+                       small_edges=None, tail_pull=None):
         """Single-root tuning; a knob left None keeps its value.  bin_edges:
         top-down levels with at least this many frontier edges are binned by
         target range (0 = never).  td_pieces: a growing bulk top-down level
@@ -394,10 +394,14 @@ class G5Context:
         order (tests lower it to reach that fallback on small graphs).
         small_edges: consecutive top-down levels with at most 4096 frontier
         vertices and this many frontier edges run in one launch that applies
-        the direction rule on the device (0 = off)."""
+        the direction rule on the device (0 = off).  tail_pull: a top-down
+        level after the bottom-up phase whose frontier holds more than
+        1/tail_pull as many vertices as remain unvisited is found by the
+        bottom-up kernel, keeping its top-down place in the schedule
+        (default 4; 0 = off; at most 64)."""
         t = np.zeros(len(self.BFS_TUNING), np.int64)
         self._check(self._lib.g5_bfs_get_tuning(self._h, _ptr(t)))
-        for i, v in enumerate((bin_edges, td_pieces, block_shift, td_head, seg_max, small_edges)):
+        for i, v in enumerate((bin_edges, td_pieces, block_shift, td_head, seg_max, small_edges, tail_pull)):
             if v is not None:
                 t[i] = v
         self._check(self._lib.g5_bfs_set_tuning(self._h, _ptr(t)))

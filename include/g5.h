@@ -257,7 +257,13 @@ int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
  *   small_edges: consecutive top-down levels whose frontiers have at most
  *     4096 vertices and at most small_edges edges run in one single-workgroup
  *     launch that applies the direction rule itself, with no host round trip
- *     between them (default 32768; 0 = off; at most 2^20). */
+ *     between them (default 32768; 0 = off; at most 2^20).
+ *   tail_pull: a top-down level after the bottom-up phase whose frontier
+ *     holds more than 1/tail_pull as many vertices as remain unvisited finds
+ *     its vertices with the bottom-up kernel (every unvisited vertex looks
+ *     for a visited neighbour, which can only be on the frontier); the level
+ *     keeps its top-down place in the schedule, its vertex set and its
+ *     counts (default 4; 0 = off; at most 64). */
 typedef struct g5_bfs_tuning {
   int64_t bin_edges;
   int64_t td_pieces;
@@ -265,6 +271,7 @@ typedef struct g5_bfs_tuning {
   int64_t td_head;
   int64_t seg_max;
   int64_t small_edges;
+  int64_t tail_pull;
 } g5_bfs_tuning;
 int g5_bfs_set_tuning(g5_ctx* ctx, const g5_bfs_tuning* tuning);
 int g5_bfs_get_tuning(g5_ctx* ctx, g5_bfs_tuning* tuning);

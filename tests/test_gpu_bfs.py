@@ -135,7 +135,9 @@ def test_bfs_64_roots_bit_exact_s16(ctx, s16, oracle_mod):
                                           (22, {"td_head": 4, "td_pieces": 64, "seg_max": 1000}),
                                           (22, {"td_head": 0, "block_shift": 14, "seg_max": 2000}),
                                           (20, {"small_edges": 0}), (22, {"small_edges": 1 << 20}),
-                                          (20, {"small_edges": 64}), (24, {"small_edges": 0})])
+                                          (20, {"small_edges": 64}), (24, {"small_edges": 0}),
+                                          (22, {"tail_pull": 0}), (24, {"tail_pull": 0, "small_edges": 0}),
+                                          (20, {"tail_pull": 64}), (24, {"tail_pull": 1})])
 def test_bfs_reference_fixtures_larger_scales(scale, tuning, oracle_mod):
     """An empty tuning keeps the defaults; bin_edges 1 bins every bulk
     top-down level; td_pieces 0 takes bulk rows in row order, 1 through a
@@ -148,7 +150,11 @@ def test_bfs_reference_fixtures_larger_scales(scale, tuning, oracle_mod):
     small_edges 0 runs every level through the per-level path, 2^20 fuses
     every top-down level with at most 4096 frontier vertices, 64 makes the
     fused launch stop at most frontiers (too many edges), often before its
-    first level."""
+    first level.  tail_pull 0 expands every top-down level after the
+    bottom-up phase top-down; the default 4 finds with the bottom-up kernel
+    those whose frontier holds more than a quarter as many vertices as
+    remain unvisited, 1 only those whose frontier outnumbers them, 64
+    nearly all."""
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
     say("scale", scale, "tuning", tuning)
@@ -301,6 +307,8 @@ def test_error_paths():
             c.bfs_set_tuning(small_edges=(1 << 20) + 1)  # beyond the largest fused level
         with pytest.raises(E):
             c.bfs_set_tuning(small_edges=-1)
+        with pytest.raises(E):
+            c.bfs_set_tuning(tail_pull=65)
         c.generate_kronecker(8)
         with pytest.raises(E):
             c.construct(7)  # unknown reorder mode

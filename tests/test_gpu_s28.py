@@ -132,12 +132,13 @@ def _variant_roots(s28):
 
 @pytest.mark.parametrize("tuning", [{"td_head": 0}, {"td_head": 16}, {"td_head": 4, "td_pieces": 64},
                                     {"td_head": 0, "block_shift": 18}, {"td_head": 4, "seg_max": 1000},
-                                    {"td_head": 4, "small_edges": 0}])
+                                    {"td_head": 4, "small_edges": 0}, {"td_head": 4, "tail_pull": 0}])
 def test_hub_level_knobs_keep_levels(s28, tuning):
     """The default single-root run uses td_head 4; these settings take the
     hub levels without the head, with a wider head, by positional pieces, by
     target blocks, or with the head limited to frontiers of 1000 rows, or
-    run the small top-down levels one launch each instead of fused.  The
+    run the small top-down levels one launch each instead of fused, or
+    expand the top-down levels after the bottom-up phase top-down.  The
     level digest, schedule and frontier sizes stay those of the default run,
     and the parents still validate."""
     c = s28["ctx"]
