@@ -271,6 +271,7 @@ def single_root_leg(ctx, roots, warmup):
         ctx.run_bfs(int(roots[i % len(roots)]))
     ctx.synchronize()
     wall, dev, edges, tot_bytes, bu_bytes, bu_ms, bu_launches = [], [], [], 0.0, 0.0, 0.0, 0
+    hub_bytes, hub_ms, hub_levels = 0.0, 0.0, 0
     for i, r in enumerate(roots):
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -288,6 +289,11 @@ def single_root_leg(ctx, roots, warmup):
                 bu_bytes += per[L]
                 bu_ms += st["level_ms"][L]
                 bu_launches += 1
+            elif not st["bu_kernel"][L] and hub_edges(per[L], st["nq"], L) > (1 << 20):
+                # a bulk top-down level (more than 2^20 frontier edges)
+                hub_bytes += per[L]
+                hub_ms += st["level_ms"][L]
+                hub_levels += 1
         if i % 16 == 0:
             log("single root %d: %.3f ms %.1f GTEPS %s" % (
                 r, 1e3 * (t1 - t0), e / (t1 - t0) / 1e9,
@@ -296,6 +302,7 @@ def single_root_leg(ctx, roots, warmup):
     hmean = len(wall) / sum(w / e for w, e in zip(wall, edges)) / 1e9
     hmean_dev = len(dev) / sum(w / e for w, e in zip(dev, edges)) / 1e9
     bu_gbs = bu_bytes / (bu_ms / 1e3) / 1e9 if bu_ms > 0 else 0.0
+    hub_gbs = hub_bytes / (hub_ms / 1e3) / 1e9 if hub_ms > 0 else 0.0
     return {
         "gteps_hmean_wall": round(hmean, 3),
         "gteps_hmean_device": round(hmean_dev, 3),
@@ -309,7 +316,21 @@ def single_root_leg(ctx, roots, warmup):
                  "launches": bu_launches,
                  "algorithmic_bytes_per_launch": round(bu_bytes / max(bu_launches, 1)),
                  "avg_launch_ms": round(bu_ms / max(bu_launches, 1), 4)},
+        # bulk top-down levels: head (k_td_head), row suffixes (k_td_expand_segs)
+        # or rows (k_td_expand), and k_td_finish, per level
+        "bulk_td_levels": {"achieved_gbs": round(hub_gbs, 1), "frac": round(hub_gbs / HBM_PEAK_GBS, 4),
+                           "levels": hub_levels,
+                           "algorithmic_bytes_per_level": round(hub_bytes / max(hub_levels, 1)),
+                           "avg_level_ms": round(hub_ms / max(hub_levels, 1), 4)},
     }
+
+
+def hub_edges(level_bytes, nq, L):
+    """Frontier edges E of top-down level L + 1 (list index L), from its
+    byte-model figure: 12 B per frontier vertex + 4 B per frontier edge + 5 B
+    per discovered vertex (level byte and queue entry)."""
+    nf = nq[L - 1] if L > 0 else 1
+    return (level_bytes - 5 * nq[L] - 12 * nf) / 4
 
 
 # Serving policy (continuous batching): a graph with searches in flight admits

@@ -1125,6 +1125,17 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
   // a fused launch that found the next frontier too large hands that level
   // to the per-level path
   bool fuse_next = true;
+  // A top-down level after the bottom-up phase whose frontier holds more
+  // than 1/tail_pull as many vertices as remain unvisited (and more than a
+  // fused launch takes) is found by the bottom-up kernel: an unvisited vertex
+  // with a visited neighbour is on this level (an older neighbour would have
+  // reached it before), so both kernels find the same vertices, and this one
+  // probes from the few unvisited ones instead of expanding the frontier.
+  // The level keeps its top-down place in the schedule.
+  const auto pulls = [&](bool fwd, int64_t frontier) {
+    return fwd && !growing && b.tail_pull > 0 && frontier > kFuseNf &&
+           b.tail_pull * frontier > g.n_active - visited;
+  };
   while (root_active) {
     ++lvl;
     G5_CHECK(lvl < 255, "g5_run_bfs: more than 254 levels (uint8 level array)");
@@ -1177,14 +1188,7 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
       continue;
     }
     fuse_next = true;
-    // A top-down level after the bottom-up phase whose frontier holds more
-    // than 1/tail_pull as many vertices as remain unvisited is found by the
-    // bottom-up kernel: an unvisited vertex with a visited neighbour is on
-    // this level (an older neighbour would have reached it before), so both
-    // kernels find the same vertices, and this one probes from the few
-    // unvisited ones instead of expanding the frontier.  The level keeps its
-    // top-down place in the schedule.
-    const bool pull_td = forward && !growing && b.tail_pull > 0 && b.tail_pull * nf > g.n_active - visited;
+    const bool pull_td = pulls(forward, nf);
     const bool td = forward && !pull_td;  // the top-down kernels run
     if (td) {
       device_scan(FrontierDegree{b.queue[qi].p, g.offsets.p, nf}, StoreU64{b.prefix.p},
@@ -1327,6 +1331,7 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
     // a small top-down level summed its winners' degrees
     e_next = (td && !bulk) ? (int64_t)b.h_counters[C_NEXT_E] : -1;
     st.direction[lvl] = forward ? 1 : 0;
+    st.bu_kernel[lvl] = td ? 0 : 1;
     st.nq[lvl] = nq;
     st.num_levels = lvl + 1;
     visited += nq;
@@ -1342,12 +1347,13 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
       }
     }
     prev_nq = nq;
-    // The next top-down level needs this level's vertices as a queue.  A small
-    // top-down level appended them to queue[qi^1]; a bulk one left them in
-    // vis[cur^1]; the bottom-up kernel left them as vis[cur^1] & ~vis[cur].
+    // The next top-down level needs this level's vertices as a queue, unless
+    // the bottom-up kernel takes it.  A small top-down level appended them to
+    // queue[qi^1]; a bulk one left them in vis[cur^1]; the bottom-up kernel
+    // left them as vis[cur^1] & ~vis[cur].
     if (td && !bulk) {
       qi ^= 1;
-    } else if (next_forward) {
+    } else if (next_forward && !pulls(true, nq)) {
       const uint64_t* before = td ? nullptr : b.vis[cur].p;
       device_scan(FrontierBits{b.vis[cur ^ 1].p, before},
                   FrontierEmit{b.vis[cur ^ 1].p, before, b.queue[qi].p}, (size_t)b.words, b.scan_tmp.p, s);

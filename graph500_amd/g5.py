@@ -82,6 +82,7 @@ class BfsStats(ctypes.Structure):
         ("level_ms", ctypes.c_double * MAX_LEVELS),
         ("device_ms", ctypes.c_double),
         ("reached", ctypes.c_int64),
+        ("bu_kernel", ctypes.c_int32 * MAX_LEVELS),
     ]
 
 
@@ -366,6 +367,7 @@ class G5Context:
             "level_ms": [st.level_ms[i] for i in range(1, L)],
             "device_ms": st.device_ms,
             "reached": st.reached,
+            "bu_kernel": [st.bu_kernel[i] for i in range(1, L)],
         }
 
     def model_bytes(self):
@@ -395,8 +397,8 @@ class G5Context:
         small_edges: consecutive top-down levels with at most 4096 frontier
         vertices and this many frontier edges run in one launch that applies
         the direction rule on the device (0 = off).  tail_pull: a top-down
-        level after the bottom-up phase whose frontier holds more than
-        1/tail_pull as many vertices as remain unvisited is found by the
+        level after the bottom-up phase whose frontier holds more than 4096
+        and 1/tail_pull as many vertices as remain unvisited is found by the
         bottom-up kernel, keeping its top-down place in the schedule
         (default 4; 0 = off; at most 64)."""
         t = np.zeros(len(self.BFS_TUNING), np.int64)

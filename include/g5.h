@@ -75,6 +75,9 @@ typedef struct g5_bfs_stats {
   double level_ms[G5_MAX_LEVELS];   /* device time of each level's kernels (HIP events) */
   double device_ms;                 /* device time of the whole run (HIP events) */
   int64_t reached;                  /* vertices with a level, root included */
+  int32_t bu_kernel[G5_MAX_LEVELS]; /* 1 = the level's vertices were found by the bottom-up kernel
+                                       (every bottom-up level, and the top-down levels tail_pull
+                                       hands to it; see g5_bfs_tuning) */
 } g5_bfs_stats;
 
 /* Trace of the last g5_run_bfs_batch.  Index 0 is unused; levels start at 1. */
@@ -259,8 +262,8 @@ int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
  *     launch that applies the direction rule itself, with no host round trip
  *     between them (default 32768; 0 = off; at most 2^20).
  *   tail_pull: a top-down level after the bottom-up phase whose frontier
- *     holds more than 1/tail_pull as many vertices as remain unvisited finds
- *     its vertices with the bottom-up kernel (every unvisited vertex looks
+ *     holds more than 4096 vertices and more than 1/tail_pull as many as
+ *     remain unvisited finds its vertices with the bottom-up kernel (every unvisited vertex looks
  *     for a visited neighbour, which can only be on the frontier); the level
  *     keeps its top-down place in the schedule, its vertex set and its
  *     counts (default 4; 0 = off; at most 64). */

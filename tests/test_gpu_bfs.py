@@ -179,6 +179,14 @@ def test_bfs_reference_fixtures_larger_scales(scale, tuning, oracle_mod):
             st = c.bfs_stats()
             assert "".join("T" if d else "B" for d in st["direction"]) == pr["directions"]
             assert st["nq"] == pr["nq"]
+            # the bottom-up kernel runs every bottom-up level, and top-down
+            # ones only after the bottom-up phase when tail_pull hands them over
+            kern = st["bu_kernel"]
+            first_bu = st["direction"].index(0) if 0 in st["direction"] else len(kern)
+            assert all(k == 1 for k, d in zip(kern, st["direction"]) if d == 0)
+            assert not any(kern[:first_bu])
+            if tuning.get("tail_pull") == 0:
+                assert kern == [1 - d for d in st["direction"]]
             assert c.teps_edges() == pr["nedge"]
             ok, errs = c.validate()  # every root's parents, on the GPU
             assert ok, (pr["root"], errs)
