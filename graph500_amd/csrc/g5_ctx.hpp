@@ -172,6 +172,12 @@ struct MsState {
   DevBuf<uint64_t> seen;     // n_rows
   DevBuf<uint64_t> nz[2];    // frontier-word-not-zero bitmaps, alternating by level
   std::vector<std::unique_ptr<DevBuf<uint64_t>>> fr;  // frontier words per level
+  // per level: one bit per vertex whose frontier word the level's last run
+  // left non-zero (a copy of that level's nonzero bitmap); valid where
+  // fr_exact is set, so the next batch clears only those words
+  std::vector<std::unique_ptr<DevBuf<uint64_t>>> fr_dirty;
+  std::vector<char> fr_exact;
+  std::vector<int64_t> fr_count;  // per level: vertices the last run marked
   DevBuf<const uint64_t*> frp;  // device copy of the fr pointers
   DevBuf<uint32_t> queue;
   DevBuf<uint32_t> touch;    // vertices a sparse push level touched
@@ -191,6 +197,9 @@ struct MsState {
     seen.release();
     for (auto& b : nz) b.release();
     fr.clear();
+    fr_dirty.clear();
+    fr_exact.clear();
+    fr_count.clear();
     frp.release();
     queue.release();
     touch.release();

@@ -549,7 +549,11 @@ void ms_prepare(g5_ctx* ctx) {
   for (int l = 0; l < kMsPreLevels; ++l) {
     while ((int)m.fr.size() <= l) m.fr.emplace_back(new DevBuf<uint64_t>());
     m.fr[l]->alloc(rows);
+    while ((int)m.fr_dirty.size() <= l) m.fr_dirty.emplace_back(new DevBuf<uint64_t>());
+    m.fr_dirty[l]->alloc(words);
   }
+  m.fr_exact.assign(m.fr.size(), 0);
+  m.fr_count.assign(m.fr.size(), 0);
   if (m.compact_below > 0) ms_alloc_compact(ctx);
   G5_HIP(hipHostMalloc((void**)&m.h_counters, (M_NUM + 1 + 128) * sizeof(uint64_t), hipHostMallocDefault));
   G5_HIP(hipEventCreate(&m.ev0));
@@ -566,8 +570,48 @@ void ms_prepare(g5_ctx* ctx) {
 // Frontier words of level l, allocated on first use and kept for reuse.
 static uint64_t* ms_level(MsState& m, int l, int64_t rows) {
   while ((int)m.fr.size() <= l) m.fr.emplace_back(new DevBuf<uint64_t>());
+  const uint64_t* before = m.fr[l]->p;
   m.fr[l]->alloc(rows);
+  if ((int)m.fr_exact.size() <= l) {
+    m.fr_exact.resize(l + 1, 0);
+    m.fr_count.resize(l + 1, 0);
+  }
+  if (m.fr[l]->p != before) m.fr_exact[l] = 0;  // new memory: contents unknown
   return m.fr[l]->p;
+}
+
+// Zeroes level l's frontier words before a level that writes them sparsely
+// (a push's atomic ORs, a listed fold or pull): only the words its last run
+// left non-zero when those are known, else the whole array.
+__global__ void __launch_bounds__(kBlock) k_ms_clear_marked(const uint64_t* __restrict__ marked, int64_t words,
+                                                            int64_t rows, uint64_t* __restrict__ fr) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t wd = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
+    const uint64_t b = marked[wd];
+    const int64_t v = wd * 64 + lane;
+    if (((b >> lane) & 1) && v < rows) fr[v] = 0;
+  }
+}
+
+static void ms_clear_frontier(MsState& m, int l, int64_t rows, hipStream_t s, bool marked) {
+  const int64_t words = (rows + 63) / 64;
+  // a dense mark set is cleared faster by a plain fill
+  if (marked && m.fr_count[l] * 16 < rows) {
+    hipLaunchKernelGGL(k_ms_clear_marked, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
+                       m.fr_dirty[l]->p, words, rows, m.fr[l]->p);
+    G5_HIP(hipGetLastError());
+  } else {
+    G5_HIP(hipMemsetAsync(m.fr[l]->p, 0, rows * 8, s));
+  }
+}
+
+// Level l's nonzero bitmap (one bit per vertex whose frontier word is not
+// zero), kept per level: it marks the words the level's next run clears.
+static uint64_t* ms_marks(MsState& m, int l, int64_t words) {
+  while ((int)m.fr_dirty.size() <= l) m.fr_dirty.emplace_back(new DevBuf<uint64_t>());
+  m.fr_dirty[l]->alloc(words);
+  return m.fr_dirty[l]->p;
 }
 
 // Host-driven levels from level l0 on (counters of level l0 - 1 in live /
@@ -602,8 +646,14 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     G5_CHECK(l < G5_MAX_LEVELS - 1, "g5_run_bfs_batch: more than 254 levels (uint8 level arrays)");
     uint64_t* next = ms_level(m, l, rows);
     const uint64_t* front = m.fr[l - 1]->p;  // after ms_level: the vector may have grown
-    const uint64_t* nz_front = m.nz[(l - 1) & 1].p;
-    uint64_t* nz_next = m.nz[l & 1].p;
+    // whether the marks describe this level's frontier words; unset until
+    // the level has been enqueued whole (an error midway leaves them unset)
+    const bool marked = m.fr_exact[l] != 0;
+    m.fr_exact[l] = 0;
+    // each level's nonzero bitmap stays with its frontier words (fr_dirty)
+    // for the next batch's clear of them
+    uint64_t* nz_next = ms_marks(m, l, words);
+    const uint64_t* nz_front = m.fr_dirty[l - 1]->p;
     bool push = (double)fdeg < push_limit && total_pairs - reached >= unseen_frac * total_pairs;
     bool sweep = !push && m.sweep_chunks && reached < sweep_limit;
     uint64_t td = 0, bu = 0;
@@ -638,7 +688,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       m.nz_sel.alloc(words);
       hipLaunchKernelGGL(k_ms_select, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s, front, rows, td,
                          m.nz_sel.p);
-      G5_HIP(hipMemsetAsync(next, 0, rows * 8, s));
+      ms_clear_frontier(m, l, rows, s, marked);
       if (m.mixed_push == 1) {
         // the pushing roots' frontier vertices as a queue (the count stays on
         // the device), their rows as one edge-balanced virtual edge list; the
@@ -699,7 +749,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       if (nchunks)
         hipLaunchKernelGGL(k_chunk_rows<kEpb>, dim3(grid_of((int64_t)(2 * nchunks), kBlock, 65536)),
                            dim3(kBlock), 0, s, m.prefix.p, nf, nchunks, m.chunk_rows.p);
-      G5_HIP(hipMemsetAsync(next, 0, rows * 8, s));
+      ms_clear_frontier(m, l, rows, s, marked);
       // a push whose frontier edges are few next to the rows lists what it
       // touches, and only those vertices are folded
       const uint64_t touch_cap = std::min<uint64_t>(fdeg, (uint64_t)rows);
@@ -770,7 +820,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
         m.ulist_count.alloc(1);
         device_scan(BitsCountN{m.miss.p, words}, BitsEmitN{m.miss.p, words, m.ulist.p, m.ulist_count.p},
                     (size_t)words + 1, m.scan_tmp.p, s);
-        G5_HIP(hipMemsetAsync(next, 0, rows * 8, s));
+        ms_clear_frontier(m, l, rows, s, marked);
         G5_HIP(hipMemsetAsync(nz_next, 0, words * 8, s));
         G5_HIP(hipMemsetAsync(m.miss.p, 0, words * 8, s));
         hipLaunchKernelGGL(k_ms_pull_list, dim3(grid_of((int64_t)ub, kBlock, m.pull_grid)), dim3(kBlock), 0, s,
@@ -787,6 +837,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       }
     }
     G5_HIP(hipGetLastError());
+    m.fr_exact[l] = 1;  // every level form leaves next non-zero exactly at the bits of nz_next
     G5_HIP(hipEventRecord(m.lev_end[l], s));
     G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, (lane_stats ? M_NUM + 1 + 128 : M_NUM) * 8,
                           hipMemcpyDeviceToHost, s));
@@ -806,6 +857,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     st.num_levels = l + 1;
     live = m.h_counters[M_LIVE];
     nz = m.h_counters[M_NZ];
+    m.fr_count[l] = (int64_t)nz;
     fdeg = m.h_counters[M_FDEG];
   }
 }
@@ -953,6 +1005,9 @@ static void ms_run_persistent(g5_ctx* ctx, const int64_t* roots, int nroots) {
   MsState& m = ctx->ms;
   hipStream_t s = ctx->stream;
   ms_persistent_setup(ctx);
+  // the persistent kernel clears and writes the frontier words of its levels
+  // itself: the per-level marks no longer describe them
+  std::fill(m.fr_exact.begin(), m.fr_exact.end(), 0);
   const int K = m.persist_levels;
   int64_t* h_roots = (int64_t*)(m.h_persist + (size_t)G5_MAX_LEVELS * (M_NUM + 2));
   for (int r = 0; r < 64; ++r) h_roots[r] = r < nroots ? roots[r] : -1;
@@ -989,7 +1044,10 @@ static void ms_run_persistent(g5_ctx* ctx, const int64_t* roots, int nroots) {
   st.persistent_levels = done;
   const uint64_t* last = h_cnt + (size_t)done * M_NUM;
   if (last[M_LIVE]) {
-    // out of preallocated levels: continue level by level from the host
+    // out of preallocated levels: continue level by level from the host,
+    // which reads the last level's nonzero bitmap from its per-level slot
+    const int64_t words = (g.n_rows + 63) / 64;
+    G5_HIP(hipMemcpyAsync(ms_marks(m, done, words), m.nz[done & 1].p, words * 8, hipMemcpyDeviceToDevice, s));
     ms_host_levels(ctx, done + 1, last[M_LIVE], last[M_NZ], last[M_FDEG], reached, roots_live);
   }
   (void)g;
@@ -1019,14 +1077,20 @@ void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots) {
   } else {
     uint64_t* fr0 = ms_level(m, 0, rows);
     const int64_t words = (rows + 63) / 64;
+    uint64_t* nz0 = ms_marks(m, 0, words);
     G5_HIP(hipMemcpyAsync(m.roots.p, roots, nroots * sizeof(int64_t), hipMemcpyHostToDevice, s));
     G5_HIP(hipEventRecord(m.ev0, s));
-    hipLaunchKernelGGL(k_ms_clear, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, m.seen.p, fr0, rows);
-    G5_HIP(hipMemsetAsync(m.nz[0].p, 0, words * 8, s));
+    G5_HIP(hipMemsetAsync(m.seen.p, 0, rows * 8, s));
+    const bool marked = m.fr_exact[0] != 0;
+    m.fr_exact[0] = 0;
+    ms_clear_frontier(m, 0, rows, s, marked);  // the previous batch's roots, or everything
+    G5_HIP(hipMemsetAsync(nz0, 0, words * 8, s));
     hipLaunchKernelGGL(k_ms_roots, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, nroots, rows, g.offsets.p,
                        m.seen.p, fr0, cnt);
-    hipLaunchKernelGGL(k_ms_roots_nz, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, nroots, rows, m.nz[0].p);
+    hipLaunchKernelGGL(k_ms_roots_nz, dim3(1), dim3(64), 0, s, g.old2new.p, m.roots.p, nroots, rows, nz0);
     G5_HIP(hipGetLastError());
+    m.fr_exact[0] = 1;
+    m.fr_count[0] = nroots;
     G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, M_NUM * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     const uint64_t live = m.h_counters[M_LIVE];
