@@ -337,8 +337,11 @@ def hub_edges(level_bytes, nq, L):
 # waiting queries into its free lanes at a level boundary while every search
 # in flight still pushes, once min(min_admit, free answer arrays) queries can
 # start together, or once the oldest has waited max_wait_s; each search pulls
-# while its frontier's edges exceed 1/alpha of its unreached edges.
-SERVE_POLICY = {"answer_buffers": 128, "min_admit": 64, "max_wait_s": 0.3, "lane_rule": 3, "alpha": 64.0}
+# while its frontier's edges exceed 1/alpha of its unreached edges.  A graph
+# with no search in flight and one waiting query answers it with the
+# single-root kernels (solo).
+SERVE_POLICY = {"answer_buffers": 128, "min_admit": 64, "max_wait_s": 0.3, "lane_rule": 3, "alpha": 64.0,
+                "solo": 1}
 
 
 def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):
@@ -376,7 +379,9 @@ def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):
             q["arrival_s"] = np.cumsum(rng.exponential(1.0 / rate, size=nq))
             check = rng.choice(nq, size=min(4, nq), replace=False)
             q["keep"][check] = 1
+            solo0 = sum(srv.graph_stats(g)["solo"] for g in range(len(scales)))
             res, rep = srv.run(q, timeout_s=300)
+            solo = sum(srv.graph_stats(g)["solo"] for g in range(len(scales))) - solo0
             lat = (res["done_s"] - q["arrival_s"]) * 1e3
             ok = True
             for i in check:
@@ -396,7 +401,8 @@ def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):
                    "served_qps": round(nq / rep["wall_s"], 1),
                    "gteps": round(rep["edges"] / rep["wall_s"] / 1e9, 2),
                    "max_lanes": rep["max_active_lanes"], "admitted_busy": rep["admitted_busy"],
-                   "admit_events": rep["admit_events"], "sample_bit_exact": ok}
+                   "admit_events": rep["admit_events"], "single_root_answers": int(solo),
+                   "sample_bit_exact": ok}
             log("serve %s" % run)
             runs.append(run)
     for c in ctxs:

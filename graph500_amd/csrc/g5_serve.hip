@@ -24,6 +24,7 @@
 #include "g5_ms_device.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <deque>
 #include <string>
@@ -320,8 +321,19 @@ struct Engine {
   std::deque<int64_t> waiting;
   int64_t levels_run = 0;
   int64_t admit_events = 0;
+  // a lone query answered by the single-root kernels on a host thread of its
+  // own (config solo): the thread owns the graph context and its stream until
+  // it sets solo_state to 2
+  std::thread solo_thread;
+  std::atomic<int> solo_state{0};  // 0 none, 1 running, 2 finished
+  int64_t solo_req = -1;
+  int64_t solo_edges = 0;
+  int solo_levels = 0;
+  double solo_done = 0;  // seconds after the run's start
+  std::string solo_err;
 
   ~Engine() {
+    if (solo_thread.joinable()) solo_thread.join();
     if (h) (void)hipHostFree(h);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_end) (void)hipEventDestroy(ev_end);
@@ -339,6 +351,7 @@ struct g5_server {
   g5_server_config cfg{};
   int device = 0;
   int64_t done_count = 0;
+  double t0 = 0;  // start of the current run (now_s clock)
   g5::DevBuf<uint8_t> lv_out;
 };
 
@@ -626,8 +639,77 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
   ++e.levels_run;
 }
 
+// A lone query on an idle graph (config solo): g5_run_bfs's kernels, driven
+// by a host thread so that the loop goes on with the other graphs.  The
+// single-root level array (n_rows bytes in row order, the root 0, unreached
+// 0xFF) is the answer array's format, and its traversed-edge count the
+// engine's.
+static void sv_start_solo(g5_server* srv, Engine& e, double t) {
+  const int64_t id = e.waiting.front();
+  e.waiting.pop_front();
+  Request& q = srv->req[id];
+  q.admit = t;
+  q.status = SV_RUNNING;
+  uint8_t* ans = nullptr;
+  if (q.keep) {
+    q.buf = e.free_bufs.back();
+    e.free_bufs.pop_back();
+    ans = e.bufs[q.buf]->p;
+  }
+  if (e.solo_thread.joinable()) e.solo_thread.join();
+  e.solo_req = id;
+  e.solo_err.clear();
+  e.solo_state.store(1, std::memory_order_release);
+  ++e.admit_events;
+  ++e.st.admitted_idle;
+  Engine* ep = &e;
+  const int dev = srv->device;
+  const double t0 = srv->t0;
+  const int64_t root = q.root;
+  e.solo_thread = std::thread([ep, dev, t0, root, ans] {
+    try {
+      G5_HIP(hipSetDevice(dev));
+      g5_ctx* ctx = ep->ctx;
+      bfs_run(ctx, root);
+      if (ans) {
+        G5_HIP(hipMemcpyAsync(ans, ctx->bfs.level.p, (size_t)ep->rows, hipMemcpyDeviceToDevice, ctx->stream));
+        G5_HIP(hipStreamSynchronize(ctx->stream));
+      }
+      ep->solo_edges = bfs_teps_edges(ctx);
+      ep->solo_levels = ctx->bfs.stats.num_levels - 1;
+      ep->solo_done = now_s() - t0;
+    } catch (const Error& x) {
+      ep->solo_err = x.msg.empty() ? "serving: single-root run failed" : x.msg;
+    } catch (...) {
+      ep->solo_err = "serving: single-root run failed";
+    }
+    ep->solo_state.store(2, std::memory_order_release);
+  });
+}
+
+static void sv_finish_solo(g5_server* srv, Engine& e) {
+  e.solo_thread.join();
+  e.solo_state.store(0, std::memory_order_relaxed);
+  G5_CHECK(e.solo_err.empty(), e.solo_err);
+  Request& q = srv->req[e.solo_req];
+  q.status = SV_DONE;
+  q.done = e.solo_done;
+  q.edges = e.solo_edges;
+  q.levels = e.solo_levels;
+  ++srv->done_count;
+  ++e.st.completed;
+  ++e.st.solo;
+  e.solo_req = -1;
+}
+
 // One non-blocking step of an engine; returns true if it made progress.
 static bool sv_step(g5_server* srv, Engine& e, double t) {
+  const int solo = e.solo_state.load(std::memory_order_acquire);
+  if (solo == 1) return false;  // the single-root run owns the graph's stream
+  if (solo == 2) {
+    sv_finish_solo(srv, e);
+    return true;
+  }
   if (e.busy) {
     if (!e.pub.ready(0, kSvPub, e.pub_seq)) {
       // not published yet; a stream that has finished without publishing
@@ -649,6 +731,10 @@ static bool sv_step(g5_server* srv, Engine& e, double t) {
                        ~e.cooling);
     G5_HIP(hipGetLastError());
     e.cooling = 0;
+  }
+  if (srv->cfg.solo && !e.active && e.waiting.size() == 1 && (!srv->req[e.waiting.front()].keep || !e.free_bufs.empty())) {
+    sv_start_solo(srv, e, t);
+    return true;
   }
   sv_admit(srv, e, t);
   if (e.active) {
@@ -678,6 +764,7 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
   int64_t next_arrival = 0, done = 0;
   const int64_t done0 = srv->done_count;
   const double t0 = now_s();
+  srv->t0 = t0;
   int64_t levels0 = 0, busy0 = 0, events0 = 0;
   for (auto& e : srv->eng) {
     levels0 += e->levels_run;
@@ -702,7 +789,7 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
     done = srv->done_count - done0;
     if (!progress) {
       bool any_busy = false;
-      for (auto& e : srv->eng) any_busy |= e->busy;
+      for (auto& e : srv->eng) any_busy |= e->busy || e->solo_state.load(std::memory_order_relaxed) != 0;
       if (!any_busy && next_arrival < nq) {
         const double wait = srv->req[base + next_arrival].arrival - (now_s() - t0);
         if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
@@ -783,7 +870,7 @@ int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_conf
   if (cfg->answer_buffers < 1 || cfg->answer_buffers > 4096 || !(cfg->alpha > 0) ||
       (cfg->pull_filter != 0 && cfg->pull_filter != 1) || cfg->min_admit < 1 || cfg->min_admit > 64 ||
       cfg->lane_rule < 0 || cfg->lane_rule > 3 ||
-      !(cfg->max_wait_s >= 0))
+      !(cfg->max_wait_s >= 0) || cfg->solo < 0 || cfg->solo > 1 || cfg->reserved != 0)
     return G5_ERR_ARG;
   for (int i = 0; i < num_graphs; ++i) {
     if (!graphs[i] || graphs[i]->device != graphs[0]->device) return G5_ERR_ARG;
@@ -798,6 +885,8 @@ int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_conf
     for (int i = 0; i < num_graphs; ++i) {
       srv->eng.emplace_back(new Engine());
       sv_setup_engine(srv, *srv->eng.back(), graphs[i]);
+      // the single-root path needs the context's BFS state
+      if (cfg->solo && !graphs[i]->bfs.prepared) bfs_prepare(graphs[i]);
     }
   });
   if (rc != G5_OK) {
@@ -815,6 +904,7 @@ void g5_server_destroy(g5_server* srv) {
   if (!srv) return;
   (void)hipSetDevice(srv->device);
   for (auto& e : srv->eng) {
+    if (e->solo_thread.joinable()) e->solo_thread.join();  // a single-root run still going
     (void)hipStreamSynchronize(e->s);
     --e->ctx->servers;
   }

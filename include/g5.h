@@ -345,8 +345,10 @@ int g5_batch_end(g5_ctx* ctx);
  * their lane and waiting queries take free lanes, while the other lanes go on
  * at whatever level they have reached.  A query holds one answer array
  * (num_rows bytes of levels) from a pool of answer_buffers per graph; a query
- * waits while its graph has no free lane or no free answer array.  The
- * graph contexts must not be used for anything else while a server runs. */
+ * waits while its graph has no free lane or no free answer array.  With
+ * config solo, a query that finds its graph idle and alone is answered by
+ * the single-root kernels instead (see g5_server_config.solo).  The graph
+ * contexts must not be used for anything else while a server runs. */
 typedef struct g5_server g5_server;
 
 typedef struct g5_server_config {
@@ -367,6 +369,12 @@ typedef struct g5_server_config {
                               lane_rule 1) */
   double max_wait_s;       /* ... or once its oldest waiting query has waited this long.  An idle
                               graph admits at once. */
+  int32_t solo;            /* 1: a graph with no search in flight and exactly one waiting query
+                              answers it with the single-root kernels of g5_run_bfs (a host thread
+                              of the server drives them, so the other graphs go on); queries that
+                              arrive meanwhile wait and are batched after it.  0: every query takes
+                              a lane. */
+  int32_t reserved;        /* must be 0 */
 } g5_server_config;
 
 typedef struct g5_query {
@@ -407,6 +415,7 @@ typedef struct g5_server_graph_stats {
   int64_t completed;      /* queries answered */
   int64_t admitted_busy;  /* queries admitted while other searches of this graph were in flight */
   int64_t admitted_idle;  /* queries admitted into an idle graph */
+  int64_t solo;           /* queries answered by the single-root kernels (g5_server_config.solo) */
 } g5_server_graph_stats;
 
 int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_config* config, g5_server** out);

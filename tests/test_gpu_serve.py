@@ -195,6 +195,30 @@ def test_cohort_admission_keeps_answers(graphs):
         _check_kept(srv, res, specs, ctxs, ocs, range(0, 200, 10))
 
 
+@pytest.mark.parametrize("solo", [1, 0])
+def test_lone_queries_single_root_path(graphs, solo):
+    """Queries spaced 15 ms apart find their graph idle: with solo 1 each is
+    answered by the single-root kernels (solo count = queries), with solo 0
+    each takes a lane; the answers are the oracle's either way."""
+    import graph500_amd
+    ctxs, ocs = graphs
+    rng = np.random.default_rng(31)
+    specs = []
+    for i in range(12):
+        g = i % 3
+        specs.append((g, _roots_with_edges(ctxs[g], 1, rng)[0], 0.015 * i, 1 if i % 2 == 0 else 0))
+    with graph500_amd.G5Server(ctxs, solo=solo) as srv:
+        say("serve lone queries, solo", solo)
+        res, rep = srv.run(_queries(specs), timeout_s=60)
+        assert (res["status"] == 2).all()
+        n_solo = sum(srv.graph_stats(g)["solo"] for g in range(3))
+        assert n_solo == (12 if solo else 0), n_solo
+        assert rep["admitted_busy"] == 0
+        for i, (g, r, _, _) in enumerate(specs):
+            assert res["edges"][i] == _oracle(ocs[g], r)[1], (i, g, r)
+        _check_kept(srv, res, specs, ctxs, ocs, range(0, 12, 2))
+
+
 def test_lane_and_memory_pressure(graphs):
     """200 simultaneous queries on one graph (more than its 64 lanes) with only
     5 answer arrays: queries wait for lanes and memory, all complete."""
@@ -338,6 +362,8 @@ def test_serve_errors(graphs):
         c.construct(0)
         with pytest.raises(graph500_amd.G5Error):
             graph500_amd.G5Server([c])  # reorder mode 0 graphs are not served
+    with pytest.raises(graph500_amd.G5Error):
+        graph500_amd.G5Server(ctxs[:1], solo=2)  # solo is 0 or 1
 
 
 def test_served_context_refuses_rebuild_and_destroy():
