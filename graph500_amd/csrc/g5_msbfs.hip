@@ -583,14 +583,15 @@ static uint64_t* ms_level(MsState& m, int l, int64_t rows) {
 // Zeroes level l's frontier words before a level that writes them sparsely
 // (a push's atomic ORs, a listed fold or pull): only the words its last run
 // left non-zero when those are known, else the whole array.
+// One bitmap word per lane (the marks are read coalesced); a lane zeroes the
+// frontier words of its word's set bits (used for sparse marks only).
 __global__ void __launch_bounds__(kBlock) k_ms_clear_marked(const uint64_t* __restrict__ marked, int64_t words,
                                                             int64_t rows, uint64_t* __restrict__ fr) {
-  const int lane = threadIdx.x & 63;
-  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
-  for (int64_t wd = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); wd < words; wd += nwaves) {
-    const uint64_t b = marked[wd];
-    const int64_t v = wd * 64 + lane;
-    if (((b >> lane) & 1) && v < rows) fr[v] = 0;
+  for (int64_t wd = (int64_t)blockIdx.x * kBlock + threadIdx.x; wd < words; wd += (int64_t)gridDim.x * kBlock) {
+    for (uint64_t b = marked[wd]; b; b &= b - 1) {
+      const int64_t v = wd * 64 + __builtin_ctzll(b);
+      if (v < rows) fr[v] = 0;
+    }
   }
 }
 
@@ -598,7 +599,7 @@ static void ms_clear_frontier(MsState& m, int l, int64_t rows, hipStream_t s, bo
   const int64_t words = (rows + 63) / 64;
   // a dense mark set is cleared faster by a plain fill
   if (marked && m.fr_count[l] * 16 < rows) {
-    hipLaunchKernelGGL(k_ms_clear_marked, dim3(grid_of(words, kBlock / 64, 8192)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_ms_clear_marked, dim3(grid_of(words, kBlock, 8192)), dim3(kBlock), 0, s,
                        m.fr_dirty[l]->p, words, rows, m.fr[l]->p);
     G5_HIP(hipGetLastError());
   } else {
