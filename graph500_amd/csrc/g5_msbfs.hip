@@ -37,40 +37,44 @@ __global__ void __launch_bounds__(kBlock) k_ms_clear(uint64_t* __restrict__ a, u
   }
 }
 
-// Marks every active root in seen and fr0 and counts the level-0 union.
+// Marks every active root in seen and fr0 and counts the level-0 union: one
+// wave, lane r takes root r; of the roots sharing a vertex, the lane whose
+// atomic OR finds its word still zero counts the vertex.
 __global__ void k_ms_roots(const uint32_t* __restrict__ old2new, const int64_t* __restrict__ roots, int nroots,
                            int64_t rows, const uint64_t* __restrict__ offsets, uint64_t* __restrict__ seen,
                            uint64_t* __restrict__ fr0, unsigned long long* __restrict__ c) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint64_t live = 0, nz = 0, fdeg = 0;
-  for (int r = 0; r < nroots; ++r) {
-    if (roots[r] < 0) continue;  // unused lane
+  if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+  const int r = threadIdx.x;
+  bool act = false, first = false;
+  uint64_t deg = 0;
+  if (r < nroots && roots[r] >= 0) {  // unused lanes hold -1
     const uint32_t v = old2new[roots[r]];
-    if ((int64_t)v >= rows) continue;  // a root without edges discovers nothing
-    const uint64_t bit = UINT64_C(1) << r;
-    if (!fr0[v]) {
-      ++nz;
-      fdeg += offsets[v + 1] - offsets[v];
+    act = (int64_t)v < rows;  // a root without edges discovers nothing
+    if (act) {
+      const unsigned long long bit = 1ull << r;
+      atomicOr((unsigned long long*)(seen + v), bit);
+      first = atomicOr((unsigned long long*)(fr0 + v), bit) == 0;
+      if (first) deg = offsets[v + 1] - offsets[v];
     }
-    seen[v] |= bit;
-    fr0[v] |= bit;
-    live |= bit;
   }
-  c[M_LIVE] = live;
-  c[M_PAIRS] = 0;
-  c[M_NZ] = nz;
-  c[M_FDEG] = fdeg;
+  const uint64_t live = __ballot(act);
+  const uint64_t nz = (uint64_t)__popcll(__ballot(first));
+  const uint64_t fdeg = wave_sum(deg);
+  if (r == 0) {
+    c[M_LIVE] = live;
+    c[M_PAIRS] = 0;
+    c[M_NZ] = nz;
+    c[M_FDEG] = fdeg;
+  }
 }
 
-// Level-0 nonzero bitmap: the root vertices.
+// Level-0 nonzero bitmap: the root vertices (lane r, root r).
 __global__ void k_ms_roots_nz(const uint32_t* __restrict__ old2new, const int64_t* __restrict__ roots, int nroots,
                               int64_t rows, uint64_t* __restrict__ nz0) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (int r = 0; r < nroots; ++r) {
-    if (roots[r] < 0) continue;
-    const uint32_t v = old2new[roots[r]];
-    if ((int64_t)v < rows) nz0[v >> 6] |= UINT64_C(1) << (v & 63);
-  }
+  const int r = threadIdx.x;
+  if (blockIdx.x != 0 || r >= nroots || roots[r] < 0) return;
+  const uint32_t v = old2new[roots[r]];
+  if ((int64_t)v < rows) atomicOr((unsigned long long*)(nz0 + (v >> 6)), 1ull << (v & 63));
 }
 
 __global__ void __launch_bounds__(kBlock) k_ms_push(const uint32_t* __restrict__ queue, int64_t nf,
