@@ -190,6 +190,8 @@ struct MsState {
   DevBuf<uint64_t> counters;
   DevBuf<int64_t> roots;
   uint64_t* h_counters = nullptr;
+  PublishBlock pub;             // the counters as each level publishes them (g5_msbfs.hip)
+  uint64_t pub_seq = 0;
   g5_batch_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> lev_begin, lev_end;
@@ -230,6 +232,7 @@ struct MsState {
     h_persist = nullptr;
     if (h_counters) (void)hipHostFree(h_counters);
     h_counters = nullptr;
+    pub.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     ev0 = ev1 = nullptr;

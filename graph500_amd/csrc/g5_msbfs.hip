@@ -29,6 +29,8 @@
 
 namespace g5 {
 
+constexpr int kMsCounters = M_NUM + 1 + 128;  // level counters, a list count, per-lane counts and degree sums
+
 __global__ void __launch_bounds__(kBlock) k_ms_clear(uint64_t* __restrict__ a, uint64_t* __restrict__ b,
                                                      int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
@@ -524,7 +526,7 @@ void ms_prepare(g5_ctx* ctx) {
   m.prefix.alloc(rows + 1);
   m.chunk_rows.alloc(2 * ((uint64_t)g.nnz / kEpb + 2));
   m.scan_tmp.alloc(scan_tmp_elems(rows + 1));
-  m.counters.alloc(M_NUM + 1 + 128);  // level counters, a list count, per-lane counts and degree sums
+  m.counters.alloc(kMsCounters);
   m.roots.alloc(64);
   m.frp.alloc(G5_MAX_LEVELS);
   if (g.reorder_mode != 0 && g.nnz > 0) {
@@ -559,7 +561,9 @@ void ms_prepare(g5_ctx* ctx) {
   m.fr_exact.assign(m.fr.size(), 0);
   m.fr_count.assign(m.fr.size(), 0);
   if (m.compact_below > 0) ms_alloc_compact(ctx);
-  G5_HIP(hipHostMalloc((void**)&m.h_counters, (M_NUM + 1 + 128) * sizeof(uint64_t), hipHostMallocDefault));
+  G5_HIP(hipHostMalloc((void**)&m.h_counters, kMsCounters * sizeof(uint64_t), hipHostMallocDefault));
+  m.pub.alloc(kMsCounters);
+  m.pub_seq = 0;
   G5_HIP(hipEventCreate(&m.ev0));
   G5_HIP(hipEventCreate(&m.ev1));
   m.lev_begin.assign(G5_MAX_LEVELS, nullptr);
@@ -569,6 +573,28 @@ void ms_prepare(g5_ctx* ctx) {
     G5_HIP(hipEventCreate(&m.lev_end[i]));
   }
   m.prepared = true;
+}
+
+// Level end: the first n_pub counters the host decides the next level with
+// are published to coherent host memory, each tagged with the level's
+// sequence number, and every counter is cleared for the next level (so no
+// fill starts a level).
+__global__ void k_ms_publish(unsigned long long* __restrict__ c, int n_pub, Published* __restrict__ p,
+                             uint64_t seq) {
+  for (int t = threadIdx.x; t < kMsCounters; t += blockDim.x) {
+    const uint64_t v = c[t];
+    c[t] = 0;
+    if (t < n_pub) publish(p + t, v, seq);
+  }
+}
+
+static void ms_publish_wait(MsState& m, int n_pub, hipStream_t s) {
+  ++m.pub_seq;
+  hipLaunchKernelGGL(k_ms_publish, dim3(1), dim3(kBlock), 0, s, (unsigned long long*)m.counters.p, n_pub, m.pub.d,
+                     m.pub_seq);
+  G5_HIP(hipGetLastError());
+  m.pub.wait(0, n_pub, m.pub_seq, s, "g5_run_bfs_batch: level counters were not published");
+  for (int i = 0; i < n_pub; ++i) m.h_counters[i] = m.pub.value(i);
 }
 
 // Frontier words of level l, allocated on first use and kept for reuse.
@@ -683,7 +709,6 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     }
     m.level_push[l] = push ? live : td;
     m.level_pull[l] = push ? 0 : (lanes_mode ? bu : live);
-    G5_HIP(hipMemsetAsync(m.counters.p, 0, (M_NUM + 1 + 128) * 8, s));
     G5_HIP(hipEventRecord(m.lev_begin[l], s));
     // statistics for the roots still choosing (a root that pulled keeps pulling)
     const LaneEpi lepi{cnt + M_NUM + 1, g.offsets.p, lanes_mode ? td : live};
@@ -844,12 +869,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     G5_HIP(hipGetLastError());
     m.fr_exact[l] = 1;  // every level form leaves next non-zero exactly at the bits of nz_next
     G5_HIP(hipEventRecord(m.lev_end[l], s));
-    G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, (lane_stats ? M_NUM + 1 + 128 : M_NUM) * 8,
-                          hipMemcpyDeviceToHost, s));
-    G5_HIP(hipStreamSynchronize(s));
-    float lms = 0;
-    G5_HIP(hipEventElapsedTime(&lms, m.lev_begin[l], m.lev_end[l]));
-    st.level_ms[l] = lms;
+    ms_publish_wait(m, lane_stats ? kMsCounters : M_NUM, s);
     st.direction[l] = (lanes_mode && td && bu) ? 3 : (push ? 1 : (sweep ? 2 : 0));
     if (lane_stats)
       for (int r = 0; r < 64; ++r) {
@@ -864,6 +884,16 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
     nz = m.h_counters[M_NZ];
     m.fr_count[l] = (int64_t)nz;
     fdeg = m.h_counters[M_FDEG];
+  }
+  // level times, once the last level's events are complete (a level's
+  // published counters can reach the host before its end event is)
+  if (st.num_levels - 1 >= l0) {
+    G5_HIP(hipEventSynchronize(m.lev_end[st.num_levels - 1]));
+    for (int k = l0; k < st.num_levels; ++k) {
+      float lms = 0;
+      G5_HIP(hipEventElapsedTime(&lms, m.lev_begin[k], m.lev_end[k]));
+      st.level_ms[k] = lms;
+    }
   }
 }
 
@@ -1053,6 +1083,7 @@ static void ms_run_persistent(g5_ctx* ctx, const int64_t* roots, int nroots) {
     // which reads the last level's nonzero bitmap from its per-level slot
     const int64_t words = (g.n_rows + 63) / 64;
     G5_HIP(hipMemcpyAsync(ms_marks(m, done, words), m.nz[done & 1].p, words * 8, hipMemcpyDeviceToDevice, s));
+    G5_HIP(hipMemsetAsync(m.counters.p, 0, kMsCounters * 8, s));  // the host levels expect them clear
     ms_host_levels(ctx, done + 1, last[M_LIVE], last[M_NZ], last[M_FDEG], reached, roots_live);
   }
   (void)g;
@@ -1085,6 +1116,7 @@ void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots) {
     uint64_t* nz0 = ms_marks(m, 0, words);
     G5_HIP(hipMemcpyAsync(m.roots.p, roots, nroots * sizeof(int64_t), hipMemcpyHostToDevice, s));
     G5_HIP(hipEventRecord(m.ev0, s));
+    G5_HIP(hipMemsetAsync(m.counters.p, 0, kMsCounters * 8, s));  // then cleared by every publish
     G5_HIP(hipMemsetAsync(m.seen.p, 0, rows * 8, s));
     const bool marked = m.fr_exact[0] != 0;
     m.fr_exact[0] = 0;
@@ -1096,8 +1128,7 @@ void ms_run(g5_ctx* ctx, const int64_t* roots, int nroots) {
     G5_HIP(hipGetLastError());
     m.fr_exact[0] = 1;
     m.fr_count[0] = nroots;
-    G5_HIP(hipMemcpyAsync(m.h_counters, m.counters.p, M_NUM * 8, hipMemcpyDeviceToHost, s));
-    G5_HIP(hipStreamSynchronize(s));
+    ms_publish_wait(m, M_NUM, s);
     const uint64_t live = m.h_counters[M_LIVE];
     const double roots_live = (double)__builtin_popcountll(live);
     ms_host_levels(ctx, 1, live, m.h_counters[M_NZ], m.h_counters[M_FDEG], roots_live, roots_live);
