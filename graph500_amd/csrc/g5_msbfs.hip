@@ -646,8 +646,9 @@ static uint64_t* ms_marks(MsState& m, int l, int64_t words) {
 }
 
 // Host-driven levels from level l0 on (counters of level l0 - 1 in live /
-// nz / fdeg, reached pairs so far in reached): one or more launches and one
-// small device-to-host copy per level.
+// nz / fdeg, reached pairs so far in reached): one or more launches per
+// level, whose counters the host reads from coherent host memory (the
+// counters are clear when it is called).
 static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint64_t fdeg, double reached,
                            double roots_live) {
   Graph& g = ctx->graph;
