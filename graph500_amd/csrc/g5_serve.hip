@@ -351,6 +351,8 @@ struct g5_server {
   g5_server_config cfg{};
   int device = 0;
   int64_t done_count = 0;
+  int64_t run_base = 0;  // first request of the current run
+  int64_t run_done = 0;  // requests of the current run answered so far
   double t0 = 0;  // start of the current run (now_s clock)
   g5::DevBuf<uint8_t> lv_out;
 };
@@ -480,6 +482,7 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
       q.status = SV_DONE;
       q.done = t;
       ++srv->done_count;
+      if (id >= srv->run_base) ++srv->run_done;
       ++e.st.completed;
       q.levels = e.level - e.lane_l0[r];
       e.teps_req[r] = id;
@@ -697,6 +700,7 @@ static void sv_finish_solo(g5_server* srv, Engine& e) {
   q.edges = e.solo_edges;
   q.levels = e.solo_levels;
   ++srv->done_count;
+  if (e.solo_req >= srv->run_base) ++srv->run_done;
   ++e.st.completed;
   ++e.st.solo;
   e.solo_req = -1;
@@ -752,6 +756,10 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
     G5_CHECK(qs[i].root >= 0 && qs[i].root < srv->eng[qs[i].graph]->n, "g5_server_run: query root out of range");
     G5_CHECK(i == 0 || qs[i].arrival_s >= qs[i - 1].arrival_s, "g5_server_run: arrivals must be non-decreasing");
   }
+  // a single-root run left by an earlier run that ended in an error finishes
+  // first, so that it does not count as one of this run's answers
+  for (auto& e : srv->eng)
+    if (e->solo_state.load(std::memory_order_acquire) != 0) sv_finish_solo(srv, *e);
   const int64_t base = (int64_t)srv->req.size();
   srv->req.resize(base + nq);
   for (int64_t i = 0; i < nq; ++i) {
@@ -762,7 +770,10 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
     q.keep = qs[i].keep != 0;
   }
   int64_t next_arrival = 0, done = 0;
-  const int64_t done0 = srv->done_count;
+  // queries left waiting or in lanes by an earlier run that ended in an
+  // error are still answered, but only this run's count toward its end
+  srv->run_base = base;
+  srv->run_done = 0;
   const double t0 = now_s();
   srv->t0 = t0;
   int64_t levels0 = 0, busy0 = 0, events0 = 0;
@@ -786,7 +797,7 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
       const int a = __builtin_popcountll(e->active);
       if (a > max_active) max_active = a;
     }
-    done = srv->done_count - done0;
+    done = srv->run_done;
     if (!progress) {
       bool any_busy = false;
       for (auto& e : srv->eng) any_busy |= e->busy || e->solo_state.load(std::memory_order_relaxed) != 0;

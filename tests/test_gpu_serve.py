@@ -219,6 +219,21 @@ def test_lone_queries_single_root_path(graphs, solo):
         _check_kept(srv, res, specs, ctxs, ocs, range(0, 12, 2))
 
 
+def test_run_after_a_timed_out_run(graphs):
+    """A run that times out (possibly with a single-root run still going)
+    leaves the server usable: the next run answers exactly its own queries."""
+    import graph500_amd
+    ctxs, ocs = graphs
+    rng = np.random.default_rng(41)
+    with graph500_amd.G5Server(ctxs) as srv:
+        with pytest.raises(graph500_amd.G5Error):
+            srv.run(_queries([(2, _roots_with_edges(ctxs[2], 1, rng)[0], 0.0, 0)]), timeout_s=1e-4)
+        specs = [(0, _roots_with_edges(ctxs[0], 1, rng)[0], 0.0, 1), (1, _roots_with_edges(ctxs[1], 1, rng)[0], 0.0, 1)]
+        res, rep = srv.run(_queries(specs), timeout_s=60)
+        assert rep["completed"] == 2 and (res["status"] == 2).all()
+        _check_kept(srv, res, specs, ctxs, ocs, range(2))
+
+
 def test_lane_and_memory_pressure(graphs):
     """200 simultaneous queries on one graph (more than its 64 lanes) with only
     5 answer arrays: queries wait for lanes and memory, all complete."""
