@@ -227,7 +227,7 @@ def test_run_after_a_timed_out_run(graphs):
     rng = np.random.default_rng(41)
     with graph500_amd.G5Server(ctxs) as srv:
         with pytest.raises(graph500_amd.G5Error):
-            srv.run(_queries([(2, _roots_with_edges(ctxs[2], 1, rng)[0], 0.0, 0)]), timeout_s=1e-4)
+            srv.run(_queries([(2, _roots_with_edges(ctxs[2], 1, rng)[0], 0.0, 0)]), timeout_s=1e-5)
         specs = [(0, _roots_with_edges(ctxs[0], 1, rng)[0], 0.0, 1), (1, _roots_with_edges(ctxs[1], 1, rng)[0], 0.0, 1)]
         res, rep = srv.run(_queries(specs), timeout_s=60)
         assert rep["completed"] == 2 and (res["status"] == 2).all()
