@@ -601,6 +601,58 @@ __device__ __forceinline__ void push_chunk(uint64_t chunk, const uint32_t* __res
   __syncthreads();
 }
 
+// push_chunk that also lists the targets it touches: the one atomicOr that
+// finds a target's word still zero (next was cleared for the level) appends
+// the target to touch (wave-aggregated appends to *touch_count); every
+// target is listed once, so touch needs room for the rows.
+__device__ __forceinline__ void push_chunk_touch(uint64_t chunk, const uint32_t* __restrict__ chunk_rows,
+                                                 uint64_t nnz, const uint64_t* __restrict__ offsets,
+                                                 const uint32_t* __restrict__ cols, const uint64_t* __restrict__ front,
+                                                 const uint64_t* __restrict__ nz_front, uint64_t* next, SweepLds& L,
+                                                 uint64_t lanes, uint32_t* __restrict__ touch,
+                                                 unsigned long long* __restrict__ touch_count) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t r0 = chunk_rows[2 * chunk];
+  const uint64_t e0 = chunk * kSweepEpb;
+  const uint64_t e1 = (e0 + kSweepEpb < nnz) ? e0 + kSweepEpb : nnz;
+  const int nr = sweep_stage(chunk, chunk_rows, offsets, L, false);
+  const int len = (int)(e1 - e0);
+  int lo = -1;
+  uint64_t fu = 0;
+  // every lane of a wave runs the same number of steps (kBlock is a multiple
+  // of the wave), so the appends' ballots see the whole wave
+  for (int e0l = threadIdx.x & ~63; e0l < len; e0l += kBlock) {
+    const int e = e0l + lane;
+    bool first = false;
+    uint32_t w = 0;
+    if (e < len) {
+      if (lo < 0 || (lo + 1 < nr && L.off[lo + 1] <= e)) {
+        int a = lo < 0 ? 0 : lo + 1, b = nr - 1;  // last staged row starting at or before e
+        while (a < b) {
+          const int mid = (a + b + 1) >> 1;
+          if (L.off[mid] <= e) a = mid; else b = mid - 1;
+        }
+        lo = a;
+        const uint32_t u = r0 + lo;
+        fu = bit_set(nz_front, u) ? front[u] & lanes : 0;
+      }
+      if (fu) {
+        w = cols[e0 + e];
+        if (fu & ~__atomic_load_n(next + w, __ATOMIC_RELAXED))
+          first = atomicOr((unsigned long long*)(next + w), (unsigned long long)fu) == 0;
+      }
+    }
+    const uint64_t b = __ballot(first);
+    if (b) {
+      unsigned long long pos = 0;
+      if (lane == 0) pos = atomicAdd(touch_count, (unsigned long long)__popcll(b));
+      pos = __shfl(pos, 0, 64);
+      if (first) touch[pos + __popcll(b & ((UINT64_C(1) << lane) - 1))] = w;
+    }
+  }
+  __syncthreads();
+}
+
 // Chunks with a frontier row -> list (wave-aggregated appends to *count,
 // which must be zero beforehand); order within the list is arbitrary.
 __device__ __forceinline__ void ms_collect_chunks_body(const uint32_t* __restrict__ chunk_rows, uint64_t nchunks,

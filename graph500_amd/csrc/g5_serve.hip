@@ -47,6 +47,7 @@ struct ServeEpi {
   uint64_t* nz_sel;
   uint64_t sel;
   uint64_t stat_lanes;  // lanes whose counts and degree sums the host still needs
+  int list;             // the fold walks a list of vertices, not every row's word
 };
 
 // Thread i of a wave keeps lane i's answer pointer and level in registers
@@ -95,8 +96,14 @@ struct ServeLocal {
         d_acc += ((uint64_t)dhi << 32) | dlo;
       }
     }
-    const uint64_t bs = __ballot(inb && (nw & p.sel) != 0);
-    if (me == 0) p.nz_sel[v >> 6] = bs;
+    if (p.list) {
+      // a listed fold: nz_sel was cleared, its bits are set one vertex at a time
+      if (inb && (nw & p.sel) != 0)
+        atomicOr((unsigned long long*)(p.nz_sel + (v >> 6)), 1ull << (v & 63));
+    } else {
+      const uint64_t bs = __ballot(inb && (nw & p.sel) != 0);
+      if (me == 0) p.nz_sel[v >> 6] = bs;
+    }
   }
   __device__ void end() const {
     const int me = threadIdx.x & 63;
@@ -117,6 +124,8 @@ struct ServeLocal {
 // commands on the stream.
 constexpr int kSvTeps = M_NUM + 128;
 constexpr int kSvPub = kSvTeps + 64;
+constexpr int kSvTouch = M_NUM + 1 + 128;  // device counter slot of a listed push's touched vertices
+constexpr int kSvCounters = kSvTouch + 1;
 __global__ void k_sv_publish(const unsigned long long* __restrict__ c, const unsigned long long* __restrict__ tacc,
                              Published* __restrict__ p, uint64_t seq) {
   const int t = threadIdx.x;
@@ -172,6 +181,43 @@ __global__ void __launch_bounds__(kBlock) k_sv_push(const uint32_t* __restrict__
   __shared__ SweepLds L;
   ms_push_list_body(list, count, chunk_rows, nnz, offsets, cols, front, nz_front, next, L, blockIdx.x, gridDim.x,
                     lanes);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sv_push_touch(const uint32_t* __restrict__ list, const unsigned* count,
+                                                          const uint32_t* __restrict__ chunk_rows, uint64_t nnz,
+                                                          const uint64_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ cols,
+                                                          const uint64_t* __restrict__ front,
+                                                          const uint64_t* __restrict__ nz_front, uint64_t* next,
+                                                          uint64_t lanes, uint32_t* __restrict__ touch,
+                                                          unsigned long long* __restrict__ touch_count) {
+  __shared__ SweepLds L;
+  const unsigned n = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (unsigned i = blockIdx.x; i < n; i += gridDim.x)
+    push_chunk_touch(list[i], chunk_rows, nnz, offsets, cols, front, nz_front, next, L, lanes, touch, touch_count);
+}
+
+// The fold of a small push level over the vertices it touched (no lane
+// cooling: the full fold also clears cooling lanes from seen).
+__global__ void __launch_bounds__(kBlock) k_sv_finish_list(uint64_t* __restrict__ seen, uint64_t* __restrict__ next,
+                                                           uint64_t* __restrict__ nz_next,
+                                                           const uint64_t* __restrict__ offsets,
+                                                           const uint32_t* __restrict__ touch,
+                                                           const unsigned long long* __restrict__ touch_count,
+                                                           unsigned long long* __restrict__ c, ServeEpi epi) {
+  const ServeLocal el{epi};
+  ms_finish_list_body(seen, next, nz_next, offsets, touch, (uint64_t)*touch_count, c, blockIdx.x, gridDim.x, el);
+}
+
+// Zeroes the frontier words marked in a nonzero bitmap (one bitmap word per
+// lane; for sparse marks).
+__global__ void __launch_bounds__(kBlock) k_sv_clear_marked(const uint64_t* __restrict__ marked, int64_t words,
+                                                            int64_t rows, uint64_t* __restrict__ fr) {
+  for (int64_t wd = (int64_t)blockIdx.x * kBlock + threadIdx.x; wd < words; wd += (int64_t)gridDim.x * kBlock)
+    for (uint64_t b = marked[wd]; b; b &= b - 1) {
+      const int64_t v = wd * 64 + __builtin_ctzll(b);
+      if (v < rows) fr[v] = 0;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_sv_select(const uint64_t* __restrict__ front, int64_t rows, uint64_t lanes,
@@ -293,7 +339,10 @@ struct Engine {
   DevBuf<uint64_t> nzs[2];  // nonzero bitmap restricted to the lanes that were top-down
   uint64_t nzs_lanes[2] = {0, 0};  // the lanes whose frontier vertices nzs[i] marks
   DevBuf<uint32_t> chunk_rows, chunk_list;
-  DevBuf<unsigned long long> cnt;    // M_NUM counters, chunk count, 64 + 64 per-lane counts and degree sums
+  DevBuf<unsigned long long> cnt;    // M_NUM counters, chunk count, 64 + 64 per-lane counts and degree
+                                     // sums, the touched-vertex count of a listed push (kSvTouch)
+  DevBuf<uint32_t> touch;            // rows: the vertices a listed push level touched
+  int64_t nzc[2] = {0, 0};           // vertices marked in nz[i] (the frontier words non-zero in fr[i])
   DevBuf<unsigned long long> tacc;   // per-lane tuple-weight sums of a retirement
   unsigned long long* h = nullptr;   // pinned: the per-lane weight sums of a retirement, copied at the end
   PublishBlock pub;                  // the level counters and per-lane counts (k_sv_publish)
@@ -376,7 +425,8 @@ static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
   e.nchunks = ((uint64_t)e.nnz + kSweepEpb - 1) / kSweepEpb;
   e.chunk_rows.alloc(2 * e.nchunks + 2);
   e.chunk_list.alloc(e.nchunks + 1);
-  e.cnt.alloc(M_NUM + 1 + 128);
+  e.cnt.alloc(kSvCounters);
+  e.touch.alloc(e.rows ? e.rows : 1);
   e.tacc.alloc(64);
   G5_HIP(hipMemsetAsync(e.tacc.p, 0, 64 * 8, e.s));
   G5_HIP(hipHostMalloc((void**)&e.h, 64 * sizeof(unsigned long long), hipHostMallocDefault));
@@ -432,6 +482,7 @@ static void sv_complete_level(g5_server* srv, Engine& e, double t) {
   sv_take_teps(srv, e, true);
   const uint64_t live = e.pub.value(M_LIVE);
   e.last_fdeg = e.pub.value(M_FDEG);
+  e.nzc[e.level & 1] = (int64_t)e.pub.value(M_NZ);  // the level's frontier words now in fr[level & 1]
   e.cooling = 0;  // cleared from seen by the level just finished
   const uint64_t finished = e.active & ~live;
   // each search's own direction for its next level
@@ -558,6 +609,7 @@ static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
                        e.fr[e.level & 1].p, e.nz[e.level & 1].p, e.nzs[e.level & 1].p);
     G5_HIP(hipGetLastError());
     e.nzs_lanes[e.level & 1] |= admitted;
+    e.nzc[e.level & 1] += __builtin_popcountll(admitted);  // their roots' words join the marks
     e.active |= admitted;
     const int64_t k = __builtin_popcountll(admitted);
     (in_flight ? e.st.admitted_busy : e.st.admitted_idle) += k;
@@ -613,19 +665,41 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
   if (reselect)
     hipLaunchKernelGGL(k_sv_select, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, front, e.rows, td,
                        e.nzs[cur].p);
-  G5_HIP(hipMemsetAsync(c, 0, (M_NUM + 1 + 128) * 8, s));
+  G5_HIP(hipMemsetAsync(c, 0, kSvCounters * 8, s));
   ++e.st.levels;
   if (td && bu) ++e.st.mixed_levels; else if (td) ++e.st.push_levels; else ++e.st.pull_levels;
   e.st.lane_levels += __builtin_popcountll(e.active);
+  // A push level with few frontier edges and no lane cooling lists the
+  // vertices it touches and folds only those (the full fold also clears the
+  // cooling lanes from seen).
+  const bool listed = td && !bu && !e.cooling && (double)e.last_fdeg * 8.0 < (double)e.rows;
+  epi.list = listed ? 1 : 0;
   if (td) {
-    hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, s, next, e.rows);
+    // next held the frontier of two levels back, non-zero exactly at the
+    // bits of nz_next's last contents: few of them are cleared one by one
+    if (e.nzc[cur ^ 1] * 16 < e.rows)
+      hipLaunchKernelGGL(k_sv_clear_marked, dim3(grid_of(e.words, kBlock, 4096)), dim3(kBlock), 0, s, nz_next,
+                         e.words, e.rows, next);
+    else
+      hipLaunchKernelGGL(k_sv_zero, dim3(grid_of(e.rows, kBlock, 4096)), dim3(kBlock), 0, s, next, e.rows);
     hipLaunchKernelGGL(k_sv_collect, dim3(grid_of((int64_t)e.nchunks, kBlock, 4096)), dim3(kBlock), 0, s,
                        e.chunk_rows.p, e.nchunks, nz_push, e.chunk_list.p, (unsigned*)(c + M_NUM));
-    hipLaunchKernelGGL(k_sv_push, dim3(grid_of((int64_t)e.nchunks, 1, 4096)), dim3(kBlock), 0, s, e.chunk_list.p,
-                       (const unsigned*)(c + M_NUM), e.chunk_rows.p, (uint64_t)e.nnz, g.offsets.p, g.cols.p, front,
-                       nz_front, next, td);
+    if (listed) {
+      G5_HIP(hipMemsetAsync(nz_next, 0, e.words * 8, s));  // after the clear above read it
+      G5_HIP(hipMemsetAsync(epi.nz_sel, 0, e.words * 8, s));
+      hipLaunchKernelGGL(k_sv_push_touch, dim3(grid_of((int64_t)e.nchunks, 1, 4096)), dim3(kBlock), 0, s,
+                         e.chunk_list.p, (const unsigned*)(c + M_NUM), e.chunk_rows.p, (uint64_t)e.nnz, g.offsets.p,
+                         g.cols.p, front, nz_front, next, td, e.touch.p, c + kSvTouch);
+    } else {
+      hipLaunchKernelGGL(k_sv_push, dim3(grid_of((int64_t)e.nchunks, 1, 4096)), dim3(kBlock), 0, s, e.chunk_list.p,
+                         (const unsigned*)(c + M_NUM), e.chunk_rows.p, (uint64_t)e.nnz, g.offsets.p, g.cols.p, front,
+                         nz_front, next, td);
+    }
   }
-  if (bu) {
+  if (listed) {
+    hipLaunchKernelGGL(k_sv_finish_list, dim3(grid_of(e.rows / 8 + 1, kBlock, 2048)), dim3(kBlock), 0, s, e.seen.p,
+                       next, nz_next, g.offsets.p, e.touch.p, (const unsigned long long*)(c + kSvTouch), c, epi);
+  } else if (bu) {
     hipLaunchKernelGGL(k_sv_pull, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, e.seen.p, front,
                        nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, e.rows, bu,
                        srv->cfg.pull_filter, c, keep, td ? 1 : 0, epi);
