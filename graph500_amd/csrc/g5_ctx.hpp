@@ -62,7 +62,7 @@ struct BfsState {
   uint64_t pub_seq = 0;
   PublishBlock fuse;              // the fused small levels' state (g5_bfs.hip)
   uint64_t fuse_seq = 0;
-  int64_t small_edges = 32768;    // small top-down levels up to this many edges run fused (0: never)
+  int64_t small_edges = 8192;     // small top-down levels up to this many edges run fused (0: never)
   int64_t tail_pull = 4;          // post-bottom-up top-down levels with tail_pull * nf > unvisited run bottom-up
   DevBuf<int64_t> pred_out;       // n, built on demand by get_pred
   DevBuf<uint8_t> level_out;      // n, built on demand by get_levels

@@ -260,7 +260,7 @@ int g5_bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
  *   small_edges: consecutive top-down levels whose frontiers have at most
  *     4096 vertices and at most small_edges edges run in one single-workgroup
  *     launch that applies the direction rule itself, with no host round trip
- *     between them (default 32768; 0 = off; at most 2^20).
+ *     between them (default 8192; 0 = off; at most 2^20).
  *   tail_pull: a top-down level after the bottom-up phase whose frontier
  *     holds more than 4096 vertices and more than 1/tail_pull as many as
  *     remain unvisited finds its vertices with the bottom-up kernel (every unvisited vertex looks
