@@ -264,6 +264,23 @@ def profile_figure(kernel, scale, key):
     return float(m.group(1)) if m else None
 
 
+def gather_peak_beyond_l2(min_bytes=32 << 20):
+    """The best random 8-byte read rate (G reads/s) of the committed gather
+    calibration (profiles/random_gather_rate.txt) over arrays of at least
+    min_bytes, i.e. larger than the 8 XCD L2s together: the request-rate
+    ceiling of random reads that miss the L2, or None."""
+    path = os.path.join(HERE, "profiles", "random_gather_rate.txt")
+    if not os.path.exists(path):
+        return None
+    unit = {"KiB": 1 << 10, "MiB": 1 << 20, "GiB": 1 << 30}
+    best = None
+    for m in re.finditer(r"^\s*(\d+) (KiB|MiB|GiB)\s+([0-9.]+)\s+([0-9.]+)", open(path).read(), re.M):
+        if int(m.group(1)) * unit[m.group(2)] >= min_bytes:
+            rate = float(m.group(4))
+            best = rate if best is None else max(best, rate)
+    return best
+
+
 def single_root_leg(ctx, roots, warmup):
     """Each root once through g5_run_bfs; host clock per call."""
     ctx.prepare_bfs()
@@ -535,6 +552,7 @@ def main():
     traffic = traffic_from_profile(BATCH_KERNELS[dom][0], a.scale)
     requests = profile_figure(BATCH_KERNELS[dom][0], a.scale, "memory_requests_per_launch")
     gather_rate = profile_figure(BATCH_KERNELS[dom][0], a.scale, "random_read_rate_g_per_s")
+    request_peak = gather_peak_beyond_l2()
     ctx_tuning = ctx.batch_tuning()
     out = {
         "metric": METRIC,
@@ -579,6 +597,11 @@ def main():
             "request_rate_g_per_s": (round(requests / (dom_ms / max(dom_launches, 1) / 1e3) / 1e9, 1)
                                      if requests and dom_ms > 0 else None),
             "random_read_rate_g_per_s": gather_rate,
+            # the request-rate roofline: the level's requests per second
+            # against the best random-read rate beyond the L2s
+            "request_peak_g_per_s": request_peak,
+            "request_frac": (round(requests / (dom_ms / max(dom_launches, 1) / 1e3) / 1e9 / request_peak, 3)
+                             if requests and dom_ms > 0 and request_peak else None),
         },
         "batch": {
             "gteps": round(gteps, 3),

@@ -1,5 +1,6 @@
 """CPU checks of bench.py's contract and of its rocprof helpers: the
-multi-GPU refusal, the traffic figure read from the committed profile, and
+multi-GPU refusal, the traffic figure read from the committed profile, the
+request-rate ceiling read from the committed gather calibration, and
 the summaries of a rocprofv3 results database (a small stand-in database with
 the tables and columns the helpers read)."""
 import os
@@ -29,6 +30,19 @@ def test_traffic_from_committed_profile():
     text = open(os.path.join(ROOT, "profiles", "k_ms_mixed_s28_traffic.txt")).read()
     assert "traffic_bytes_per_launch: %.0f" % t in text
     assert bench.traffic_from_profile("no_such_kernel", 28) is None
+
+
+def test_request_peak_from_committed_calibration():
+    """The request-rate ceiling is the best random-read rate of the committed
+    gather calibration over arrays past the L2s (32 MiB and up), read from
+    its table; restricted to 1 GiB and up it is the 1 GiB row."""
+    text = open(os.path.join(ROOT, "profiles", "random_gather_rate.txt")).read()
+    peak = bench.gather_peak_beyond_l2()
+    assert peak is not None and 40.0 < peak < 100.0
+    assert " %.1f " % peak in text
+    big = bench.gather_peak_beyond_l2(1 << 30)
+    assert big is not None and big <= peak
+    assert bench.gather_peak_beyond_l2(1 << 40) is None
 
 
 def _trace_db(path):
