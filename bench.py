@@ -355,10 +355,11 @@ def hub_edges(level_bytes, nq, L):
 # in flight still pushes, once min(min_admit, free answer arrays) queries can
 # start together, or once the oldest has waited max_wait_s; each search pulls
 # while its frontier's edges exceed 1/alpha of its unreached edges.  A graph
-# with no search in flight and one waiting query answers it with the
-# single-root kernels (solo).
+# with no search in lanes and at most 8 waiting queries answers them with the
+# single-root kernels, up to 3 at once on slots of their own (solo,
+# solo_slots); a longer queue is batched into lanes.
 SERVE_POLICY = {"answer_buffers": 128, "min_admit": 64, "max_wait_s": 0.3, "lane_rule": 3, "alpha": 64.0,
-                "solo": 1}
+                "solo": 8, "solo_slots": 3}
 
 
 def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):

@@ -1004,7 +1004,7 @@ static void wait_published(BfsState& b, hipStream_t s) {
   for (int i = 0; i < C_NUM; ++i) b.h_counters[i] = b.pub.value(i);
 }
 
-static void build_hub_index(g5_ctx* ctx);
+static void build_hub_index(g5_ctx* ctx, BfsState& b, hipStream_t s);
 
 // Buffers of the segment-list levels, sized for the tuning so that no run
 // allocates: a growing top-down level has at most n_active / 2000 frontier
@@ -1013,10 +1013,11 @@ static void build_hub_index(g5_ctx* ctx);
 // times its pieces, within seg_max.  The block index of the first rows is
 // built here too.  (The binned form, off by default, sizes its pair buffer
 // by the level's edges inside the run.)
-void bfs_tuning_buffers(g5_ctx* ctx) {
+void bfs_tuning_buffers(g5_ctx* ctx) { bfs_tuning_buffers(ctx, ctx->bfs, ctx->stream); }
+
+void bfs_tuning_buffers(g5_ctx* ctx, BfsState& b, hipStream_t s) {
   Graph& g = ctx->graph;
-  BfsState& b = ctx->bfs;
-  if (b.hub_shift > 0 && b.hub_built_shift != b.hub_shift) build_hub_index(ctx);
+  if (b.hub_shift > 0 && b.hub_built_shift != b.hub_shift) build_hub_index(ctx, b, s);
   const int64_t seg_max = b.seg_max > 0 ? std::min<int64_t>(b.seg_max, kSegMax) : kSegMax;
   const int64_t nf_max = g.n_active / 2000 + 1;
   int64_t pieces = 1;
@@ -1031,10 +1032,11 @@ void bfs_tuning_buffers(g5_ctx* ctx) {
   b.seg_scan_tmp.alloc(scan_tmp_elems(seg_n));
 }
 
-void bfs_prepare(g5_ctx* ctx) {
+void bfs_prepare(g5_ctx* ctx) { bfs_prepare(ctx, ctx->bfs, ctx->stream); }
+
+void bfs_prepare(g5_ctx* ctx, BfsState& b, hipStream_t s) {
   Graph& g = ctx->graph;
   G5_CHECK(g.built, "g5_prepare_bfs: no graph (call g5_construct first)");
-  BfsState& b = ctx->bfs;
   b.release();
   const int64_t rows = g.n_rows;
   b.words = (rows + 63) / 64;
@@ -1047,7 +1049,7 @@ void bfs_prepare(g5_ctx* ctx) {
   const size_t scan_n = (size_t)rows + 1;
   b.scan_tmp.alloc(scan_tmp_elems(scan_n));
   b.counters.alloc(C_NUM);
-  bfs_tuning_buffers(ctx);
+  bfs_tuning_buffers(ctx, b, s);
   G5_HIP(hipHostMalloc((void**)&b.h_counters, C_NUM * sizeof(uint64_t), hipHostMallocDefault));
   b.pub.alloc(C_NUM);
   b.pub_seq = 0;
@@ -1067,27 +1069,26 @@ void bfs_prepare(g5_ctx* ctx) {
 // The block index of the first rows: as many as fit kHubIndexBytes.
 constexpr int64_t kHubIndexBytes = INT64_C(512) << 20;
 
-static void build_hub_index(g5_ctx* ctx) {
+static void build_hub_index(g5_ctx* ctx, BfsState& b, hipStream_t s) {
   Graph& g = ctx->graph;
-  BfsState& b = ctx->bfs;
   const int shift = (int)b.hub_shift;
   const int64_t nb = ((g.n_rows - 1) >> shift) + 1;
   b.hub_pieces = nb + 1;
   b.hub_rows = std::min<int64_t>(g.n_rows, kHubIndexBytes / (4 * b.hub_pieces));
   b.hub_bnd.alloc((size_t)(b.hub_rows * b.hub_pieces));
   hipLaunchKernelGGL(k_hub_bounds, dim3(grid_of(b.hub_rows * b.hub_pieces, kBlock, 65536)), dim3(kBlock), 0,
-                     ctx->stream, g.offsets.p, g.cols.p, b.hub_rows, b.hub_pieces, shift, b.hub_bnd.p);
+                     s, g.offsets.p, g.cols.p, b.hub_rows, b.hub_pieces, shift, b.hub_bnd.p);
   G5_HIP(hipGetLastError());
-  G5_HIP(hipStreamSynchronize(ctx->stream));
+  G5_HIP(hipStreamSynchronize(s));
   b.hub_built_shift = b.hub_shift;
 }
 
-void bfs_run(g5_ctx* ctx, int64_t root) {
+void bfs_run(g5_ctx* ctx, int64_t root) { bfs_run(ctx, ctx->bfs, ctx->stream, root); }
+
+void bfs_run(g5_ctx* ctx, BfsState& b, hipStream_t s, int64_t root) {
   Graph& g = ctx->graph;
-  BfsState& b = ctx->bfs;
   G5_CHECK(b.prepared, "g5_run_bfs: call g5_prepare_bfs first");
   G5_CHECK(root >= 0 && root < g.n, "g5_run_bfs: root out of range");
-  hipStream_t s = ctx->stream;
   const int64_t rows = g.n_rows;
   unsigned long long* cnt = (unsigned long long*)b.counters.p;
   b.have_result = false;
@@ -1095,7 +1096,7 @@ void bfs_run(g5_ctx* ctx, int64_t root) {
   g5_bfs_stats& st = b.stats;
   st = g5_bfs_stats{};
 
-  if (b.hub_shift > 0 && b.hub_built_shift != b.hub_shift) bfs_tuning_buffers(ctx);
+  if (b.hub_shift > 0 && b.hub_built_shift != b.hub_shift) bfs_tuning_buffers(ctx, b, s);
 
   G5_HIP(hipEventRecord(b.ev0, s));
   hipLaunchKernelGGL(k_bfs_init, dim3(grid_of(rows, kBlock, 4096)), dim3(kBlock), 0, s, b.level.p,
@@ -1401,12 +1402,15 @@ void bfs_get_levels(g5_ctx* ctx, uint8_t* host_levels) {
 }
 
 int64_t bfs_teps_edges(g5_ctx* ctx) {
-  Graph& g = ctx->graph;
-  BfsState& b = ctx->bfs;
-  G5_CHECK(b.have_result, "g5_teps_edges: no BFS result (call g5_run_bfs first)");
   ctx->acc.alloc(4);
-  const uint64_t sum = device_reduce_sum(ReachedWeight{b.level.p, g.weight.p}, (size_t)g.n_rows,
-                                         ctx->acc.p, ctx->stream);
+  return bfs_teps_edges(ctx, ctx->bfs, ctx->acc, ctx->stream);
+}
+
+int64_t bfs_teps_edges(g5_ctx* ctx, const BfsState& b, DevBuf<uint64_t>& acc, hipStream_t s) {
+  Graph& g = ctx->graph;
+  G5_CHECK(b.have_result, "g5_teps_edges: no BFS result (call g5_run_bfs first)");
+  G5_CHECK(acc.n >= 4, "g5_teps_edges: reduction buffer too small");
+  const uint64_t sum = device_reduce_sum(ReachedWeight{b.level.p, g.weight.p}, (size_t)g.n_rows, acc.p, s);
   return (int64_t)(sum / 2);
 }
 

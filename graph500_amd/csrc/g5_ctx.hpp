@@ -86,6 +86,16 @@ struct BfsState {
   g5_bfs_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> lev_begin, lev_end;  // per level
+  // the tuning (g5_bfs_set_tuning's fields) of another state
+  void copy_tuning(const BfsState& o) {
+    small_edges = o.small_edges;
+    tail_pull = o.tail_pull;
+    bin_edges = o.bin_edges;
+    td_pieces = o.td_pieces;
+    td_head = o.td_head;
+    seg_max = o.seg_max;
+    hub_shift = o.hub_shift;
+  }
   void release() {
     for (auto& v : vis) v.release();
     level.release();
@@ -280,6 +290,12 @@ void bfs_run(g5_ctx* ctx, int64_t root_orig);
 void bfs_get_pred(g5_ctx* ctx, int64_t* host_pred);
 void bfs_get_levels(g5_ctx* ctx, uint8_t* host_levels);
 int64_t bfs_teps_edges(g5_ctx* ctx);
+// the same on a BFS state and stream other than the context's own (the
+// graph is the context's; several states may run on one graph at once)
+void bfs_prepare(g5_ctx* ctx, BfsState& b, hipStream_t s);
+void bfs_tuning_buffers(g5_ctx* ctx, BfsState& b, hipStream_t s);
+void bfs_run(g5_ctx* ctx, BfsState& b, hipStream_t s, int64_t root_orig);
+int64_t bfs_teps_edges(g5_ctx* ctx, const BfsState& b, DevBuf<uint64_t>& acc, hipStream_t s);
 void bfs_model_bytes(g5_ctx* ctx, double* per_level, double* total);
 // g5_validate.hip
 void bfs_validate(g5_ctx* ctx, int64_t* errors_out);

@@ -330,6 +330,33 @@ struct Request {
   int64_t result_slot = -1;
 };
 
+// A slot that answers one query at a time with the single-root kernels
+// (config solo), driven by a host thread of its own so that the server loop
+// goes on with the other graphs and slots; the thread owns the slot from
+// state 1 until it sets state 2.  Slot 0 runs on the graph context's own BFS
+// state and stream, every further slot (config solo_slots) on a state and a
+// stream of its own, so that the slots of one graph overlap on the device.
+struct SoloSlot {
+  BfsState own;              // the state of a slot beyond the first
+  BfsState* b = nullptr;     // the state the runs use
+  hipStream_t s = nullptr;
+  bool own_stream = false;
+  DevBuf<uint64_t> acc;      // the traversed-edge reduction's scratch
+  std::thread th;
+  std::atomic<int> state{0};  // 0 free, 1 running, 2 finished
+  int64_t req = -1;
+  int64_t edges = 0;
+  int levels = 0;
+  double done = 0;  // seconds after the run's start
+  std::string err;
+
+  ~SoloSlot() {
+    if (th.joinable()) th.join();
+    own.release();
+    if (own_stream && s) (void)hipStreamDestroy(s);
+  }
+};
+
 struct Engine {
   g5_ctx* ctx = nullptr;
   hipStream_t s = nullptr;
@@ -370,19 +397,17 @@ struct Engine {
   std::deque<int64_t> waiting;
   int64_t levels_run = 0;
   int64_t admit_events = 0;
-  // a lone query answered by the single-root kernels on a host thread of its
-  // own (config solo): the thread owns the graph context and its stream until
-  // it sets solo_state to 2
-  std::thread solo_thread;
-  std::atomic<int> solo_state{0};  // 0 none, 1 running, 2 finished
-  int64_t solo_req = -1;
-  int64_t solo_edges = 0;
-  int solo_levels = 0;
-  double solo_done = 0;  // seconds after the run's start
-  std::string solo_err;
+  // single-root run slots (config solo, solo_slots)
+  std::vector<std::unique_ptr<SoloSlot>> solo;
+
+  bool solo_running() const {
+    for (auto& sl : solo)
+      if (sl->state.load(std::memory_order_acquire) != 0) return true;
+    return false;
+  }
 
   ~Engine() {
-    if (solo_thread.joinable()) solo_thread.join();
+    solo.clear();  // joins the runs still going
     if (h) (void)hipHostFree(h);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_end) (void)hipEventDestroy(ev_end);
@@ -407,6 +432,29 @@ struct g5_server {
 };
 
 namespace g5 {
+
+// The single-root slots of a graph (config solo): slot 0 on the context's own
+// BFS state and stream, the others on states of their own with the context's
+// BFS tuning.
+static void sv_setup_solo(Engine& e, int slots) {
+  g5_ctx* ctx = e.ctx;
+  if (!ctx->bfs.prepared) bfs_prepare(ctx);
+  for (int i = 0; i < slots; ++i) {
+    std::unique_ptr<SoloSlot> sl(new SoloSlot());
+    if (i == 0) {
+      sl->b = &ctx->bfs;
+      sl->s = ctx->stream;
+    } else {
+      sl->own.copy_tuning(ctx->bfs);
+      G5_HIP(hipStreamCreate(&sl->s));
+      sl->own_stream = true;
+      bfs_prepare(ctx, sl->own, sl->s);
+      sl->b = &sl->own;
+    }
+    sl->acc.alloc(4);
+    e.solo.push_back(std::move(sl));
+  }
+}
 
 static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
   Graph& g = ctx->graph;
@@ -716,12 +764,11 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
   ++e.levels_run;
 }
 
-// A lone query on an idle graph (config solo): g5_run_bfs's kernels, driven
-// by a host thread so that the loop goes on with the other graphs.  The
-// single-root level array (n_rows bytes in row order, the root 0, unreached
-// 0xFF) is the answer array's format, and its traversed-edge count the
-// engine's.
-static void sv_start_solo(g5_server* srv, Engine& e, double t) {
+// The oldest waiting query of an idle graph, answered on a single-root slot:
+// g5_run_bfs's kernels on the slot's state and stream.  The single-root level
+// array (n_rows bytes in row order, the root 0, unreached 0xFF) is the answer
+// array's format, and its traversed-edge count the engine's.
+static void sv_start_solo(g5_server* srv, Engine& e, SoloSlot& sl, double t) {
   const int64_t id = e.waiting.front();
   e.waiting.pop_front();
   Request& q = srv->req[id];
@@ -733,61 +780,84 @@ static void sv_start_solo(g5_server* srv, Engine& e, double t) {
     e.free_bufs.pop_back();
     ans = e.bufs[q.buf]->p;
   }
-  if (e.solo_thread.joinable()) e.solo_thread.join();
-  e.solo_req = id;
-  e.solo_err.clear();
-  e.solo_state.store(1, std::memory_order_release);
+  if (sl.th.joinable()) sl.th.join();
+  sl.req = id;
+  sl.err.clear();
+  sl.state.store(1, std::memory_order_release);
   ++e.admit_events;
   ++e.st.admitted_idle;
-  Engine* ep = &e;
+  SoloSlot* sp = &sl;
+  g5_ctx* ctx = e.ctx;
+  const int64_t rows = e.rows;
   const int dev = srv->device;
   const double t0 = srv->t0;
   const int64_t root = q.root;
-  e.solo_thread = std::thread([ep, dev, t0, root, ans] {
+  sl.th = std::thread([sp, ctx, rows, dev, t0, root, ans] {
     try {
       G5_HIP(hipSetDevice(dev));
-      g5_ctx* ctx = ep->ctx;
-      bfs_run(ctx, root);
+      bfs_run(ctx, *sp->b, sp->s, root);
       if (ans) {
-        G5_HIP(hipMemcpyAsync(ans, ctx->bfs.level.p, (size_t)ep->rows, hipMemcpyDeviceToDevice, ctx->stream));
-        G5_HIP(hipStreamSynchronize(ctx->stream));
+        G5_HIP(hipMemcpyAsync(ans, sp->b->level.p, (size_t)rows, hipMemcpyDeviceToDevice, sp->s));
+        G5_HIP(hipStreamSynchronize(sp->s));
       }
-      ep->solo_edges = bfs_teps_edges(ctx);
-      ep->solo_levels = ctx->bfs.stats.num_levels - 1;
-      ep->solo_done = now_s() - t0;
+      sp->edges = bfs_teps_edges(ctx, *sp->b, sp->acc, sp->s);
+      sp->levels = sp->b->stats.num_levels - 1;
+      sp->done = now_s() - t0;
     } catch (const Error& x) {
-      ep->solo_err = x.msg.empty() ? "serving: single-root run failed" : x.msg;
+      sp->err = x.msg.empty() ? "serving: single-root run failed" : x.msg;
     } catch (...) {
-      ep->solo_err = "serving: single-root run failed";
+      sp->err = "serving: single-root run failed";
     }
-    ep->solo_state.store(2, std::memory_order_release);
+    sp->state.store(2, std::memory_order_release);
   });
 }
 
-static void sv_finish_solo(g5_server* srv, Engine& e) {
-  e.solo_thread.join();
-  e.solo_state.store(0, std::memory_order_relaxed);
-  G5_CHECK(e.solo_err.empty(), e.solo_err);
-  Request& q = srv->req[e.solo_req];
+static void sv_finish_solo(g5_server* srv, Engine& e, SoloSlot& sl) {
+  sl.th.join();
+  sl.state.store(0, std::memory_order_relaxed);
+  G5_CHECK(sl.err.empty(), sl.err);
+  Request& q = srv->req[sl.req];
   q.status = SV_DONE;
-  q.done = e.solo_done;
-  q.edges = e.solo_edges;
-  q.levels = e.solo_levels;
+  q.done = sl.done;
+  q.edges = sl.edges;
+  q.levels = sl.levels;
   ++srv->done_count;
-  if (e.solo_req >= srv->run_base) ++srv->run_done;
+  if (sl.req >= srv->run_base) ++srv->run_done;
   ++e.st.completed;
   ++e.st.solo;
-  e.solo_req = -1;
+  sl.req = -1;
+}
+
+// An idle graph (no search in lanes) with at most cfg.solo waiting queries
+// answers the oldest on a free single-root slot, as long as the queue stays
+// that short; a longer queue is batched into lanes once the slots are done.
+static bool sv_start_solos(g5_server* srv, Engine& e, double t) {
+  if (!srv->cfg.solo || e.active) return false;
+  bool started = false;
+  for (auto& sl : e.solo) {
+    if (e.waiting.empty() || e.waiting.size() > (size_t)srv->cfg.solo) break;
+    if (srv->req[e.waiting.front()].keep && e.free_bufs.empty()) break;
+    if (sl->state.load(std::memory_order_acquire) != 0) continue;
+    sv_start_solo(srv, e, *sl, t);
+    started = true;
+  }
+  return started;
 }
 
 // One non-blocking step of an engine; returns true if it made progress.
 static bool sv_step(g5_server* srv, Engine& e, double t) {
-  const int solo = e.solo_state.load(std::memory_order_acquire);
-  if (solo == 1) return false;  // the single-root run owns the graph's stream
-  if (solo == 2) {
-    sv_finish_solo(srv, e);
-    return true;
+  bool progress = false, running = false;
+  for (auto& sl : e.solo) {
+    const int st = sl->state.load(std::memory_order_acquire);
+    if (st == 2) {
+      sv_finish_solo(srv, e, *sl);
+      progress = true;
+    } else if (st == 1) {
+      running = true;
+    }
   }
+  // while single-root runs go on, more may start but no query takes a lane
+  if (running) return sv_start_solos(srv, e, t) || progress;
   if (e.busy) {
     if (!e.pub.ready(0, kSvPub, e.pub_seq)) {
       // not published yet; a stream that has finished without publishing
@@ -810,16 +880,13 @@ static bool sv_step(g5_server* srv, Engine& e, double t) {
     G5_HIP(hipGetLastError());
     e.cooling = 0;
   }
-  if (srv->cfg.solo && !e.active && e.waiting.size() == 1 && (!srv->req[e.waiting.front()].keep || !e.free_bufs.empty())) {
-    sv_start_solo(srv, e, t);
-    return true;
-  }
+  if (sv_start_solos(srv, e, t)) return true;
   sv_admit(srv, e, t);
   if (e.active) {
     sv_launch_level(srv, e);
     return true;
   }
-  return false;
+  return progress;
 }
 
 static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeout_s, g5_query_result* res,
@@ -833,7 +900,8 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
   // a single-root run left by an earlier run that ended in an error finishes
   // first, so that it does not count as one of this run's answers
   for (auto& e : srv->eng)
-    if (e->solo_state.load(std::memory_order_acquire) != 0) sv_finish_solo(srv, *e);
+    for (auto& sl : e->solo)
+      if (sl->state.load(std::memory_order_acquire) != 0) sv_finish_solo(srv, *e, *sl);
   const int64_t base = (int64_t)srv->req.size();
   srv->req.resize(base + nq);
   for (int64_t i = 0; i < nq; ++i) {
@@ -874,7 +942,7 @@ static void sv_run(g5_server* srv, const g5_query* qs, int64_t nq, double timeou
     done = srv->run_done;
     if (!progress) {
       bool any_busy = false;
-      for (auto& e : srv->eng) any_busy |= e->busy || e->solo_state.load(std::memory_order_relaxed) != 0;
+      for (auto& e : srv->eng) any_busy |= e->busy || e->solo_running();
       if (!any_busy && next_arrival < nq) {
         const double wait = srv->req[base + next_arrival].arrival - (now_s() - t0);
         if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
@@ -955,7 +1023,7 @@ int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_conf
   if (cfg->answer_buffers < 1 || cfg->answer_buffers > 4096 || !(cfg->alpha > 0) ||
       (cfg->pull_filter != 0 && cfg->pull_filter != 1) || cfg->min_admit < 1 || cfg->min_admit > 64 ||
       cfg->lane_rule < 0 || cfg->lane_rule > 3 ||
-      !(cfg->max_wait_s >= 0) || cfg->solo < 0 || cfg->solo > 1 || cfg->reserved != 0)
+      !(cfg->max_wait_s >= 0) || cfg->solo < 0 || cfg->solo > 64 || cfg->solo_slots < 0 || cfg->solo_slots > 8)
     return G5_ERR_ARG;
   for (int i = 0; i < num_graphs; ++i) {
     if (!graphs[i] || graphs[i]->device != graphs[0]->device) return G5_ERR_ARG;
@@ -970,8 +1038,7 @@ int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_conf
     for (int i = 0; i < num_graphs; ++i) {
       srv->eng.emplace_back(new Engine());
       sv_setup_engine(srv, *srv->eng.back(), graphs[i]);
-      // the single-root path needs the context's BFS state
-      if (cfg->solo && !graphs[i]->bfs.prepared) bfs_prepare(graphs[i]);
+      if (cfg->solo) sv_setup_solo(*srv->eng.back(), std::max(cfg->solo_slots, 1));
     }
   });
   if (rc != G5_OK) {
@@ -989,7 +1056,8 @@ void g5_server_destroy(g5_server* srv) {
   if (!srv) return;
   (void)hipSetDevice(srv->device);
   for (auto& e : srv->eng) {
-    if (e->solo_thread.joinable()) e->solo_thread.join();  // a single-root run still going
+    for (auto& sl : e->solo)
+      if (sl->th.joinable()) sl->th.join();  // a single-root run still going
     (void)hipStreamSynchronize(e->s);
     --e->ctx->servers;
   }

@@ -102,7 +102,7 @@ class BatchStats(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("answer_buffers", ctypes.c_int32), ("pull_filter", ctypes.c_int32), ("alpha", ctypes.c_double),
                 ("min_admit", ctypes.c_int32), ("lane_rule", ctypes.c_int32), ("max_wait_s", ctypes.c_double),
-                ("solo", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("solo", ctypes.c_int32), ("solo_slots", ctypes.c_int32)]
 
 
 class Query(ctypes.Structure):
@@ -540,19 +540,22 @@ class G5Server:
     batching of (graph, root) BFS queries, up to 64 in flight per graph."""
 
     def __init__(self, contexts, answer_buffers=128, alpha=64.0, pull_filter=0, min_admit=64, max_wait_s=0.3,
-                 lane_rule=3, solo=1):
+                 lane_rule=3, solo=1, solo_slots=1):
         """Defaults: a graph with searches in flight admits waiting queries
         into its free lanes at a level boundary while every search in flight
         still pushes (newcomers join a young cohort), once 64 of them can
         start together, or once the oldest has waited 0.3 s; each search
         pulls while its frontier's edges exceed 1/64 of the edges it has not
-        reached, and pushes otherwise.  solo 1: a graph with no search in
-        flight and exactly one waiting query answers it with the single-root
-        kernels (the contexts are prepared for g5_run_bfs if they are not)."""
+        reached, and pushes otherwise.  solo k (1..64): a graph with no search
+        in flight and at most k waiting queries answers the oldest with the
+        single-root kernels, then looks again (the contexts are prepared for
+        g5_run_bfs if they are not); 0: every query takes a lane.
+        solo_slots: single-root runs of one graph that may go on at once
+        (each slot beyond the first holds a BFS state of its own)."""
         self._lib = load_library()
         self._ctxs = list(contexts)  # the server uses their graphs and streams
         arr = (ctypes.c_void_p * len(self._ctxs))(*[c._h for c in self._ctxs])
-        cfg = ServerConfig(answer_buffers, pull_filter, alpha, min_admit, lane_rule, max_wait_s, solo, 0)
+        cfg = ServerConfig(answer_buffers, pull_filter, alpha, min_admit, lane_rule, max_wait_s, solo, solo_slots)
         h = ctypes.c_void_p()
         rc = self._lib.g5_server_create(arr, len(self._ctxs), ctypes.byref(cfg), ctypes.byref(h))
         if rc != G5_OK:

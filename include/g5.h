@@ -346,8 +346,8 @@ int g5_batch_end(g5_ctx* ctx);
  * at whatever level they have reached.  A query holds one answer array
  * (num_rows bytes of levels) from a pool of answer_buffers per graph; a query
  * waits while its graph has no free lane or no free answer array.  With
- * config solo, a query that finds its graph idle and alone is answered by
- * the single-root kernels instead (see g5_server_config.solo).  The graph
+ * config solo, a query that finds its graph idle and few queries waiting is
+ * answered by the single-root kernels instead (see g5_server_config.solo).  The graph
  * contexts must not be used for anything else while a server runs. */
 typedef struct g5_server g5_server;
 
@@ -369,12 +369,14 @@ typedef struct g5_server_config {
                               lane_rule 1) */
   double max_wait_s;       /* ... or once its oldest waiting query has waited this long.  An idle
                               graph admits at once. */
-  int32_t solo;            /* 1: a graph with no search in flight and exactly one waiting query
-                              answers it with the single-root kernels of g5_run_bfs (a host thread
-                              of the server drives them, so the other graphs go on); queries that
-                              arrive meanwhile wait and are batched after it.  0: every query takes
-                              a lane. */
-  int32_t reserved;        /* must be 0 */
+  int32_t solo;            /* 1..64: a graph with no search in flight and at most solo waiting
+                              queries answers the oldest with the single-root kernels of
+                              g5_run_bfs (a host thread of the server drives them, so the other
+                              graphs go on), then looks again; a longer queue is batched into
+                              lanes.  0: every query takes a lane. */
+  int32_t solo_slots;      /* single-root runs of one graph that may go on at once (0 or 1: one;
+                              at most 8); each slot beyond the first holds a BFS state of its own
+                              (as g5_prepare_bfs allocates) and a stream */
 } g5_server_config;
 
 typedef struct g5_query {

@@ -219,6 +219,28 @@ def test_lone_queries_single_root_path(graphs, solo):
         _check_kept(srv, res, specs, ctxs, ocs, range(0, 12, 2))
 
 
+@pytest.mark.parametrize("solo,slots", [(4, 1), (2, 1), (4, 3), (4, 2)])
+def test_short_queue_single_root_path(graphs, solo, slots):
+    """Three queries arrive at once on each of two idle graphs: with solo 4
+    (a queue of three is short) all six are answered by the single-root
+    kernels, one after another with one slot per graph or overlapping with
+    two or three; with solo 2 each graph batches its three into lanes.  The
+    answers are the oracle's either way."""
+    import graph500_amd
+    ctxs, ocs = graphs
+    rng = np.random.default_rng(37)
+    specs = [(g, r, 0.0, 1) for g in (1, 2) for r in _roots_with_edges(ctxs[g], 3, rng)]
+    with graph500_amd.G5Server(ctxs, solo=solo, solo_slots=slots) as srv:
+        say("serve short queues, solo", solo, "slots", slots)
+        res, rep = srv.run(_queries(specs), timeout_s=60)
+        assert (res["status"] == 2).all()
+        n_solo = sum(srv.graph_stats(g)["solo"] for g in range(3))
+        assert n_solo == (6 if solo == 4 else 0), n_solo
+        for i, (g, r, _, _) in enumerate(specs):
+            assert res["edges"][i] == _oracle(ocs[g], r)[1], (i, g, r)
+        _check_kept(srv, res, specs, ctxs, ocs, range(len(specs)))
+
+
 def test_run_after_a_timed_out_run(graphs):
     """A run that times out (possibly with a single-root run still going)
     leaves the server usable: the next run answers exactly its own queries."""
@@ -378,7 +400,9 @@ def test_serve_errors(graphs):
         with pytest.raises(graph500_amd.G5Error):
             graph500_amd.G5Server([c])  # reorder mode 0 graphs are not served
     with pytest.raises(graph500_amd.G5Error):
-        graph500_amd.G5Server(ctxs[:1], solo=2)  # solo is 0 or 1
+        graph500_amd.G5Server(ctxs[:1], solo=65)  # solo is 0..64
+    with pytest.raises(graph500_amd.G5Error):
+        graph500_amd.G5Server(ctxs[:1], solo_slots=9)  # at most 8 slots
 
 
 def test_served_context_refuses_rebuild_and_destroy():
