@@ -47,6 +47,10 @@ Other modes:
                           rocprofv3 results database (the text kept in profiles/)
   --rocprof-dispatches DB KERNEL
                           per-dispatch durations or counters of one kernel
+  --rocprof-overlap DB KERNEL
+                          how kernels overlap on the device from the first
+                          dispatch of KERNEL on (busy time, time with 2+ and 3+
+                          kernels at once, dispatches per stream)
   --tuning JSON           run the batch with this tuning instead of autotuning
   --single-root-only N    only the single-root leg, over the first N roots (a
                           short run to profile under rocprofv3)
@@ -248,6 +252,42 @@ def rocprof_dispatches(db, pattern):
     return "\n".join(lines)
 
 
+def rocprof_overlap(db, pattern):
+    """How the kernels of a --kernel-trace database overlap on the device,
+    from the first dispatch whose name contains `pattern` to the last
+    dispatch: busy time (the union of kernel intervals), the time with two
+    or more and with three or more kernels running at once, the largest
+    number at once, and the dispatches per stream."""
+    import collections
+    import sqlite3
+    cur = sqlite3.connect(db).cursor()
+    rows = list(cur.execute("select name, start, end, stream_id from kernels order by start"))
+    first = next((i for i, r in enumerate(rows) if pattern in r[0]), None)
+    if first is None:
+        return "no dispatch matches %r" % pattern
+    rows = rows[first:]
+    ev = sorted([(r[1], 1) for r in rows] + [(r[2], -1) for r in rows], key=lambda x: (x[0], x[1]))
+    depth, last, top = 0, ev[0][0], 0
+    t = collections.Counter()
+    for when, d in ev:
+        if depth > 0:
+            t[min(depth, 3)] += when - last
+        depth += d
+        top = max(top, depth)
+        last = when
+    busy = sum(t.values())
+    window = rows[-1][2] - rows[0][1] if rows else 0
+    per_stream = collections.Counter(r[3] for r in rows)
+    lines = ["window from the first %s dispatch: %.1f ms, %d dispatches on %d streams" %
+             (pattern, window / 1e6, len(rows), len(per_stream)),
+             "device busy (some kernel running): %.1f ms (%.1f %% of the window)" % (busy / 1e6, 100.0 * busy / max(window, 1)),
+             "two or more kernels at once: %.1f ms (%.1f %% of busy)" % ((t[2] + t[3]) / 1e6, 100.0 * (t[2] + t[3]) / max(busy, 1)),
+             "three or more kernels at once: %.1f ms (%.1f %% of busy)" % (t[3] / 1e6, 100.0 * t[3] / max(busy, 1)),
+             "most kernels at once: %d" % top,
+             "dispatches per stream: %s" % ", ".join("%d" % c for _, c in per_stream.most_common())]
+    return "\n".join(lines)
+
+
 def traffic_from_profile(kernel, scale):
     """HBM bytes per launch of `kernel` from the committed rocprof PMC summary
     (profiles/<kernel>_s<scale>_traffic.txt, line 'traffic_bytes_per_launch: X')."""
@@ -398,8 +438,10 @@ def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):
             check = rng.choice(nq, size=min(4, nq), replace=False)
             q["keep"][check] = 1
             solo0 = sum(srv.graph_stats(g)["solo"] for g in range(len(scales)))
+            over0 = sum(srv.graph_stats(g)["solo_overlap"] for g in range(len(scales)))
             res, rep = srv.run(q, timeout_s=300)
             solo = sum(srv.graph_stats(g)["solo"] for g in range(len(scales))) - solo0
+            over = sum(srv.graph_stats(g)["solo_overlap"] for g in range(len(scales))) - over0
             lat = (res["done_s"] - q["arrival_s"]) * 1e3
             ok = True
             for i in check:
@@ -420,6 +462,7 @@ def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):
                    "gteps": round(rep["edges"] / rep["wall_s"] / 1e9, 2),
                    "max_lanes": rep["max_active_lanes"], "admitted_busy": rep["admitted_busy"],
                    "admit_events": rep["admit_events"], "single_root_answers": int(solo),
+                   "single_root_overlapped": int(over),
                    "sample_bit_exact": ok}
             log("serve %s" % run)
             runs.append(run)
@@ -429,7 +472,9 @@ def serve_leg(scales, rates, max_queries, edgefactor, policy=None, seed=1):
     return {"graphs_scale": list(scales), "policy": policy, "stated_load_qps": STATED_LOAD_QPS,
             "p50_ms_at_stated_load": stated["p50_ms"] if stated else None,
             "p99_ms_at_stated_load": stated["p99_ms"] if stated else None,
-            "admitted_busy_at_stated_load": stated["admitted_busy"] if stated else None, "runs": runs}
+            "admitted_busy_at_stated_load": stated["admitted_busy"] if stated else None,
+            "single_root_overlapped_at_stated_load": stated["single_root_overlapped"] if stated else None,
+            "runs": runs}
 
 
 def main():
@@ -459,6 +504,7 @@ def main():
                     help="array sizes of --gather-calibration, bytes, comma-separated")
     ap.add_argument("--rocprof-summary", metavar="DB")
     ap.add_argument("--rocprof-dispatches", nargs=2, metavar=("DB", "KERNEL"))
+    ap.add_argument("--rocprof-overlap", nargs=2, metavar=("DB", "KERNEL"))
     ap.add_argument("--single-root-only", type=int, metavar="N")
     ap.add_argument("--tuning", type=json.loads, default=None,
                     help="batch tuning as JSON (skips the autotuner), e.g. a previous run's batch.tuning")
@@ -469,6 +515,9 @@ def main():
         return 0
     if a.rocprof_dispatches:
         print(rocprof_dispatches(*a.rocprof_dispatches))
+        return 0
+    if a.rocprof_overlap:
+        print(rocprof_overlap(*a.rocprof_overlap))
         return 0
     if a.copy_calibration:
         copy_calibration()

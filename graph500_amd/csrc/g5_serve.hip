@@ -838,6 +838,7 @@ static bool sv_start_solos(g5_server* srv, Engine& e, double t) {
     if (e.waiting.empty() || e.waiting.size() > (size_t)srv->cfg.solo) break;
     if (srv->req[e.waiting.front()].keep && e.free_bufs.empty()) break;
     if (sl->state.load(std::memory_order_acquire) != 0) continue;
+    if (e.solo_running()) ++e.st.solo_overlap;
     sv_start_solo(srv, e, *sl, t);
     started = true;
   }

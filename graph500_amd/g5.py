@@ -119,7 +119,7 @@ class ServerGraphStats(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int64), ("push_levels", ctypes.c_int64), ("pull_levels", ctypes.c_int64),
                 ("mixed_levels", ctypes.c_int64), ("lane_levels", ctypes.c_int64), ("busy_ms", ctypes.c_double),
                 ("completed", ctypes.c_int64), ("admitted_busy", ctypes.c_int64), ("admitted_idle", ctypes.c_int64),
-                ("solo", ctypes.c_int64)]
+                ("solo", ctypes.c_int64), ("solo_overlap", ctypes.c_int64)]
 
 
 class ServerReport(ctypes.Structure):

@@ -418,6 +418,8 @@ typedef struct g5_server_graph_stats {
   int64_t admitted_busy;  /* queries admitted while other searches of this graph were in flight */
   int64_t admitted_idle;  /* queries admitted into an idle graph */
   int64_t solo;           /* queries answered by the single-root kernels (g5_server_config.solo) */
+  int64_t solo_overlap;   /* of those, runs started while another single-root run of this graph
+                             was going on (g5_server_config.solo_slots) */
 } g5_server_graph_stats;
 
 int g5_server_create(g5_ctx* const* graphs, int num_graphs, const g5_server_config* config, g5_server** out);

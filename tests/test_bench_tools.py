@@ -92,3 +92,40 @@ def test_rocprof_summary_both_kinds(tmp_path):
     s2 = bench.rocprof_summary(db2)
     line = [l for l in s2.splitlines() if "k_ms_pull_lanes" in l][0]
     assert line.split()[-2] == "2" and float(line.split()[-1]) == pytest.approx(200.0)
+
+
+def test_rocprof_overlap_by_brute_force(tmp_path):
+    """Busy time and the time with 2+ / 3+ kernels at once equal a count over
+    every nanosecond of a small random trace."""
+    import numpy as np
+    rng = np.random.default_rng(3)
+    unit = 10_000  # ns: the trace spans about 24 ms
+    rows = []
+    for i in range(40):
+        s = int(rng.integers(0, 2000))
+        rows.append(("g5::k_%s(int)" % ("sv_pull" if i else "build"), s, s + int(rng.integers(1, 300)),
+                     int(rng.integers(0, 5))))
+    rows.sort(key=lambda r: r[1])
+    rows[0] = ("g5::k_build(int)",) + rows[0][1:]
+    rows[1:] = [("g5::k_sv_pull(int)",) + r[1:] for r in rows[1:]]
+    scaled = [(n, s * unit, e * unit, st) for n, s, e, st in rows]
+    db = str(tmp_path / "trace.db")
+    con = sqlite3.connect(db)
+    con.execute("create table kernels (name text, start integer, end integer, stream_id integer)")
+    con.executemany("insert into kernels values (?, ?, ?, ?)", scaled)
+    con.commit()
+    con.close()
+    cover = np.zeros(2400, np.int64)
+    for _, s, e, _ in rows[1:]:  # from the first k_sv_ dispatch on
+        cover[s:e] += 1
+    ms = unit / 1e6
+    out = bench.rocprof_overlap(db, "k_sv_").splitlines()
+    busy = float(out[1].split(": ")[1].split()[0])
+    two = float(out[2].split(": ")[1].split()[0])
+    three = float(out[3].split(": ")[1].split()[0])
+    assert (cover > 2).sum() > 0  # the trace has 3-deep overlaps
+    assert busy == pytest.approx((cover > 0).sum() * ms, abs=0.06)
+    assert two == pytest.approx((cover > 1).sum() * ms, abs=0.06)
+    assert three == pytest.approx((cover > 2).sum() * ms, abs=0.06)
+    assert int(out[4].split(": ")[1]) == cover.max()
+    assert bench.rocprof_overlap(db, "k_nothing").startswith("no dispatch")

@@ -236,6 +236,16 @@ def test_short_queue_single_root_path(graphs, solo, slots):
         assert (res["status"] == 2).all()
         n_solo = sum(srv.graph_stats(g)["solo"] for g in range(3))
         assert n_solo == (6 if solo == 4 else 0), n_solo
+        # runs started while another of the same graph was going on: the
+        # second and third of each graph with three slots, at least the second
+        # with two, none with one
+        over = [srv.graph_stats(g)["solo_overlap"] for g in (1, 2)]
+        if solo == 4 and slots == 3:
+            assert over == [2, 2], over
+        elif solo == 4 and slots == 2:
+            assert all(o >= 1 for o in over), over
+        else:
+            assert over == [0, 0], over
         for i, (g, r, _, _) in enumerate(specs):
             assert res["edges"][i] == _oracle(ocs[g], r)[1], (i, g, r)
         _check_kept(srv, res, specs, ctxs, ocs, range(len(specs)))
