@@ -174,8 +174,9 @@ __device__ __forceinline__ void ms_finish_body(uint64_t* __restrict__ seen, uint
     if (inb) {
       const uint64_t s0 = seen[v];
       const uint64_t s = s0 & keep;
-      nw = next[v] & ~s;
-      next[v] = nw;
+      const uint64_t x = next[v];
+      nw = x & ~s;
+      if (nw != x) next[v] = nw;  // most words are already right (zero or all new)
       if (nw || s != s0) seen[v] = s | nw;
       if (nw) {
         live |= nw;
@@ -215,8 +216,9 @@ __device__ __forceinline__ void ms_finish_list_body(uint64_t* __restrict__ seen,
     uint64_t nw = 0;
     if (inb) {
       const uint64_t s0 = seen[v];
-      nw = next[v] & ~s0;
-      next[v] = nw;
+      const uint64_t x = next[v];
+      nw = x & ~s0;
+      if (nw != x) next[v] = nw;
       if (nw) {
         seen[v] = s0 | nw;
         live |= nw;
@@ -363,7 +365,7 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
                                              int64_t rows, uint64_t live_in, unsigned long long* __restrict__ c,
                                              unsigned bid, unsigned nblk, uint64_t keep = ~UINT64_C(0),
                                              const Epi& epi = Epi(), bool merge = false,
-                                             uint64_t* __restrict__ miss = nullptr) {
+                                             uint64_t* __restrict__ miss = nullptr, bool zeroed = false) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
   uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
@@ -376,7 +378,8 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
     const uint64_t s = inb ? (s0 & keep) : s0;
     const uint64_t need = ~s & live_in;
     // lanes pushed into next earlier in this level (merge): new ones join nw
-    const uint64_t pushed = (merge && inb) ? next[v] & ~s : 0;
+    const uint64_t old = (merge && inb) ? next[v] : 0;
+    const uint64_t pushed = old & ~s;
     const uint64_t acc = ms_pull_probe<kU, kW>(fv, v, need, first_nbr, offsets, cols);
     const uint64_t nw = (acc & need) | pushed;
     epi(v, inb, nw);
@@ -388,7 +391,9 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
     const uint64_t b = __ballot(nw != 0);
     if (lane == 0) nz_next[wd] = b;
     if (inb) {
-      next[v] = nw;
+      // a next array cleared before the level (zeroed; merge implies it) holds
+      // zeros or pushed lanes: only words that change are written
+      if (!(zeroed || merge) || nw != old) next[v] = nw;
       if (nw || s != s0) seen[v] = s | nw;
       if (nw) {
         live |= nw;

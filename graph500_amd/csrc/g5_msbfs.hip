@@ -173,10 +173,11 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
                                                     const uint32_t* __restrict__ cols, int64_t rows,
                                                     uint64_t live_in, int mode, const uint64_t* __restrict__ pack,
                                                     const uint64_t* __restrict__ compact, uint32_t filter_from,
-                                                    unsigned long long* __restrict__ c, uint64_t* __restrict__ miss) {
+                                                    unsigned long long* __restrict__ c, uint64_t* __restrict__ miss,
+                                                    int zeroed) {
   const FrontView fv{front, nz_front, pack, compact, mode, filter_from};
   ms_pull_body<NoEpilogue, kU, kW>(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x,
-                                   gridDim.x, ~UINT64_C(0), NoEpilogue(), false, miss);
+                                   gridDim.x, ~UINT64_C(0), NoEpilogue(), false, miss, zeroed != 0);
 }
 
 // The pull of a tail level over the listed vertices (see ms_pull_list_body).
@@ -861,9 +862,14 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       } else {
         const bool mark = unreached * kListBelow < (double)rows;
         if (mark) m.miss.alloc(words);
+        // a sparse old content of the level's frontier array is cleared word
+        // by word first, and the pull then writes only its new words;
+        // otherwise the pull writes every word
+        const bool zeroed = marked && m.fr_count[l] * 16 < rows;
+        if (zeroed) ms_clear_frontier(m, l, rows, s, true);
         hipLaunchKernelGGL(kp, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s, m.seen.p, front,
                            nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, live, mode,
-                           m.pack.p, m.compact.p, filter_from, cnt, mark ? m.miss.p : nullptr);
+                           m.pack.p, m.compact.p, filter_from, cnt, mark ? m.miss.p : nullptr, zeroed ? 1 : 0);
         list_level = mark;
       }
     }
