@@ -40,7 +40,7 @@ SPACE = [
     ("pull_filter", [0, 1, 2]),
     ("filter_from", [0.001, 0.01, 0.05]),
     ("compact_below", [0.0, 0.25, 0.5]),
-    ("pull_grid", [4096, 8192, 16384]),
+    ("pull_grid", [4096, 8192, 16384, 32768, 65536]),
     ("persist_waves", [4, 6, 8]),
 ]
 # a second pass over the knobs that depend most on each other (the probe

@@ -365,7 +365,7 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
                                              int64_t rows, uint64_t live_in, unsigned long long* __restrict__ c,
                                              unsigned bid, unsigned nblk, uint64_t keep = ~UINT64_C(0),
                                              const Epi& epi = Epi(), bool merge = false,
-                                             uint64_t* __restrict__ miss = nullptr, bool zeroed = false) {
+                                             uint64_t* __restrict__ miss = nullptr) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)nblk * (kBlock / 64);
   uint64_t live = 0, pairs = 0, nz = 0, fdeg = 0;
@@ -391,9 +391,9 @@ __device__ __forceinline__ void ms_pull_body(uint64_t* __restrict__ seen, const 
     const uint64_t b = __ballot(nw != 0);
     if (lane == 0) nz_next[wd] = b;
     if (inb) {
-      // a next array cleared before the level (zeroed; merge implies it) holds
-      // zeros or pushed lanes: only words that change are written
-      if (!(zeroed || merge) || nw != old) next[v] = nw;
+      // with merge the array was cleared before the push and holds zeros or
+      // pushed lanes: only words that change are written
+      if (!merge || nw != old) next[v] = nw;
       if (nw || s != s0) seen[v] = s | nw;
       if (nw) {
         live |= nw;

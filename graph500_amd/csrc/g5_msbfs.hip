@@ -173,11 +173,10 @@ __global__ void __launch_bounds__(kBlock) k_ms_pull(uint64_t* __restrict__ seen,
                                                     const uint32_t* __restrict__ cols, int64_t rows,
                                                     uint64_t live_in, int mode, const uint64_t* __restrict__ pack,
                                                     const uint64_t* __restrict__ compact, uint32_t filter_from,
-                                                    unsigned long long* __restrict__ c, uint64_t* __restrict__ miss,
-                                                    int zeroed) {
+                                                    unsigned long long* __restrict__ c, uint64_t* __restrict__ miss) {
   const FrontView fv{front, nz_front, pack, compact, mode, filter_from};
   ms_pull_body<NoEpilogue, kU, kW>(seen, fv, next, nz_next, first_nbr, offsets, cols, rows, live_in, c, blockIdx.x,
-                                   gridDim.x, ~UINT64_C(0), NoEpilogue(), false, miss, zeroed != 0);
+                                   gridDim.x, ~UINT64_C(0), NoEpilogue(), false, miss);
 }
 
 // The pull of a tail level over the listed vertices (see ms_pull_list_body).
@@ -280,6 +279,15 @@ constexpr double kListBelow = 16.0;
 // A push level lists the vertices it touches when its frontier edges times
 // this are fewer than the rows (the touched vertices are at most the edges).
 constexpr double kTouchBelow = 8.0;
+
+// Pull levels with fewer than 1/kLightBelow of the (root, vertex) pairs
+// unreached run on at most kLightGrid workgroups: their pass over every
+// vertex finds little to probe, and at SCALE 28 it took 0.48 ms on 2,048
+// workgroups against 0.86 ms on 16,384 and 1.59 ms on 32,768, while the
+// mixed level gains from the larger grids (26.2 ms on 2,048, 24.6 ms on
+// 32,768).
+constexpr double kLightBelow = 8.0;
+constexpr int64_t kLightGrid = 2048;
 
 constexpr unsigned kBarGroup = 16;
 constexpr unsigned kBarMaxGroups = 1024;
@@ -846,6 +854,8 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
       // next pull takes only those, listed by a scan of the bits (after
       // clearing its frontier array, which must hold zeros elsewhere).
       const double unreached = std::max(0.0, total_pairs - reached);
+      const int64_t grid_cap =
+          unreached * kLightBelow < total_pairs ? std::min<int64_t>(m.pull_grid, kLightGrid) : m.pull_grid;
       if (have_miss && mode == kFrontPlain) {
         const uint64_t ub = (uint64_t)std::min<double>((double)rows, unreached);  // list length bound
         m.ulist.alloc(std::max<uint64_t>(ub, 1));
@@ -855,21 +865,19 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
         ms_clear_frontier(m, l, rows, s, marked);
         G5_HIP(hipMemsetAsync(nz_next, 0, words * 8, s));
         G5_HIP(hipMemsetAsync(m.miss.p, 0, words * 8, s));
-        hipLaunchKernelGGL(k_ms_pull_list, dim3(grid_of((int64_t)ub, kBlock, m.pull_grid)), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(k_ms_pull_list, dim3(grid_of((int64_t)ub, kBlock, grid_cap)), dim3(kBlock), 0, s,
                            m.seen.p, front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, m.ulist.p,
                            m.ulist_count.p, live, m.miss.p, cnt);
         list_level = true;
       } else {
         const bool mark = unreached * kListBelow < (double)rows;
         if (mark) m.miss.alloc(words);
-        // a sparse old content of the level's frontier array is cleared word
-        // by word first, and the pull then writes only its new words;
-        // otherwise the pull writes every word
-        const bool zeroed = marked && m.fr_count[l] * 16 < rows;
-        if (zeroed) ms_clear_frontier(m, l, rows, s, true);
-        hipLaunchKernelGGL(kp, dim3(grid_of(words, kBlock / 64, m.pull_grid)), dim3(kBlock), 0, s, m.seen.p, front,
+        // the pull writes every word of the level's frontier array (clearing
+        // its old words first, one store per marked vertex, measured slower
+        // than the coalesced writes even at a few million marks)
+        hipLaunchKernelGGL(kp, dim3(grid_of(words, kBlock / 64, grid_cap)), dim3(kBlock), 0, s, m.seen.p, front,
                            nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, live, mode,
-                           m.pack.p, m.compact.p, filter_from, cnt, mark ? m.miss.p : nullptr, zeroed ? 1 : 0);
+                           m.pack.p, m.compact.p, filter_from, cnt, mark ? m.miss.p : nullptr);
         list_level = mark;
       }
     }

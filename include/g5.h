@@ -100,7 +100,8 @@ typedef struct g5_batch_stats {
  * pull_filter 0, beta 50, filter_from 0.01, pull_unroll 0, compact_below 0, mixed_push 1). */
 typedef struct g5_batch_tuning {
   double alpha;        /* push while the frontier union has fewer than nnz/alpha edges */
-  int64_t pull_grid;   /* workgroups of the vertex-parallel pull (256 threads each) */
+  int64_t pull_grid;   /* workgroups of the vertex-parallel pull (256 threads each); pull levels with
+                          fewer than 1/8 of the (root, vertex) pairs unreached use at most 2048 */
   double sweep_below;  /* (lane_rule 0) pull levels use the edge-parallel sweep while the (root, vertex)
                           pairs reached so far are below sweep_below x roots x rows
                           (reorder modes 1 and 2; 0 disables the sweep) */
