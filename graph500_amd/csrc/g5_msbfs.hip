@@ -751,7 +751,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
         if (nchunks) {
           hipLaunchKernelGGL(k_chunk_rows<kEpb>, dim3(grid_of((int64_t)(2 * nchunks), kBlock, 65536)), dim3(kBlock),
                              0, s, m.prefix.p, nf, nchunks, m.chunk_rows.p);
-          hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)E_ub, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p, nf,
+          hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)E_ub, kEpb, 8192)), dim3(kBlock), 0, s, m.queue.p, nf,
                              m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, next, td);
         }
       } else {
@@ -805,7 +805,7 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
                            m.seen.p, next, nz_next, g.offsets.p, m.touch.p, (const unsigned long long*)touch_count,
                            cnt, lepi, lane_stats ? 1 : 0);
       } else {
-        hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)fdeg, kEpb, 2048)), dim3(kBlock), 0, s, m.queue.p,
+        hipLaunchKernelGGL(k_ms_push, dim3(grid_of((int64_t)fdeg, kEpb, 8192)), dim3(kBlock), 0, s, m.queue.p,
                            nf, m.prefix.p, m.chunk_rows.p, g.offsets.p, g.cols.p, front, next, ~UINT64_C(0));
       }
       G5_HIP(hipGetLastError());
