@@ -873,8 +873,8 @@ static void ms_host_levels(g5_ctx* ctx, int l0, uint64_t live, uint64_t nz, uint
         const bool mark = unreached * kListBelow < (double)rows;
         if (mark) m.miss.alloc(words);
         // the pull writes every word of the level's frontier array (clearing
-        // its old words first, one store per marked vertex, measured slower
-        // than the coalesced writes even at a few million marks)
+        // its old words first, one store per marked vertex, so that the pull
+        // could skip its zero words, measured no faster)
         hipLaunchKernelGGL(kp, dim3(grid_of(words, kBlock / 64, grid_cap)), dim3(kBlock), 0, s, m.seen.p, front,
                            nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, rows, live, mode,
                            m.pack.p, m.compact.p, filter_from, cnt, mark ? m.miss.p : nullptr);
