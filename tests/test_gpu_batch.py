@@ -261,14 +261,17 @@ def test_batch_matches_single_root_gpu(scale, sweep, rule, beta):
             assert c.teps_edges() == teps[i]
 
 
-@pytest.mark.parametrize("unroll,alpha,mixed_push", [(0, 256.0, 0), (4, 256.0, 0), (4, 256.0, 1), (0, 32.0, 1),
-                                                      (4, 48.0, 1)])
+@pytest.mark.parametrize("unroll,alpha,mixed_push,grid", [(0, 256.0, 0, 8192), (4, 256.0, 0, 8192),
+                                                           (4, 256.0, 1, 8192), (0, 32.0, 1, 8192),
+                                                           (4, 48.0, 1, 8192), (4, 64.0, 1, 32768),
+                                                           (4, 64.0, 1, 65536)])
 @pytest.mark.parametrize("scale", [20, 22])
-def test_batch_tuned_combination_matches_reference(scale, unroll, alpha, mixed_push):
+def test_batch_tuned_combination_matches_reference(scale, unroll, alpha, mixed_push, grid):
     """Combinations the autotuner picks at SCALE 28 (per-root rule, beta 200,
     filter_from 0.001; alpha 256 with the chunk push of mixed levels, alpha
-    32-48 with the queue push; probe unrolling 4 or 0): the fixture roots'
-    level hashes and edge counts, and single-root runs of roots 16-63."""
+    32-64 with the queue push; probe unrolling 4 or 0; pull grids up to
+    65,536 workgroups, light levels capped): the fixture roots' level hashes
+    and edge counts, and single-root runs of roots 16-63."""
     import graph500_amd
     g = gold("ref_s%d_16roots.json" % scale)
     with graph500_amd.G5Context(0) as c:
@@ -277,9 +280,10 @@ def test_batch_tuned_combination_matches_reference(scale, unroll, alpha, mixed_p
         roots = c.find_roots(64)
         c.batch_prepare()
         c.batch_set_tuning(lane_rule=1, alpha=alpha, beta=200.0, filter_from=0.001, pull_unroll=unroll,
-                           pull_grid=8192, sweep_below=0.0, persistent=0, pull_filter=0, compact_below=0.0,
+                           pull_grid=grid, sweep_below=0.0, persistent=0, pull_filter=0, compact_below=0.0,
                            mixed_push=mixed_push)
-        say("batch s%d tuned combination, unroll %d alpha %g mixed_push %d" % (scale, unroll, alpha, mixed_push))
+        say("batch s%d tuned combination, unroll %d alpha %g mixed_push %d grid %d" % (scale, unroll, alpha,
+                                                                                      mixed_push, grid))
         c.run_bfs_batch(roots)
         teps = c.batch_teps_edges()
         for i, pr in enumerate(g["per_root"]):

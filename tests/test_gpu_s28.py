@@ -8,8 +8,8 @@ SCALE 24), so these tests pin the result through what SCALE fixes:
     4,294,927,670 (mpi/utils.hpp:3693), the count the reference tabulates for
     roots in the giant component at edgefactor 16, seeds (2, 3);
   * every lane of a 64-root batch equals the single-root level array of its
-    root, byte for byte, under the default batch tuning and under a tuned
-    combination of knobs, with the same traversed-edge counts;
+    root, byte for byte, under the default batch tuning and under tuned
+    combinations of knobs, with the same traversed-edge counts;
   * the single-root level arrays, schedules and frontier sizes do not move
     under the hub-level knobs (td_head 0/4/16, positional pieces, target
     blocks), for 8 roots.
@@ -37,7 +37,10 @@ QUEUE = dict(TUNED, alpha=48.0, mixed_push=1)
 DEFAULT = {"lane_rule": 1, "alpha": 128.0, "beta": 50.0, "pull_unroll": 0, "filter_from": 0.01,
            "pull_grid": 8192, "sweep_below": 0.0, "persistent": 0, "pull_filter": 0, "compact_below": 0.0,
            "mixed_push": 1}
-TUNINGS = {"tuned": TUNED, "queue": QUEUE, "default": DEFAULT}
+# the pick of the latest full bench: the queue push, alpha 64, 32,768 pull
+# workgroups (the light pull levels capped at 2,048)
+WIDE = dict(QUEUE, alpha=64.0, pull_grid=32768, filter_from=0.01)
+TUNINGS = {"tuned": TUNED, "queue": QUEUE, "default": DEFAULT, "wide": WIDE}
 
 
 def say(*a):
@@ -99,7 +102,7 @@ def test_single_root_all_64_validate_and_count_pf_nedge(s28):
     assert len(s28["single"]) == 64
 
 
-@pytest.mark.parametrize("name", ["tuned", "queue", "default"])
+@pytest.mark.parametrize("name", ["tuned", "queue", "default", "wide"])
 def test_batch_lanes_equal_single_root(s28, name):
     """One 64-root batch; lane i's levels equal root i's single-root levels and
     every lane counts pf_nedge[28] traversed edges."""
