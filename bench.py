@@ -564,7 +564,7 @@ def main():
         ctx.batch_set_tuning(**a.tuning)
     elif not a.no_autotune:
         from graph500_amd import autotune
-        tuned = autotune.autotune(ctx, roots, reps=2, budget_s=a.autotune_budget, use_cache=False, log=log)
+        tuned = autotune.autotune(ctx, roots, reps=3, budget_s=a.autotune_budget, use_cache=False, log=log)
         log("autotune: %s -> %.3f ms (%d trials, %.1f s)" % (tuned["tuning"], tuned["ms"], tuned["trials"],
                                                              tuned["seconds"]))
     for _ in range(a.warmup):

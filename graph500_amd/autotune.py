@@ -84,8 +84,10 @@ def _measure(ctx, roots, reps):
     return best, pairs
 
 
-def autotune(ctx, roots, reps=2, budget_s=60.0, cache=DEFAULT_CACHE, use_cache=True, log=None):
+def autotune(ctx, roots, reps=2, budget_s=60.0, cache=DEFAULT_CACHE, use_cache=True, log=None, min_gain=0.003):
     """Tunes ctx's batched BFS for `roots` (batch_prepare must have run).
+    A trial replaces the best so far only when it is faster by more than the
+    fraction min_gain, so that run-to-run noise does not move the tuning.
     Returns {"tuning": {...}, "ms": best batch ms, "trials": n, "seconds": s,
     "cached": bool}.  Stops early when budget_s is spent."""
     key = graph_key(ctx)
@@ -120,7 +122,7 @@ def autotune(ctx, roots, reps=2, budget_s=60.0, cache=DEFAULT_CACHE, use_cache=T
                 raise RuntimeError("autotune: pairs discovered per level changed under %s" % trial)
             if log:
                 log("autotune %s=%s: %.3f ms (best %.3f)" % (knob, v, ms, best_ms))
-            if ms < best_ms:
+            if ms < best_ms * (1.0 - min_gain):
                 best_ms, cur = ms, trial
         ctx.batch_set_tuning(**cur)
     res = {"tuning": cur, "ms": best_ms, "trials": trials, "seconds": round(time.time() - t0, 2),

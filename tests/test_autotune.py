@@ -11,13 +11,14 @@ class FakeCtx:
     """Batch time is a fixed function of the tuning; optionally one knob
     value corrupts the edge counts."""
 
-    def __init__(self, bad=None, bad_pairs=None):
+    def __init__(self, bad=None, bad_pairs=None, grid_gain=0.5):
         self.t = {"alpha": 16.0, "pull_grid": 8192, "sweep_below": 0.25, "sweep_grid": 4096, "persistent": 0,
                   "persist_levels": 16, "persist_grid": 0, "persist_waves": 8, "pull_filter": 0, "beta": 50.0,
                   "compact_below": 0.0, "filter_from": 0.01, "pull_unroll": 0, "lane_rule": 0,
                   "mixed_push": 1}
         self.bad = bad
         self.bad_pairs = bad_pairs
+        self.grid_gain = grid_gain
         self.runs = 0
 
     def graph_info(self):
@@ -40,7 +41,7 @@ class FakeCtx:
         # the per-root rule wins only with its own alpha scale
         a = abs(t["alpha"] - 128.0) / 64 - 1 if t["lane_rule"] else abs(t["alpha"] - 2.0) / 8
         ms = 10.0 + a + (0 if t["beta"] == 20.0 else 1) + t["pull_filter"] \
-            + (2 if t["persistent"] else 0) + (0 if t["pull_grid"] == 16384 else 0.5)
+            + (2 if t["persistent"] else 0) + (0 if t["pull_grid"] == 16384 else self.grid_gain)
         pairs = [3, 40, 500, 7]
         if self.bad_pairs and self.t.get(self.bad_pairs[0]) == self.bad_pairs[1]:
             pairs = [3, 41, 499, 7]  # a vertex given the wrong level: same reached set
@@ -92,3 +93,13 @@ def test_union_rule_alpha_is_kept_when_it_wins(tmp_path):
             return {"device_ms": st["device_ms"] + (5.0 if self.t["lane_rule"] else 0.0), "pairs": st["pairs"]}
     res = autotune.autotune(UnionWins(), [1], reps=1, cache=str(tmp_path / "t.json"))
     assert res["tuning"]["lane_rule"] == 0 and res["tuning"]["alpha"] == 2.0
+
+
+def test_noise_sized_gains_do_not_move_the_tuning(tmp_path):
+    """A knob value faster by less than min_gain (0.3 %) is not taken; one
+    faster by more is."""
+    small = FakeCtx(grid_gain=0.01)  # 16384 saves 0.01 ms of about 9 ms
+    res = autotune.autotune(small, [1], reps=1, cache=str(tmp_path / "a.json"))
+    assert res["tuning"]["pull_grid"] == 8192
+    res = autotune.autotune(FakeCtx(grid_gain=0.01), [1], reps=1, cache=str(tmp_path / "b.json"), min_gain=0.0)
+    assert res["tuning"]["pull_grid"] == 16384
