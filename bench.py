@@ -28,8 +28,12 @@ or a level where some roots push and the others pull): algorithmic bytes of its 
 GPU from the final levels, outside timing; equal to the CPU oracle's count,
 see tests) over their HIP-event durations on the library stream.  `traffic`
 is the HBM bytes per launch measured by rocprof (FETCH_SIZE + WRITE_SIZE,
-scaled by the same counters on a plain copy) when a summary for this kernel
-and workload is present under profiles/, else null.  `cpu_baseline` times
+scaled as the plain-copy and random-gather calibrations of the same counters
+direct) when a summary for this kernel and workload is present under
+profiles/, else null; `request_frac` is the kernel's memory-request rate
+over the best random-read rate past the L2s in the gather calibration.
+`serve` quotes its policy: short queues on single-root slots, longer ones
+in lanes.  `cpu_baseline` times
 the reference CPU runnable (oracle/_ref/runnable) on a bounded sample on this
 host, or the oracle port when that cannot run.
 
