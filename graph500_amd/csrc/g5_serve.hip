@@ -385,6 +385,7 @@ struct Engine {
   uint64_t pulled = 0;  // lane rule 3: searches that have pulled keep pulling
   bool lane_growing[64];
   int64_t lane_prev_nq[64];
+  int lane_pulls[64];    // pull levels each search has taken
   int64_t lane_visited[64];
   double lane_vdeg[64];  // row lengths of the vertices each search has reached
   uint64_t active = 0, cooling = 0;
@@ -499,6 +500,7 @@ static void sv_setup_engine(g5_server* srv, Engine& e, g5_ctx* ctx) {
   for (int r = 0; r < 64; ++r) {
     e.lane_req[r] = -1;
     e.lane_l0[r] = 0;
+    e.lane_pulls[r] = 0;
     e.teps_req[r] = -1;
   }
   G5_HIP(hipStreamSynchronize(s));
@@ -642,6 +644,7 @@ static uint64_t sv_admit(g5_server* srv, Engine& e, double t) {
     e.lane_fwd[r] = true;
     e.pulled &= ~(UINT64_C(1) << r);
     e.lane_growing[r] = true;
+    e.lane_pulls[r] = 0;
     e.lane_prev_nq[r] = 0;
     e.lane_visited[r] = 1;
     e.lane_vdeg[r] = 0;
@@ -748,7 +751,17 @@ static void sv_launch_level(g5_server* srv, Engine& e) {
     hipLaunchKernelGGL(k_sv_finish_list, dim3(grid_of(e.rows / 8 + 1, kBlock, 2048)), dim3(kBlock), 0, s, e.seen.p,
                        next, nz_next, g.offsets.p, e.touch.p, (const unsigned long long*)(c + kSvTouch), c, epi);
   } else if (bu) {
-    hipLaunchKernelGGL(k_sv_pull, dim3(grid_of(e.words, kBlock / 64, 8192)), dim3(kBlock), 0, s, e.seen.p, front,
+    // a pull of searches that have each pulled twice before finds little to
+    // probe (their frontiers have passed the hubs): its pass over every
+    // vertex runs on a smaller grid, as the batch's light levels do
+    bool light = true;
+    for (uint64_t x = bu; x; x &= x - 1) {
+      const int r = __builtin_ctzll(x);
+      light = light && e.lane_pulls[r] >= 2;
+      ++e.lane_pulls[r];
+    }
+    hipLaunchKernelGGL(k_sv_pull, dim3(grid_of(e.words, kBlock / 64, light ? 2048 : 8192)), dim3(kBlock), 0, s,
+                       e.seen.p, front,
                        nz_front, next, nz_next, g.first_nbr.p, g.offsets.p, g.cols.p, e.rows, bu,
                        srv->cfg.pull_filter, c, keep, td ? 1 : 0, epi);
   } else {
