@@ -32,8 +32,8 @@ SPACE = [
     ("pull_unroll", [0, 1, 2, 3, 4, 5, 6, 7]),
     ("mixed_push", [0, 1]),
     ("lane_rule,alpha", [(0, 1.0), (0, 2.0), (0, 4.0), (0, 8.0), (0, 16.0), (0, 64.0),
-                         (1, 16.0), (1, 24.0), (1, 32.0), (1, 48.0), (1, 64.0), (1, 96.0), (1, 128.0), (1, 192.0),
-                         (1, 256.0)]),
+                         (1, 16.0), (1, 24.0), (1, 32.0), (1, 40.0), (1, 48.0), (1, 56.0), (1, 64.0), (1, 80.0),
+                         (1, 96.0), (1, 128.0), (1, 192.0), (1, 256.0)]),
     ("persistent", [0, 1]),
     ("sweep_below", [0.0, 0.25]),
     ("beta", [0.0, 20.0, 50.0, 200.0]),
